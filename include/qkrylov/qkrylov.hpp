@@ -1,0 +1,338 @@
+// qkrylov.hpp -- C++ drop-in surface of the MI355X engine, mirroring the
+// reference's public C++ names (namespace qkrylov::fp64):
+//   Sector                       include/qkrylov/symmetry/sector.hpp:10-41
+//   Basis + 4 kinds              include/qkrylov/basis/*.hpp
+//   Site + 4 kinds, LocalAction  include/qkrylov/sites/*.hpp, operators/local_action.hpp
+//   OperatorTerm/LocalOp/OpSum   include/qkrylov/operators/{operator_term,opsum,local_op}.hpp
+//   Device                       include/qkrylov/core/device.hpp:27-84
+//   MatrixFreeHamiltonian        include/qkrylov/hamiltonian/matrix_free_hamiltonian.hpp:24-77
+//   lanczos_ground_state, davidson_lowest, continued_fraction_coeffs,
+//   evaluate_spectral_function, ftlm   include/qkrylov/solvers/*.hpp
+// Differences: no Kokkos template parameter (the device is always HIP); the
+// Kokkos View overload of apply() becomes device-pointer overloads; batched
+// entry points (ftlm_batched, continued_fraction_batched) are additions.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace qk {
+class BasisSpec;
+class Engine;
+}
+
+namespace qkrylov {
+namespace fp64 {
+
+using Real = double;
+using Complex = std::complex<double>;
+using StateID = uint64_t;
+using Index = std::size_t;
+using HostVector = std::vector<Complex>;
+
+inline int popcount(uint64_t x) noexcept { return __builtin_popcountll(x); }
+
+struct Sector {
+    bool use_sz = false;
+    int sz2 = 0;
+    bool use_nup = false;
+    bool use_ndn = false;
+    int nup = 0;
+    int ndn = 0;
+    bool use_n = false;
+    int n = 0;
+    bool use_nb = false;
+    int nb = 0;
+};
+
+struct Device {
+    int id = 0;
+    Device() = default;
+    explicit Device(int device_id) : id(device_id) {}
+    explicit Device(const std::string& s);
+    static bool is_gpu_build() { return true; }
+    static std::string backend_name() { return "hip"; }
+    static int gpu_count();
+};
+
+class Basis {
+public:
+    virtual ~Basis();
+    virtual Index size() const;
+    virtual StateID state(Index i) const;     // throws std::out_of_range
+    virtual Index index(StateID s) const;     // throws std::runtime_error if absent
+    virtual bool contains(StateID s) const;
+    int nsites() const noexcept;
+    const Sector& sector() const noexcept { return sector_; }
+    std::shared_ptr<const qk::BasisSpec> spec() const { return spec_; }
+
+protected:
+    Basis(int kind, int N, const Sector& sector);
+    std::shared_ptr<const qk::BasisSpec> spec_;
+    Sector sector_;
+};
+
+class SpinHalfBasis : public Basis {
+public:
+    SpinHalfBasis(int N, const Sector& sector = Sector{});
+};
+class FermionBasis : public Basis {
+public:
+    FermionBasis(int N, const Sector& sector = Sector{});
+};
+class HubbardBasis : public Basis {
+public:
+    HubbardBasis(int N, const Sector& sector = Sector{});
+};
+class TJBasis : public Basis {
+public:
+    TJBasis(int N, const Sector& sector = Sector{});
+};
+
+struct LocalAction {
+    bool valid = false;
+    StateID new_state = 0;
+    Complex matrix_element = 0.0;
+};
+
+class Site {
+public:
+    virtual ~Site() = default;
+    virtual LocalAction apply(const std::string& op, int site, StateID state) const = 0;
+    virtual int kind() const = 0;   // 0 spin-1/2, 1 fermion, 2 Hubbard, 3 t-J
+};
+class SpinHalfSite : public Site {
+public:
+    LocalAction apply(const std::string& op, int site, StateID state) const override;
+    int kind() const override { return 0; }
+};
+class FermionSite : public Site {
+public:
+    LocalAction apply(const std::string& op, int site, StateID state) const override;
+    int kind() const override { return 1; }
+};
+class HubbardSite : public Site {
+public:
+    LocalAction apply(const std::string& op, int site, StateID state) const override;
+    int kind() const override { return 2; }
+};
+class TJSite : public Site {
+public:
+    LocalAction apply(const std::string& op, int site, StateID state) const override;
+    int kind() const override { return 3; }
+};
+
+struct OperatorFactor {
+    std::string op;
+    int site = 0;
+    OperatorFactor() = default;
+    OperatorFactor(std::string op_, int site_) : op(std::move(op_)), site(site_) {}
+};
+
+struct OperatorTerm {
+    Complex coeff = 1.0;
+    std::vector<OperatorFactor> factors;
+    OperatorTerm() = default;
+    OperatorTerm(Complex c, std::initializer_list<OperatorFactor> f) : coeff(c), factors(f) {}
+};
+
+struct LocalOp {
+    std::string name;
+    int site = 0;
+    LocalOp() = default;
+    LocalOp(std::string n, int s) : name(std::move(n)), site(s) {}
+};
+
+inline OperatorTerm operator*(Complex c, const LocalOp& op)
+{
+    OperatorTerm t;
+    t.coeff = c;
+    t.factors.push_back({op.name, op.site});
+    return t;
+}
+inline OperatorTerm operator*(Real c, const LocalOp& op) { return Complex(c, 0.0) * op; }
+inline OperatorTerm operator*(const LocalOp& a, const LocalOp& b)
+{
+    OperatorTerm t;
+    t.coeff = 1.0;
+    t.factors.push_back({a.name, a.site});
+    t.factors.push_back({b.name, b.site});
+    return t;
+}
+inline OperatorTerm operator*(OperatorTerm t, const LocalOp& op)
+{
+    t.factors.push_back({op.name, op.site});
+    return t;
+}
+inline OperatorTerm operator*(OperatorTerm a, const OperatorTerm& b)
+{
+    a.coeff *= b.coeff;
+    a.factors.insert(a.factors.end(), b.factors.begin(), b.factors.end());
+    return a;
+}
+
+struct OpSumExpr {
+    std::vector<OperatorTerm> terms;
+    OpSumExpr(const OperatorTerm& t) : terms({t}) {}
+    OpSumExpr(const std::vector<OperatorTerm>& ts) : terms(ts) {}
+};
+inline OpSumExpr operator+(const OperatorTerm& a, const OperatorTerm& b) { return OpSumExpr({a, b}); }
+inline OpSumExpr operator+(OpSumExpr e, const OperatorTerm& t)
+{
+    e.terms.push_back(t);
+    return e;
+}
+inline OpSumExpr operator+(const OperatorTerm& t, OpSumExpr e)
+{
+    e.terms.insert(e.terms.begin(), t);
+    return e;
+}
+inline OpSumExpr operator+(OpSumExpr a, const OpSumExpr& b)
+{
+    a.terms.insert(a.terms.end(), b.terms.begin(), b.terms.end());
+    return a;
+}
+inline OpSumExpr operator-(const OperatorTerm& a, OperatorTerm b)
+{
+    b.coeff = -b.coeff;
+    return OpSumExpr({a, b});
+}
+inline OpSumExpr operator-(OpSumExpr e, OperatorTerm t)
+{
+    t.coeff = -t.coeff;
+    e.terms.push_back(t);
+    return e;
+}
+
+class OpSum {
+public:
+    void add_term(const OperatorTerm& term) { terms_.push_back(term); }
+    OpSum& operator+=(const OperatorTerm& t)
+    {
+        add_term(t);
+        return *this;
+    }
+    OpSum& operator+=(const OpSumExpr& e)
+    {
+        for (const auto& t : e.terms) add_term(t);
+        return *this;
+    }
+    void clear() { terms_.clear(); }
+    std::size_t size() const { return terms_.size(); }
+    const OperatorTerm& operator[](std::size_t i) const { return terms_[i]; }
+    const std::vector<OperatorTerm>& terms() const { return terms_; }
+
+private:
+    std::vector<OperatorTerm> terms_;
+};
+
+inline LocalOp Sz(int i) { return {"Sz", i}; }
+inline LocalOp Sp(int i) { return {"Sp", i}; }
+inline LocalOp Sm(int i) { return {"Sm", i}; }
+inline LocalOp Sx(int i) { return {"Sx", i}; }
+inline LocalOp Sy(int i) { return {"Sy", i}; }
+inline LocalOp CdagUp(int i) { return {"CdagUp", i}; }
+inline LocalOp CUp(int i) { return {"CUp", i}; }
+inline LocalOp CdagDn(int i) { return {"CdagDn", i}; }
+inline LocalOp CDn(int i) { return {"CDn", i}; }
+inline LocalOp Nup(int i) { return {"Nup", i}; }
+inline LocalOp Ndn(int i) { return {"Ndn", i}; }
+inline LocalOp Nupdn(int i) { return {"Nupdn", i}; }
+inline LocalOp Bdag(int i) { return {"Bdag", i}; }
+inline LocalOp B(int i) { return {"B", i}; }
+inline LocalOp N(int i) { return {"N", i}; }
+
+class MatrixFreeHamiltonian {
+public:
+    // split_bits: number of low state bits ranked per lane (-1: automatic)
+    MatrixFreeHamiltonian(std::shared_ptr<Basis> basis, std::shared_ptr<Site> site, const OpSum& ops,
+                          Device device = Device(), int split_bits = -1);
+    ~MatrixFreeHamiltonian();
+
+    // host pointers: y = H x as the reference (y[a] = sum conj(amp) x[b])
+    void apply(const Complex* x, Complex* y) const;
+    // device pointers, real vectors; nvec > 1: row-major block x[row * nvec + c]
+    void apply_device(const double* x, double* y, int nvec = 1, void* stream = nullptr) const;
+    // device pointers, interleaved complex vectors
+    void apply_device_complex(const double* x, double* y, void* stream = nullptr) const;
+
+    HostVector diagonal_host() const;
+    Index dimension() const;
+    const Device& device() const { return device_; }
+    bool is_real() const;
+    std::shared_ptr<const qk::Engine> engine() const { return engine_; }
+    std::shared_ptr<Basis> basis() const { return basis_; }
+
+private:
+    Device device_;
+    std::shared_ptr<Basis> basis_;
+    std::shared_ptr<Site> site_;
+    OpSum ops_;
+    std::shared_ptr<const qk::Engine> engine_;
+};
+
+struct LanczosResult {
+    Real energy = 0.0;
+    int iterations = 0;
+    bool converged = false;
+    HostVector eigenvector;
+};
+
+struct LanczosOptions {
+    int maxiter = 200;
+    Real tol = 1e-12;
+    bool reorthogonalize = true;    // reference behaviour (lanczos.cpp:149-151)
+    bool want_eigenvector = true;   // the reference always builds the Ritz vector
+    uint64_t seed = 1234;
+};
+
+LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian& H, int maxiter = 200, Real tol = 1.0e-12);
+LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian& H, const LanczosOptions& opt);
+
+struct DavidsonResult {
+    std::vector<Real> eigenvalues;
+    std::vector<HostVector> eigenvectors;
+    int iterations = 0;
+    bool converged = false;
+};
+DavidsonResult davidson_lowest(const MatrixFreeHamiltonian& H, int n_eig = 1, int max_subspace = 20,
+                               Real tol = 1.0e-8);
+
+struct DynamicsResult {
+    std::vector<Real> alphas;
+    std::vector<Real> betas;
+    Real norm_phi0 = 0.0;
+};
+DynamicsResult continued_fraction_coeffs(const MatrixFreeHamiltonian& H, const HostVector& phi0, int n_iter = 100);
+Real evaluate_spectral_function(const Real* alphas, const Real* betas, size_t n, Real norm_phi0, Real omega, Real E0,
+                                Real eta = 0.1);
+
+struct FTLMResult {
+    Real beta;
+    Real partition_function = 0.0;
+    Real internal_energy = 0.0;
+    Real specific_heat = 0.0;
+};
+FTLMResult ftlm(const MatrixFreeHamiltonian& H, Real beta, int n_random = 50, int n_steps = 100);
+
+// ---- additions: batched solves (one multi-vector recurrence for R vectors)
+struct FTLMGridResult {
+    std::vector<Real> betas, partition_function, internal_energy, specific_heat;
+    std::vector<std::vector<Real>> alphas, betas_lanczos;   // per random vector
+    double seconds = 0.0;                                   // recurrence wall time
+};
+// R real Rademacher (+-1) vectors from the counter RNG (seed, column, row);
+// sequential = true runs them one at a time through the plain path.
+FTLMGridResult ftlm_batched(const MatrixFreeHamiltonian& H, int R, int M, uint64_t seed,
+                            const std::vector<Real>& betas, bool sequential = false);
+
+// continued fractions for several seeds at once; seeds are device arrays,
+// complex seeds occupy two real columns (group = 2)
+std::vector<DynamicsResult> continued_fraction_batched(const MatrixFreeHamiltonian& H, const std::vector<HostVector>& seeds,
+                                                       int n_iter, double* seconds = nullptr);
+
+} // namespace fp64
+} // namespace qkrylov

@@ -1,0 +1,152 @@
+/* qkrylov_amd.h -- extensions of the drop-in C ABI (qkrylov/c_api.h) for
+ * the MI355X engine: device memory helpers, device-resident and batched
+ * applies, the full Lanczos driver, batched FTLM / continued fraction,
+ * operator application, diagnostics and the serving loop. None of these has
+ * a counterpart in the reference ABI; they extend it for callers that keep
+ * vectors in HBM (see the per-function notes). All return 0 on success,
+ * -1 for an invalid argument, -2 for a runtime failure (message in
+ * qkrylov_amd_last_error()).
+ */
+#ifndef QKRYLOV_AMD_H
+#define QKRYLOV_AMD_H
+
+#include "qkrylov/c_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ---- */
+const char* qkrylov_amd_last_error(void);
+int qkrylov_amd_device_count(void);
+int qkrylov_amd_set_device(int device_id);
+void* qkrylov_amd_malloc(size_t bytes);                 /* device memory */
+int qkrylov_amd_free(void* ptr);
+int qkrylov_amd_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int qkrylov_amd_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int qkrylov_amd_synchronize(void);
+/* x[i] = uniform[-1,1) (kind 0) or +-1 (kind 1) from the counter RNG
+ * (splitmix64 of seed and index i); on the library stream, synchronous */
+int qkrylov_amd_fill_random(double* x_dev, size_t n, uint64_t seed, int kind);
+
+/* ---- Hamiltonian construction options and info ---- */
+/* NL: number of low state bits ranked per lane (-1: automatic) */
+qkrylov_hamiltonian_h qkrylov_hamiltonian_create_ex(qkrylov_basis_h basis, qkrylov_site_h site,
+                                                    qkrylov_opsum_h opsum, int NL);
+/* Operator O whose apply gives y = O x (the Hamiltonian apply of the
+ * reference computes y = O^dagger x, matrix_free_hamiltonian.cpp:75-84):
+ * built from the adjoint OpSum. Used to form seeds such as S^z_q |psi0>. */
+qkrylov_hamiltonian_h qkrylov_operator_create(qkrylov_basis_h basis, qkrylov_site_h site, qkrylov_opsum_h opsum);
+
+typedef struct {
+    uint64_t dim;
+    int nbits, NL, is_real, n_offdiag_branches, n_diag_terms;
+    uint32_t ntiles;
+    uint64_t table_bytes;
+} qkrylov_hamiltonian_info_t;
+int qkrylov_hamiltonian_info(qkrylov_hamiltonian_h h, qkrylov_hamiltonian_info_t* info);
+
+/* ---- device-resident applies (pointers from qkrylov_amd_malloc or any
+ * device allocation; stream may be NULL for the library stream) ---- */
+/* real vectors; nvec > 1: row-major block x[row * nvec + c] */
+int qkrylov_hamiltonian_apply_device_f64(qkrylov_hamiltonian_h h, const double* x_dev, double* y_dev, int nvec,
+                                         void* stream);
+/* interleaved complex vectors */
+int qkrylov_hamiltonian_apply_device_complex_f64(qkrylov_hamiltonian_h h, const double* x_dev, double* y_dev,
+                                                 void* stream);
+/* diagonal into a device array (complex interleaved when cplx) */
+int qkrylov_hamiltonian_diagonal_device_f64(qkrylov_hamiltonian_h h, double* d_dev, int cplx, void* stream);
+
+/* entries of rows [row0, row0 + nrows) as produced by the device kernels:
+ * diagonal first (col = row), then off-diagonal entries; host buffers
+ * cols[nrows * cap], vals[2 * nrows * cap] (complex), counts[nrows] */
+int qkrylov_hamiltonian_row_entries(qkrylov_hamiltonian_h h, uint64_t row0, uint64_t nrows, int cap, int64_t* cols,
+                                    double* vals, int* counts);
+/* basis states of rows [row0, row0 + n) computed on the device (host out) */
+int qkrylov_hamiltonian_states(qkrylov_hamiltonian_h h, uint64_t row0, uint64_t n, uint64_t* out);
+
+/* timing helper: `reps` applies of y = H x on random device vectors
+ * (nvec columns), timed with HIP events on the library stream; reports the
+ * mean per-apply time of the events bracket (ms) */
+int qkrylov_hamiltonian_time_apply(qkrylov_hamiltonian_h h, int nvec, int warmup, int reps, double* mean_ms);
+
+/* ---- Lanczos driver ---- */
+typedef struct {
+    int maxiter;          /* reference default 200 */
+    double tol;           /* |E_k - E_{k-1}| < tol, reference default 1e-12 */
+    int reorthogonalize;  /* 1 = reference behaviour */
+    int fixed_steps;      /* 1: run min(maxiter, dim) steps, no test */
+    int want_vector;
+    uint64_t seed;        /* start vector uniform[-1,1) from (seed, index) */
+    int check_every;      /* 0: automatic */
+} qkrylov_lanczos_options_t;
+
+typedef struct {
+    double energy;
+    int iterations;
+    int converged;
+    int steps_run;
+    int n_alpha, n_beta;
+    double loop_seconds;
+    double ritz_seconds;
+} qkrylov_lanczos_report_t;
+
+/* start: optional host start vector (real for real H, interleaved complex
+ * otherwise); alphas/betas: host arrays of maxiter entries (optional);
+ * vector: host output (dim doubles for real H, 2*dim otherwise; optional) */
+int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_options_t* opt, const double* start,
+                            double* alphas, double* betas, double* vector, qkrylov_lanczos_report_t* report);
+
+/* ---- batched FTLM: R Rademacher vectors (counter RNG: column r, row i)
+ * run as one multi-vector recurrence of M steps (sequential = 1 runs the
+ * same vectors one at a time); observables for nT inverse temperatures.
+ * alphas/betas_l: R*M host arrays (row r at r*M), nsteps[R]. ---- */
+int qkrylov_ftlm_batched_f64(qkrylov_hamiltonian_h h, int R, int M, uint64_t seed, const double* betas_T, int nT,
+                             int sequential, double* Z, double* E, double* Cv, double* alphas, double* betas_l,
+                             int* nsteps, double* seconds);
+
+/* ---- batched continued fraction: nseeds complex host seeds (each 2*dim
+ * doubles, seed s at s*2*dim); alphas/betas: nseeds*n_iter; norms[nseeds];
+ * n_alpha[nseeds] ---- */
+int qkrylov_cf_batched_f64(qkrylov_hamiltonian_h h, int nseeds, const double* seeds_complex, int n_iter,
+                           double* alphas, double* betas, double* norms, int* n_alpha, double* seconds);
+
+/* ---- serving loop (continuous batching of ground-state and S^zz(omega)
+ * solves of XXZ rings): see qkrylov_amd/csrc/qk_serving.cpp ---- */
+typedef struct {
+    int L;            /* sites */
+    int sz2;          /* 2 Sz */
+    double delta;     /* anisotropy */
+    int kind;         /* 0 ground state, 1 ground state + S^zz continued fraction */
+    double arrival;   /* seconds from trace start */
+    double tol;       /* ground-state tolerance */
+    int cf_steps;     /* continued-fraction steps */
+    int maxiter;      /* Lanczos cap */
+} qkrylov_serve_request_t;
+
+typedef struct {
+    double energy;
+    int iterations;
+    int converged;
+    int n_cf;
+    double cf_norm;
+    double start, finish;   /* seconds from trace start */
+    double latency;         /* finish - arrival */
+} qkrylov_serve_result_t;
+
+typedef struct {
+    size_t hbm_budget_bytes;   /* admission budget (0: 80% of free memory) */
+    int max_active;            /* concurrent requests on the device */
+    int realtime;              /* 1: honour arrival times; 0: all arrive at t=0 */
+} qkrylov_serve_options_t;
+
+/* Runs the trace. cf_alphas/cf_betas: nreq * cf_cap host arrays (optional).
+ * totals: [0] wall seconds, [1] solves/s, [2] p50 latency, [3] p99 latency */
+int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, const qkrylov_serve_options_t* opt,
+                      qkrylov_serve_result_t* results, double* cf_alphas, double* cf_betas, int cf_cap,
+                      double* totals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QKRYLOV_AMD_H */

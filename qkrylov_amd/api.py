@@ -1,0 +1,479 @@
+"""Python mirror of the reference's Python package (bindings/python/qkrylov:
+basis.py, site.py, operators.py, hamiltonian.py, solvers.py) over the
+engine's C ABI. Default dtype is float64 (the reference defaults to
+float32). Device memory and streams are the engine's own; no torch."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+# ---------------------------------------------------------------- operators
+class LocalOpExpr:
+    def __init__(self, name, site):
+        self.name, self.site = name, int(site)
+
+    def __mul__(self, other):
+        if isinstance(other, LocalOpExpr):
+            return TermExpr(1.0, [self, other])
+        if isinstance(other, (int, float, complex)):
+            return TermExpr(complex(other), [self])
+        return NotImplemented
+
+    def __rmul__(self, other):
+        if isinstance(other, (int, float, complex)):
+            return TermExpr(complex(other), [self])
+        return NotImplemented
+
+
+class TermExpr:
+    def __init__(self, coeff, ops):
+        self.coeff, self.ops = complex(coeff), list(ops)
+
+    def __mul__(self, other):
+        if isinstance(other, LocalOpExpr):
+            return TermExpr(self.coeff, self.ops + [other])
+        if isinstance(other, TermExpr):
+            return TermExpr(self.coeff * other.coeff, self.ops + other.ops)
+        if isinstance(other, (int, float, complex)):
+            return TermExpr(self.coeff * complex(other), self.ops)
+        return NotImplemented
+
+    def __rmul__(self, other):
+        if isinstance(other, (int, float, complex)):
+            return TermExpr(self.coeff * complex(other), self.ops)
+        return NotImplemented
+
+    def __add__(self, other):
+        if isinstance(other, TermExpr):
+            return OpSumExpr([self, other])
+        if isinstance(other, OpSumExpr):
+            return OpSumExpr([self] + other.terms)
+        return NotImplemented
+
+    def __sub__(self, other):
+        if isinstance(other, TermExpr):
+            return OpSumExpr([self, TermExpr(-other.coeff, other.ops)])
+        return NotImplemented
+
+
+class OpSumExpr:
+    def __init__(self, terms):
+        self.terms = list(terms)
+
+    def __add__(self, other):
+        if isinstance(other, TermExpr):
+            return OpSumExpr(self.terms + [other])
+        if isinstance(other, OpSumExpr):
+            return OpSumExpr(self.terms + other.terms)
+        return NotImplemented
+
+    def __sub__(self, other):
+        if isinstance(other, TermExpr):
+            return OpSumExpr(self.terms + [TermExpr(-other.coeff, other.ops)])
+        if isinstance(other, OpSumExpr):
+            return OpSumExpr(self.terms + [TermExpr(-t.coeff, t.ops) for t in other.terms])
+        return NotImplemented
+
+    def __mul__(self, other):
+        if isinstance(other, (int, float, complex)):
+            return OpSumExpr([t * other for t in self.terms])
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+
+def _mk(name):
+    def f(i):
+        return LocalOpExpr(name, i)
+    f.__name__ = name
+    return f
+
+
+Sz, Sp, Sm, Sx, Sy = (_mk(n) for n in ("Sz", "Sp", "Sm", "Sx", "Sy"))
+CdagUp, CUp, CdagDn, CDn = (_mk(n) for n in ("CdagUp", "CUp", "CdagDn", "CDn"))
+Nup, Ndn, Nupdn = (_mk(n) for n in ("Nup", "Ndn", "Nupdn"))
+Bdag, B, N = (_mk(n) for n in ("Bdag", "B", "N"))
+Cdag, C = _mk("Cdag"), _mk("C")
+
+
+class OpSum:
+    """Ordered list of terms (coefficient, [(op, site), ...]); the first
+    factor acts first, as in the reference."""
+
+    def __init__(self, terms=None):
+        self.terms = []
+        for t in terms or []:
+            self.add(*t)
+
+    def add(self, coeff, factors):
+        self.terms.append((complex(coeff), [(str(o), int(s)) for o, s in factors]))
+        return self
+
+    def __iadd__(self, other):
+        if isinstance(other, TermExpr):
+            other = OpSumExpr([other])
+        if isinstance(other, OpSumExpr):
+            for t in other.terms:
+                self.add(t.coeff, [(o.name, o.site) for o in t.ops])
+            return self
+        if isinstance(other, OpSum):
+            self.terms += other.terms
+            return self
+        return NotImplemented
+
+    def __len__(self):
+        return len(self.terms)
+
+    def _handle(self):
+        L = _lib.load()
+        h = L.qkrylov_opsum_create()
+        for c, fs in self.terms:
+            ops = (ctypes.c_char_p * len(fs))(*[f[0].encode() for f in fs])
+            sites = (ctypes.c_int * len(fs))(*[f[1] for f in fs])
+            check(L.qkrylov_opsum_add_term_nbody_f64(h, c.real, c.imag, len(fs), ops, sites), "add_term")
+        return h
+
+
+# ---------------------------------------------------------------- sites
+class Site:
+    _create = None
+
+    def __init__(self):
+        self._h = getattr(_lib.load(), self._create)()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().qkrylov_site_destroy(self._h)
+            self._h = None
+
+
+class SpinHalfSite(Site):
+    _create = "qkrylov_spinhalf_site_create"
+
+
+class FermionSite(Site):
+    _create = "qkrylov_fermion_site_create"
+
+
+class HubbardSite(Site):
+    _create = "qkrylov_hubbard_site_create"
+
+
+class TJSite(Site):
+    _create = "qkrylov_tj_site_create"
+
+
+# ---------------------------------------------------------------- bases
+class Basis:
+    _create = None
+    site_class = None
+
+    def _build(self, N, sz2=None, n=None, nup=None, ndn=None):
+        L = _lib.load()
+        sec = L.qkrylov_sector_create()
+        if sz2 is not None:
+            L.qkrylov_sector_set_sz(sec, int(sz2))
+        if n is not None:
+            L.qkrylov_sector_set_n(sec, int(n))
+        if nup is not None or ndn is not None:
+            L.qkrylov_sector_set_hubbard_particles(sec, int(nup or 0), int(ndn or 0))
+        self._h = getattr(L, self._create)(int(N), sec)
+        L.qkrylov_sector_destroy(sec)
+        if not self._h:
+            raise ValueError(f"basis creation failed: {_lib.last_error()}")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().qkrylov_basis_destroy(self._h)
+            self._h = None
+
+    @property
+    def size(self):
+        return int(_lib.load().qkrylov_basis_dimension(self._h))
+
+    @property
+    def nsites(self):
+        return int(_lib.load().qkrylov_basis_nsites(self._h))
+
+    def state(self, i):
+        return int(_lib.load().qkrylov_basis_state(self._h, int(i)))
+
+    def index(self, s):
+        return int(_lib.load().qkrylov_basis_index(self._h, int(s)))
+
+    def contains(self, s):
+        return bool(_lib.load().qkrylov_basis_contains(self._h, int(s)))
+
+
+class SpinHalfBasis(Basis):
+    _create = "qkrylov_spinhalf_basis_create"
+    site_class = SpinHalfSite
+
+    def __init__(self, N, conserve_sz=False, sz=None, sz2=None):
+        if sz2 is None and (sz is not None or conserve_sz):
+            sz2 = int(2 * (sz or 0))
+        self._build(N, sz2=sz2)
+
+
+class FermionBasis(Basis):
+    _create = "qkrylov_fermion_basis_create"
+    site_class = FermionSite
+
+    def __init__(self, N, conserve_n=False, n=0):
+        self._build(N, n=n if (conserve_n or n) else None)
+
+
+class HubbardBasis(Basis):
+    _create = "qkrylov_hubbard_basis_create"
+    site_class = HubbardSite
+
+    def __init__(self, N, conserve_nup=False, nup=0, conserve_ndn=False, ndn=0):
+        fix = conserve_nup or conserve_ndn or nup or ndn
+        self._build(N, nup=nup if fix else None, ndn=ndn if fix else None)
+
+
+class TJBasis(HubbardBasis):
+    _create = "qkrylov_tj_basis_create"
+    site_class = TJSite
+
+
+# ---------------------------------------------------------------- Hamiltonian
+def _dp(a):
+    return a.ctypes.data_as(_lib.p_dbl)
+
+
+class MatrixFreeHamiltonian:
+    """y = H x with the reference's conventions (row a sums
+    conj(<b|term|a>) x[b]); FP64 on the device."""
+
+    def __init__(self, basis, site=None, ops=None, split_bits=-1, operator=False):
+        if ops is None and isinstance(site, (OpSum, OpSumExpr, TermExpr)):
+            site, ops = None, site
+        if site is None:
+            site = basis.site_class()
+        if not isinstance(ops, OpSum):
+            o = OpSum()
+            o += ops
+            ops = o
+        L = _lib.load()
+        self.basis, self.site, self.ops = basis, site, ops
+        oh = ops._handle()
+        try:
+            if operator:
+                self._h = L.qkrylov_operator_create(basis._h, site._h, oh)
+            else:
+                self._h = L.qkrylov_hamiltonian_create_ex(basis._h, site._h, oh, int(split_bits))
+        finally:
+            L.qkrylov_opsum_destroy(oh)
+        if not self._h:
+            raise ValueError(f"Hamiltonian creation failed: {_lib.last_error()}")
+        info = _lib.HamInfo()
+        check(L.qkrylov_hamiltonian_info(self._h, ctypes.byref(info)), "info")
+        self.info = {f: getattr(info, f) for f, _ in info._fields_}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().qkrylov_hamiltonian_destroy(self._h)
+            self._h = None
+
+    @property
+    def dimension(self):
+        return int(self.info["dim"])
+
+    @property
+    def is_real(self):
+        return bool(self.info["is_real"])
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.dimension, dtype=np.complex128)
+        check(_lib.load().qkrylov_hamiltonian_apply_complex_f64(self._h, _dp(x.view(np.float64)),
+                                                               _dp(y.view(np.float64))), "apply")
+        return y
+
+    __matmul__ = apply
+
+    def diagonal(self):
+        d = np.zeros(self.dimension)
+        check(_lib.load().qkrylov_hamiltonian_diagonal_f64(self._h, _dp(d)), "diagonal")
+        return d
+
+    def apply_device(self, x_ptr, y_ptr, nvec=1, stream=None):
+        check(_lib.load().qkrylov_hamiltonian_apply_device_f64(self._h, x_ptr, y_ptr, nvec, stream), "apply_device")
+
+    def apply_device_complex(self, x_ptr, y_ptr, stream=None):
+        check(_lib.load().qkrylov_hamiltonian_apply_device_complex_f64(self._h, x_ptr, y_ptr, stream),
+              "apply_device_complex")
+
+    def row_entries(self, row0, nrows, cap=None):
+        cap = cap or (self.info["n_offdiag_branches"] + 1)
+        cols = np.zeros(nrows * cap, dtype=np.int64)
+        vals = np.zeros(2 * nrows * cap)
+        cnt = np.zeros(nrows, dtype=np.int32)
+        check(_lib.load().qkrylov_hamiltonian_row_entries(self._h, row0, nrows, cap, cols.ctypes.data_as(_lib.p_i64),
+                                                          _dp(vals), cnt.ctypes.data_as(_lib.p_int)), "row_entries")
+        vals = vals.view(np.complex128).reshape(nrows, cap)
+        cols = cols.reshape(nrows, cap)
+        return [(cols[i, :cnt[i]].copy(), vals[i, :cnt[i]].copy()) for i in range(nrows)]
+
+    def states(self, row0, n):
+        out = np.zeros(n, dtype=np.uint64)
+        check(_lib.load().qkrylov_hamiltonian_states(self._h, row0, n, out.ctypes.data_as(_lib.p_u64)), "states")
+        return out
+
+    def time_apply(self, nvec=1, warmup=2, reps=10):
+        ms = ctypes.c_double()
+        check(_lib.load().qkrylov_hamiltonian_time_apply(self._h, nvec, warmup, reps, ctypes.byref(ms)), "time_apply")
+        return ms.value
+
+    def aslinearoperator(self):
+        import scipy.sparse.linalg as sla
+        return sla.LinearOperator((self.dimension, self.dimension), matvec=self.apply, dtype=np.complex128)
+
+    def __repr__(self):
+        return f"MatrixFreeHamiltonian(dim={self.dimension})"
+
+
+# ---------------------------------------------------------------- solvers
+class LanczosResult:
+    def __init__(self, energy, iterations, converged, eigenvector=None, alphas=None, betas=None, report=None):
+        self.energy, self.iterations, self.converged = energy, iterations, converged
+        self.eigenvector, self.alphas, self.betas, self.report = eigenvector, alphas, betas, report
+
+    def __repr__(self):
+        return f"LanczosResult(energy={self.energy:.12f}, iterations={self.iterations}, converged={self.converged})"
+
+
+def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return_state=False, seed=1234,
+                         fixed_steps=False, start=None, check_every=0):
+    L = _lib.load()
+    o = _lib.LanczosOptions(int(maxiter), float(tol), 1 if reorthogonalize else 0, 1 if fixed_steps else 0,
+                            1 if return_state else 0, int(seed), int(check_every))
+    al = np.zeros(max(maxiter, 1))
+    be = np.zeros(max(maxiter, 1))
+    n = H.dimension if H.is_real else 2 * H.dimension
+    vec = np.zeros(n) if return_state else None
+    st = None
+    if start is not None:
+        st = np.ascontiguousarray(start, dtype=np.float64 if H.is_real else np.complex128).view(np.float64)
+    rep = _lib.LanczosReport()
+    check(L.qkrylov_lanczos_run_f64(H._h, ctypes.byref(o), _dp(st) if st is not None else None, _dp(al), _dp(be),
+                                     _dp(vec) if vec is not None else None, ctypes.byref(rep)), "lanczos")
+    if vec is not None and not H.is_real:
+        vec = vec.view(np.complex128)
+    return LanczosResult(rep.energy, rep.iterations, bool(rep.converged), vec, al[:rep.n_alpha].copy(),
+                         be[:rep.n_beta].copy(), {f: getattr(rep, f) for f, _ in rep._fields_})
+
+
+class DavidsonResult:
+    def __init__(self, eigenvalues, eigenvectors, iterations, converged):
+        self.eigenvalues, self.eigenvectors = eigenvalues, eigenvectors
+        self.iterations, self.converged = iterations, converged
+
+
+def davidson_lowest(H, n_eig=1, max_subspace=20, tol=1e-8):
+    ev = np.zeros(n_eig)
+    vecs = np.zeros((n_eig, 2 * H.dimension))
+    info = _lib.DavidsonResultF64()
+    check(_lib.load().qkrylov_davidson_lowest_complex_f64(H._h, n_eig, max_subspace, tol, _dp(ev), _dp(vecs),
+                                                         ctypes.byref(info)), "davidson")
+    return DavidsonResult(ev, [v.view(np.complex128) for v in vecs], info.iterations, bool(info.converged))
+
+
+class DynamicsResult:
+    def __init__(self, alphas, betas, norm_phi0):
+        self.alphas, self.betas, self.norm_phi0 = np.asarray(alphas), np.asarray(betas), norm_phi0
+
+
+def continued_fraction_coeffs(H, phi0, n_iter=100):
+    phi = np.ascontiguousarray(phi0, dtype=np.complex128)
+    al, be = np.zeros(n_iter), np.zeros(n_iter)
+    nrm = ctypes.c_double()
+    na = ctypes.c_int()
+    check(_lib.load().qkrylov_continued_fraction_coeffs_complex_f64(H._h, _dp(phi.view(np.float64)), n_iter,
+                                                                   _dp(al), _dp(be), ctypes.byref(nrm),
+                                                                   ctypes.byref(na)), "continued_fraction")
+    n = na.value
+    return DynamicsResult(al[:n].copy(), be[:max(n - 1, 0)].copy() if n else be[:0], nrm.value)
+
+
+def evaluate_spectral_function(res, omega, E0, eta=0.1):
+    a = np.ascontiguousarray(res.alphas, dtype=np.float64)
+    b = np.ascontiguousarray(res.betas if len(res.betas) else [0.0], dtype=np.float64)
+    return _lib.load().qkrylov_evaluate_spectral_function_f64(_dp(a), _dp(b), len(a), res.norm_phi0, omega, E0, eta)
+
+
+class FTLMResult:
+    def __init__(self, beta, Z, E, Cv):
+        self.beta, self.partition_function, self.internal_energy, self.specific_heat = beta, Z, E, Cv
+
+
+def ftlm(H, beta, n_random=50, n_steps=100):
+    r = _lib.FTLMResultF64()
+    check(_lib.load().qkrylov_ftlm_f64(H._h, beta, n_random, n_steps, ctypes.byref(r)), "ftlm")
+    return FTLMResult(r.beta, r.partition_function, r.internal_energy, r.specific_heat)
+
+
+def ftlm_batched(H, R, M, betas, seed=42, sequential=False):
+    """R Rademacher vectors (counter RNG) as one multi-vector recurrence;
+    observables for every inverse temperature in `betas`."""
+    bT = np.ascontiguousarray(betas, dtype=np.float64)
+    nT = len(bT)
+    Z, E, Cv = np.zeros(nT), np.zeros(nT), np.zeros(nT)
+    al, be = np.zeros(R * M), np.zeros(R * M)
+    ns = np.zeros(R, dtype=np.int32)
+    sec = ctypes.c_double()
+    check(_lib.load().qkrylov_ftlm_batched_f64(H._h, R, M, seed, _dp(bT), nT, 1 if sequential else 0, _dp(Z), _dp(E),
+                                              _dp(Cv), _dp(al), _dp(be), ns.ctypes.data_as(_lib.p_int),
+                                              ctypes.byref(sec)), "ftlm_batched")
+    al, be = al.reshape(R, M), be.reshape(R, M)
+    return dict(Z=Z, E=E, Cv=Cv, alphas=[al[r, :ns[r]].copy() for r in range(R)],
+                betas=[be[r, :max(ns[r] - 1, 0)].copy() for r in range(R)], nsteps=ns, seconds=sec.value)
+
+
+def continued_fraction_batched(H, seeds, n_iter=100):
+    seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.complex128))
+    ns = seeds.shape[0]
+    al, be = np.zeros(ns * n_iter), np.zeros(ns * n_iter)
+    nrm = np.zeros(ns)
+    na = np.zeros(ns, dtype=np.int32)
+    sec = ctypes.c_double()
+    check(_lib.load().qkrylov_cf_batched_f64(H._h, ns, _dp(seeds.view(np.float64)), n_iter, _dp(al), _dp(be),
+                                            _dp(nrm), na.ctypes.data_as(_lib.p_int), ctypes.byref(sec)), "cf_batched")
+    al, be = al.reshape(ns, n_iter), be.reshape(ns, n_iter)
+    out = [DynamicsResult(al[s, :na[s]].copy(), be[s, :max(na[s] - 1, 0)].copy(), nrm[s]) for s in range(ns)]
+    return out, sec.value
+
+
+# ---------------------------------------------------------------- device memory
+class DeviceArray:
+    """Minimal device buffer of float64 owned by the engine's allocator."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = _lib.load().qkrylov_amd_malloc(max(self.n, 1) * 8)
+        if not self.ptr:
+            raise MemoryError(_lib.last_error())
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        d = cls(a.size)
+        check(_lib.load().qkrylov_amd_memcpy_h2d(d.ptr, a.ctypes.data, a.size * 8), "h2d")
+        return d
+
+    def to_host(self):
+        out = np.zeros(self.n)
+        check(_lib.load().qkrylov_amd_synchronize(), "sync")
+        check(_lib.load().qkrylov_amd_memcpy_d2h(out.ctypes.data, self.ptr, self.n * 8), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _lib.load().qkrylov_amd_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()

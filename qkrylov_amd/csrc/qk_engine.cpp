@@ -1,0 +1,878 @@
+// qk_engine.cpp -- Engine (device tables), tridiagonal QR, and the Krylov
+// drivers: Lanczos ground state, plain three-term recurrences (FTLM / CF),
+// the block recurrence for batched FTLM / CF, and Davidson.
+#include "qk_engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+namespace qk {
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static void lcheck(int rc, const char* what)
+{
+    if (rc != 0) throw std::runtime_error(std::string("kernel launch failed: ") + what + " (" + std::to_string(rc) + ")");
+}
+
+DevBuf::DevBuf(size_t bytes) : bytes_(bytes)
+{
+    if (bytes == 0) return;
+    hipError_t e = hipMalloc(&p_, bytes);
+    if (e != hipSuccess) {
+        p_ = nullptr;
+        throw std::runtime_error("device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                 hipGetErrorString(e));
+    }
+}
+
+DevBuf::~DevBuf() { reset(); }
+
+void DevBuf::reset()
+{
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    bytes_ = 0;
+}
+
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept
+{
+    if (this != &o) {
+        reset();
+        p_ = o.p_;
+        bytes_ = o.bytes_;
+        o.p_ = nullptr;
+        o.bytes_ = 0;
+    }
+    return *this;
+}
+
+hipStream_t default_stream()
+{
+    struct Holder {
+        hipStream_t s = nullptr;
+        ~Holder() { if (s) (void)hipStreamDestroy(s); }
+    };
+    thread_local Holder h;
+    if (!h.s) hip_check(hipStreamCreateWithFlags(&h.s, hipStreamNonBlocking), "hipStreamCreate");
+    return h.s;
+}
+
+void ensure_device(int id)
+{
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (n <= 0) throw std::runtime_error("no HIP device available");
+    if (id < 0 || id >= n) throw std::invalid_argument("device id out of range");
+    hip_check(hipSetDevice(id), "hipSetDevice");
+}
+
+template <class T>
+static DevBuf upload(const std::vector<T>& v)
+{
+    DevBuf b(std::max<size_t>(v.size() * sizeof(T), 8));
+    if (!v.empty()) hip_check(hipMemcpy(b.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    return b;
+}
+
+// ---------------------------------------------------------------- Engine
+
+Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpSum& ops, EngineOptions opt)
+    : basis_(std::move(basis))
+{
+    if (basis_->dim() > 0) comp_ = compile_opsum(*basis_, site_kind, ops);
+    const int NL = opt.NL >= 0 ? opt.NL : choose_NL(*basis_);
+    host_ = build_tables(*basis_, NL);
+    tiles_ = upload(host_.tiles);
+    hoff_ = upload(host_.hoff);
+    lrank_ = upload(host_.lrank);
+    lolist_ = upload(host_.lolist);
+    br_ = upload(comp_.offdiag);
+    dg_ = upload(comp_.diag);
+    T_.tiles = static_cast<const QkTile*>(tiles_.get());
+    T_.hoff = static_cast<const uint32_t*>(hoff_.get());
+    T_.lrank = static_cast<const uint32_t*>(lrank_.get());
+    T_.lolist = static_cast<const uint32_t*>(lolist_.get());
+    T_.br = static_cast<const QkBranch*>(br_.get());
+    T_.dg = static_cast<const QkDiagTerm*>(dg_.get());
+    T_.lomask = (host_.NL >= 64) ? ~0ULL : ((1ULL << host_.NL) - 1ULL);
+    T_.dim = static_cast<uint64_t>(basis_->dim());
+    T_.dconst_re = comp_.dconst.real();
+    T_.dconst_im = comp_.dconst.imag();
+    T_.ntiles = static_cast<uint32_t>(host_.tiles.size());
+    T_.nbr = static_cast<int32_t>(comp_.offdiag.size());
+    T_.ndg = static_cast<int32_t>(comp_.diag.size());
+    T_.NL = host_.NL;
+    T_.lrank_u16 = host_.lrank_u16 ? 1 : 0;
+    T_.nlo = 1 << host_.NL;
+    table_bytes_ = tiles_.bytes() + hoff_.bytes() + lrank_.bytes() + lolist_.bytes() + br_.bytes() + dg_.bytes();
+}
+
+void Engine::apply(const double* x, double* y, bool cplx, hipStream_t s) const
+{
+    if (!cplx && !is_real()) throw std::invalid_argument("complex Hamiltonian needs complex vectors");
+    lcheck(qkl_apply(&T_, cplx ? 1 : 0, wide() ? 1 : 0, x, y, -1, nullptr, nullptr, nullptr, s), "apply");
+}
+
+void Engine::apply_block(const double* X, double* Y, int ld, int ncols, hipStream_t s) const
+{
+    if (!is_real()) throw std::invalid_argument("block apply needs a real Hamiltonian");
+    for (int c0 = 0; c0 < ncols; c0 += 64) {
+        const int nc = std::min(64, ncols - c0);
+        lcheck(qkl_apply_block(&T_, wide() ? 1 : 0, X, Y, ld, c0, nc, -1, nullptr, nullptr, 0, nullptr, 0, s),
+               "apply_block");
+    }
+}
+
+void Engine::diagonal(double* d, bool cplx, hipStream_t s) const
+{
+    lcheck(qkl_diagonal(&T_, cplx ? 1 : 0, wide() ? 1 : 0, d, s), "diagonal");
+}
+
+void Engine::apply_diag_operator(const double* x, bool xcplx, double* y, hipStream_t s) const
+{
+    if (!comp_.diag_only) throw std::invalid_argument("operator is not diagonal");
+    lcheck(qkl_apply_diag_terms(&T_, wide() ? 1 : 0, xcplx ? 1 : 0, x, y, s), "apply_diag_operator");
+}
+
+// ---------------------------------------------------------------- tridiagonal QR
+
+void tridiag_qr(const double* alpha, const double* beta, int n, double* d, double* z)
+{
+    std::vector<double> e(std::max(n, 1), 0.0);
+    for (int i = 0; i < n; ++i) d[i] = alpha[i];
+    for (int i = 0; i + 1 < n; ++i) e[i] = beta[i];
+    if (z) {
+        std::fill(z, z + (size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
+    }
+    for (int sweep = 0; sweep < 1000; ++sweep) {
+        for (int i = 0; i + 1 < n; ++i)
+            if (std::abs(e[i]) < 1e-14 * (std::abs(d[i]) + std::abs(d[i + 1]))) e[i] = 0.0;
+        int m = n - 1;
+        while (m > 0 && e[m - 1] == 0.0) --m;
+        if (m == 0) break;
+        int l = m - 1;
+        while (l > 0 && e[l - 1] != 0.0) --l;
+        // Wilkinson shift from the trailing 2x2 block
+        const double b = (d[m - 1] - d[m]) / 2.0;
+        const double c = e[m - 1] * e[m - 1];
+        const double r0 = std::sqrt(b * b + c);
+        const double mu = (b > 0) ? d[m] - c / (b + r0) : d[m] - c / (b - r0);
+        double p = d[l] - mu, g = e[l];
+        for (int i = l; i < m; ++i) {
+            const double r = std::hypot(p, g);
+            const double cs = p / r, sn = g / r;
+            if (i > l) e[i - 1] = r;
+            const double f = cs * d[i] + sn * e[i];
+            const double gn = cs * e[i] + sn * d[i + 1];
+            const double h = sn * d[i] - cs * e[i];
+            const double k = sn * e[i] - cs * d[i + 1];
+            d[i] = cs * f + sn * gn;
+            e[i] = cs * h + sn * k;
+            d[i + 1] = sn * h - cs * k;
+            if (z) {
+                for (int j = 0; j < n; ++j) {
+                    const double z1 = z[(size_t)j * n + i], z2 = z[(size_t)j * n + i + 1];
+                    z[(size_t)j * n + i] = cs * z1 + sn * z2;
+                    z[(size_t)j * n + i + 1] = sn * z1 - cs * z2;
+                }
+            }
+            if (i < m - 1) {
+                p = e[i];
+                g = sn * e[i + 1];
+                e[i + 1] = -cs * e[i + 1];
+            }
+        }
+    }
+}
+
+double tridiag_min(const double* alpha, const double* beta, int n, std::vector<double>* evec)
+{
+    if (n == 0) {
+        if (evec) evec->clear();
+        return 0.0;
+    }
+    if (n == 1) {
+        if (evec) evec->assign(1, 1.0);
+        return alpha[0];
+    }
+    std::vector<double> d(n);
+    std::vector<double> z;
+    if (evec) z.resize((size_t)n * n);
+    tridiag_qr(alpha, beta, n, d.data(), evec ? z.data() : nullptr);
+    int mi = 0;
+    for (int i = 1; i < n; ++i)
+        if (d[i] < d[mi]) mi = i;
+    if (evec) {
+        evec->resize(n);
+        for (int i = 0; i < n; ++i) (*evec)[i] = z[(size_t)i * n + mi];
+    }
+    return d[mi];
+}
+
+// ---------------------------------------------------------------- helpers
+
+namespace {
+
+struct Scratch {
+    DevBuf partA, partB, scal;
+};
+
+double host_scalar(const double* dev, int idx, hipStream_t s)
+{
+    double v;
+    hip_check(hipMemcpyAsync(&v, dev + idx, sizeof(double), hipMemcpyDeviceToHost, s), "d2h scalar");
+    hip_check(hipStreamSynchronize(s), "sync");
+    return v;
+}
+
+// <x, y> (complex conj-linear when cplx) -> host
+std::complex<double> dot_host(bool cplx, size_t n, const double* x, const double* y, double* part, double* out,
+                              hipStream_t s)
+{
+    lcheck(qkl_dot_partials(cplx ? 1 : 0, n, x, y, part, s), "dot");
+    lcheck(qkl_reduce(part, qkl_blas_partials(), 1, 1, cplx ? 2 : 0, cplx ? 1 : 0, out, 0, 2, s), "reduce");
+    double v[2] = {0, 0};
+    hip_check(hipMemcpyAsync(v, out, (cplx ? 2 : 1) * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+    hip_check(hipStreamSynchronize(s), "sync");
+    return {v[0], cplx ? v[1] : 0.0};
+}
+
+// reference replay of the convergence logic of lanczos_ground_state
+// (src/solvers/lanczos.cpp:135-183)
+struct Replay {
+    int kmax;
+    double tol;
+    bool fixed;
+    std::vector<double> alphas, betas;
+    double e_old = 1e100;
+    int next = 0;
+    bool done = false, converged = false;
+
+    // feed step `it` with alpha_it and beta_it (norm after step it)
+    void feed(double a, double b)
+    {
+        const int it = next++;
+        alphas.push_back(a);
+        if (b < 1e-15) {
+            converged = true;
+            done = true;
+            return;
+        }
+        if (it + 1 == kmax) {
+            done = true;
+            return;
+        }
+        betas.push_back(b);
+        if (!fixed && it > 0) {
+            const double e = tridiag_min(alphas.data(), betas.data(), (int)alphas.size(), nullptr);
+            if (std::abs(e - e_old) < tol) {
+                e_old = e;
+                converged = true;
+                done = true;
+                return;
+            }
+            e_old = e;
+        }
+    }
+};
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// device state of the fused real recurrence
+struct RealCore {
+    const Engine& E;
+    hipStream_t s;
+    size_t D;
+    int kcap;
+    DevBuf Q[3];
+    DevBuf alpha, nrm, hcoef, partA, partB, partM;
+    int npA, NB;
+    int ip = 0, ic = 1, iw = 2;
+
+    RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth)
+        : E(e), s(st), D((size_t)e.dim()), kcap(kcap_)
+    {
+        for (auto& q : Q) q = DevBuf(D * sizeof(double));
+        alpha = DevBuf((kcap + 2) * sizeof(double));
+        nrm = DevBuf((kcap + 3) * sizeof(double));
+        npA = qkl_apply_partials(E.tables());
+        NB = qkl_blas_partials();
+        partA = DevBuf(npA * sizeof(double));
+        partB = DevBuf(NB * sizeof(double));
+        if (reorth) {
+            hcoef = DevBuf((kcap + 2) * sizeof(double));
+            partM = DevBuf((size_t)NB * (kcap + 2) * sizeof(double));
+        }
+    }
+
+    double* cur() { return Q[ic].d(); }
+    double* prv() { return Q[ip].d(); }
+    double* wrk() { return Q[iw].d(); }
+
+    void init_norm()
+    {
+        lcheck(qkl_fill_zero(D, prv(), s), "zero");
+        lcheck(qkl_dot_partials(0, D, cur(), cur(), partB.d(), s), "dot");
+        lcheck(qkl_reduce(partB.d(), NB, 1, 1, 1, 0, nrm.d(), 0, 1, s), "reduce");
+    }
+
+    // one step k: w = H v_k - beta v_{k-1}; alpha_k; q_{k+1}; nrm[k+1]
+    void step(int k, const double* basis)
+    {
+        lcheck(qkl_apply(E.tables(), 0, E.wide() ? 1 : 0, cur(), wrk(), k, prv(), nrm.d(), partA.d(), s), "apply");
+        lcheck(qkl_reduce(partA.d(), npA, 1, 1, 0, 0, alpha.d(), k, 1, s), "reduce alpha");
+        lcheck(qkl_lanczos_update(0, D, 1, 1, wrk(), cur(), alpha.d(), nrm.d(), k, 1, partB.d(), s), "update");
+        if (basis) {
+            // classical Gram-Schmidt against v_0..v_k (stored normalised)
+            lcheck(qkl_multidot(D, k + 1, basis, D, wrk(), partM.d(), s), "multidot");
+            lcheck(qkl_reduce(partM.d(), NB, k + 1, 1, 0, 0, hcoef.d(), 0, 0, s), "reduce h");
+            lcheck(qkl_multiaxpy(D, k + 1, basis, D, hcoef.d(), -1.0, wrk(), s), "multiaxpy");
+            lcheck(qkl_dot_partials(0, D, wrk(), wrk(), partB.d(), s), "dot");
+        }
+        lcheck(qkl_reduce(partB.d(), NB, 1, 1, 1, 0, nrm.d(), k + 1, 1, s), "reduce beta");
+        const int t = ip;
+        ip = ic;
+        ic = iw;
+        iw = t;
+    }
+};
+
+} // namespace
+
+// ---------------------------------------------------------------- Lanczos (real, fused)
+
+static LanczosOutput lanczos_complex(const Engine& E, const LanczosOptions& o, hipStream_t s);
+
+LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t s)
+{
+    LanczosOutput out;
+    const int64_t D = E.dim();
+    if (D == 0) return out;
+    if (!E.is_real()) return lanczos_complex(E, o, s);
+    const int kmax = (int)std::min<int64_t>(o.maxiter, D);
+    if (kmax <= 0) return out;
+    const bool reorth = o.reorth;
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t basis_bytes = (size_t)(kmax + 1) * (size_t)D * sizeof(double);
+    const bool store = reorth || (o.want_vector && basis_bytes < free_b / 3);
+    if (reorth && basis_bytes > (free_b / 10) * 9)
+        throw std::runtime_error("full reorthogonalisation needs more device memory than is free");
+    RealCore C(E, kmax, s, reorth);
+    DevBuf basis;
+    if (store) basis = DevBuf(basis_bytes);
+
+    auto load_start = [&](double* dst) {
+        if (o.start_dev) hip_check(hipMemcpyAsync(dst, o.start_dev, D * sizeof(double), hipMemcpyDeviceToDevice, s), "start");
+        else if (o.start) hip_check(hipMemcpyAsync(dst, o.start, D * sizeof(double), hipMemcpyHostToDevice, s), "start");
+        else lcheck(qkl_fill_random(0, (size_t)D, o.seed, 0, 1, 1, 0, dst, s), "random start");
+    };
+    load_start(C.cur());
+    C.init_norm();
+    if (store) lcheck(qkl_copy_scaled((size_t)D, C.cur(), basis.d(), C.nrm.d(), 0, s), "store v0");
+
+    Replay rp{kmax, o.tol, o.fixed_steps, {}, {}};
+    int check = o.check_every > 0 ? o.check_every : (D < (1 << 22) ? 8 : 1);
+    if (o.fixed_steps) check = kmax;
+    std::vector<double> ha(kmax + 2), hn(kmax + 3);
+    const double t0 = now_s();
+    int k = 0;
+    while (!rp.done) {
+        const int kend = std::min(kmax, k + check);
+        for (; k < kend; ++k) {
+            C.step(k, reorth ? basis.d() : nullptr);
+            if (store)
+                lcheck(qkl_copy_scaled((size_t)D, C.cur(), basis.d() + (size_t)(k + 1) * D, C.nrm.d(), k + 1, s),
+                       "store v");
+        }
+        hip_check(hipMemcpyAsync(ha.data(), C.alpha.d(), k * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        hip_check(hipMemcpyAsync(hn.data(), C.nrm.d(), (k + 1) * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        hip_check(hipStreamSynchronize(s), "sync");
+        while (!rp.done && rp.next < k) rp.feed(ha[rp.next], hn[rp.next + 1]);
+        if (k >= kmax) rp.done = true;
+    }
+    out.loop_seconds = now_s() - t0;
+    out.steps_run = k;
+    const int n = (int)rp.alphas.size();
+    out.iterations = rp.next;
+    out.converged = rp.converged;
+    out.alphas = rp.alphas;
+    out.betas = rp.betas;
+    std::vector<double> z;
+    out.energy = tridiag_min(rp.alphas.data(), rp.betas.data(), n, o.want_vector ? &z : nullptr);
+
+    if (o.want_vector) {
+        const double t1 = now_s();
+        DevBuf ritz(D * sizeof(double));
+        lcheck(qkl_fill_zero((size_t)D, ritz.d(), s), "zero");
+        if (store) {
+            DevBuf zc(n * sizeof(double));
+            hip_check(hipMemcpyAsync(zc.get(), z.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+            lcheck(qkl_multiaxpy((size_t)D, n, basis.d(), (size_t)D, zc.d(), 1.0, ritz.d(), s), "ritz");
+            hip_check(hipStreamSynchronize(s), "sync");
+        } else {
+            // second pass: rerun the same deterministic recurrence and
+            // accumulate sum_k z_k v_k (v_k = q_k / nrm_k)
+            RealCore R(E, n, s, false);
+            load_start(R.cur());
+            R.init_norm();
+            for (int kk = 0; kk < n; ++kk) {
+                lcheck(qkl_axpy(0, (size_t)D, z[kk] / hn[kk], 0.0, R.cur(), ritz.d(), s), "axpy");
+                if (kk == n - 1) break;
+                R.step(kk, nullptr);
+            }
+        }
+        DevBuf tmp(2 * sizeof(double));
+        DevBuf part(qkl_blas_partials() * 2 * sizeof(double));
+        const double nr = std::sqrt(dot_host(false, (size_t)D, ritz.d(), ritz.d(), part.d(), tmp.d(), s).real());
+        if (nr != 0.0) lcheck(qkl_scal(0, (size_t)D, 1.0 / nr, 0.0, ritz.d(), s), "scal");
+        out.vector.resize(D);
+        hip_check(hipMemcpyAsync(out.vector.data(), ritz.get(), D * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        hip_check(hipStreamSynchronize(s), "sync");
+        out.ritz_seconds = now_s() - t1;
+    }
+    return out;
+}
+
+// complex Hamiltonian: reference-shaped loop with host scalars
+static LanczosOutput lanczos_complex(const Engine& E, const LanczosOptions& o, hipStream_t s)
+{
+    LanczosOutput out;
+    const size_t D = (size_t)E.dim();
+    const int kmax = (int)std::min<int64_t>(o.maxiter, (int64_t)D);
+    DevBuf vp(2 * D * sizeof(double)), vc(2 * D * sizeof(double)), w(2 * D * sizeof(double));
+    DevBuf part(qkl_blas_partials() * 2 * sizeof(double)), tmp(4 * sizeof(double));
+    std::vector<DevBuf> basis;
+    if (o.start_dev) hip_check(hipMemcpyAsync(vc.get(), o.start_dev, 2 * D * sizeof(double), hipMemcpyDeviceToDevice, s), "start");
+    else if (o.start) hip_check(hipMemcpyAsync(vc.get(), o.start, 2 * D * sizeof(double), hipMemcpyHostToDevice, s), "start");
+    else lcheck(qkl_fill_random(0, 2 * D, o.seed, 0, 1, 1, 0, vc.d(), s), "random start");
+    lcheck(qkl_fill_zero(2 * D, vp.d(), s), "zero");
+    auto dotc = [&](const double* a, const double* b) { return dot_host(true, D, a, b, part.d(), tmp.d(), s); };
+    double n0 = std::sqrt(dotc(vc.d(), vc.d()).real());
+    if (n0 != 0.0) lcheck(qkl_scal(1, D, 1.0 / n0, 0.0, vc.d(), s), "scal");
+    const bool store = o.reorth || o.want_vector;
+    auto push_basis = [&](const double* v) {
+        basis.emplace_back(2 * D * sizeof(double));
+        hip_check(hipMemcpyAsync(basis.back().get(), v, 2 * D * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+    };
+    if (store) push_basis(vc.d());
+    Replay rp{kmax, o.tol, o.fixed_steps, {}, {}};
+    const double t0 = now_s();
+    for (int it = 0; it < kmax && !rp.done; ++it) {
+        E.apply(vc.d(), w.d(), true, s);
+        const double a = dotc(vc.d(), w.d()).real();
+        lcheck(qkl_axpy(1, D, -a, 0.0, vc.d(), w.d(), s), "axpy");
+        if (it > 0) lcheck(qkl_axpy(1, D, -rp.betas.back(), 0.0, vp.d(), w.d(), s), "axpy");
+        if (o.reorth)
+            for (auto& b : basis) {
+                const auto h = dotc(b.d(), w.d());
+                lcheck(qkl_axpy(1, D, -h.real(), -h.imag(), b.d(), w.d(), s), "axpy");
+            }
+        const double b = std::sqrt(dotc(w.d(), w.d()).real());
+        rp.feed(a, b);
+        if (rp.done) break;
+        std::swap(vp, vc);
+        std::swap(vc, w);
+        lcheck(qkl_scal(1, D, 1.0 / b, 0.0, vc.d(), s), "scal");
+        if (store) push_basis(vc.d());
+    }
+    out.loop_seconds = now_s() - t0;
+    const int n = (int)rp.alphas.size();
+    out.iterations = rp.next;
+    out.steps_run = rp.next;
+    out.converged = rp.converged;
+    out.alphas = rp.alphas;
+    out.betas = rp.betas;
+    std::vector<double> z;
+    out.energy = tridiag_min(rp.alphas.data(), rp.betas.data(), n, o.want_vector ? &z : nullptr);
+    if (o.want_vector) {
+        DevBuf ritz(2 * D * sizeof(double));
+        lcheck(qkl_fill_zero(2 * D, ritz.d(), s), "zero");
+        for (int i = 0; i < n; ++i) lcheck(qkl_axpy(1, D, z[i], 0.0, basis[i].d(), ritz.d(), s), "axpy");
+        const double nr = std::sqrt(dotc(ritz.d(), ritz.d()).real());
+        if (nr != 0.0) lcheck(qkl_scal(1, D, 1.0 / nr, 0.0, ritz.d(), s), "scal");
+        out.vector.resize(2 * D);
+        hip_check(hipMemcpyAsync(out.vector.data(), ritz.get(), 2 * D * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        hip_check(hipStreamSynchronize(s), "sync");
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- plain recurrences
+
+TridiagOutput tridiag_run_dev(const Engine& E, const double* start_dev, int n_steps, hipStream_t s, double* seconds)
+{
+    TridiagOutput out;
+    const size_t D = (size_t)E.dim();
+    if (D == 0 || n_steps <= 0) return out;
+    if (!E.is_real()) throw std::invalid_argument("real recurrence needs a real Hamiltonian");
+    RealCore C(E, n_steps, s, false);
+    hip_check(hipMemcpyAsync(C.cur(), start_dev, D * sizeof(double), hipMemcpyDeviceToDevice, s), "start");
+    hip_check(hipStreamSynchronize(s), "sync");
+    const double t0 = now_s();
+    C.init_norm();
+    for (int k = 0; k < n_steps; ++k) C.step(k, nullptr);
+    std::vector<double> ha(n_steps), hn(n_steps + 1);
+    hip_check(hipMemcpyAsync(ha.data(), C.alpha.d(), n_steps * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+    hip_check(hipMemcpyAsync(hn.data(), C.nrm.d(), (n_steps + 1) * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+    hip_check(hipStreamSynchronize(s), "sync");
+    if (seconds) *seconds = now_s() - t0;
+    out.norm0 = hn[0];
+    if (hn[0] < 1e-15) return out;
+    for (int k = 0; k < n_steps; ++k) {
+        out.alphas.push_back(ha[k]);
+        if (hn[k + 1] < 1e-15) break;
+        out.betas.push_back(hn[k + 1]);
+    }
+    return out;
+}
+
+TridiagOutput tridiag_run(const Engine& E, const double* start, bool start_cplx, int n_steps, hipStream_t s)
+{
+    TridiagOutput out;
+    const size_t D = (size_t)E.dim();
+    if (D == 0 || n_steps <= 0) return out;
+    if (E.is_real() && !start_cplx) {
+        DevBuf st(D * sizeof(double));
+        hip_check(hipMemcpyAsync(st.get(), start, D * sizeof(double), hipMemcpyHostToDevice, s), "start");
+        return tridiag_run_dev(E, st.d(), n_steps, s, nullptr);
+    }
+    // complex start (or complex H): host-scalar loop
+    DevBuf vp(2 * D * sizeof(double)), vc(2 * D * sizeof(double)), w(2 * D * sizeof(double));
+    DevBuf part(qkl_blas_partials() * 2 * sizeof(double)), tmp(4 * sizeof(double));
+    if (start_cplx) hip_check(hipMemcpyAsync(vc.get(), start, 2 * D * sizeof(double), hipMemcpyHostToDevice, s), "start");
+    else {
+        std::vector<double> c(2 * D, 0.0);
+        for (size_t i = 0; i < D; ++i) c[2 * i] = start[i];
+        hip_check(hipMemcpyAsync(vc.get(), c.data(), 2 * D * sizeof(double), hipMemcpyHostToDevice, s), "start");
+        hip_check(hipStreamSynchronize(s), "sync");
+    }
+    lcheck(qkl_fill_zero(2 * D, vp.d(), s), "zero");
+    auto dotc = [&](const double* a, const double* b) { return dot_host(true, D, a, b, part.d(), tmp.d(), s); };
+    const double n0 = std::sqrt(dotc(vc.d(), vc.d()).real());
+    out.norm0 = n0;
+    if (n0 < 1e-15) return out;
+    lcheck(qkl_scal(1, D, 1.0 / n0, 0.0, vc.d(), s), "scal");
+    for (int it = 0; it < n_steps; ++it) {
+        E.apply(vc.d(), w.d(), true, s);
+        const double a = dotc(vc.d(), w.d()).real();
+        out.alphas.push_back(a);
+        lcheck(qkl_axpy(1, D, -a, 0.0, vc.d(), w.d(), s), "axpy");
+        if (it > 0) lcheck(qkl_axpy(1, D, -out.betas.back(), 0.0, vp.d(), w.d(), s), "axpy");
+        const double b = std::sqrt(dotc(w.d(), w.d()).real());
+        if (b < 1e-15) break;
+        out.betas.push_back(b);
+        std::swap(vp, vc);
+        std::swap(vc, w);
+        lcheck(qkl_scal(1, D, 1.0 / b, 0.0, vc.d(), s), "scal");
+    }
+    return out;
+}
+
+BlockTridiag block_tridiag_run(const Engine& E, double* X0, int ncol, int group, int n_steps, hipStream_t s,
+                               double* seconds)
+{
+    BlockTridiag out;
+    const size_t D = (size_t)E.dim();
+    if (ncol <= 0 || ncol > 64) throw std::invalid_argument("block width must be 1..64");
+    if (group <= 0 || ncol % group) throw std::invalid_argument("columns must split into whole groups");
+    if (!E.is_real()) throw std::invalid_argument("block recurrences need a real Hamiltonian");
+    out.ncol = ncol;
+    out.group = group;
+    const int ng = ncol / group;
+    out.alphas.assign(ng, {});
+    out.betas.assign(ng, {});
+    out.norm0.assign(ng, 0.0);
+    if (D == 0 || n_steps <= 0) return out;
+    const size_t nel = D * (size_t)ncol;
+    DevBuf Pb(nel * sizeof(double)), Wb(nel * sizeof(double));
+    double* P = Pb.d();
+    double* Cc = X0;
+    double* W = Wb.d();
+    const int npA = qkl_block_partials(E.tables()), NB = qkl_blas_partials();
+    DevBuf partA((size_t)npA * ncol * sizeof(double)), partB((size_t)NB * ncol * sizeof(double));
+    DevBuf alpha((size_t)(n_steps + 1) * ncol * sizeof(double)), nrm((size_t)(n_steps + 2) * ncol * sizeof(double));
+    // initial norms: an update with zero coefficient leaves X unchanged and
+    // yields the column sums of squares
+    {
+        std::vector<double> z0(ncol, 0.0), one(ncol, 1.0);
+        DevBuf za(ncol * sizeof(double)), on(ncol * sizeof(double));
+        hip_check(hipMemcpyAsync(za.get(), z0.data(), ncol * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+        hip_check(hipMemcpyAsync(on.get(), one.data(), ncol * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+        lcheck(qkl_lanczos_update(0, D, ncol, ncol, Cc, Cc, za.d(), on.d(), 0, ncol, partB.d(), s), "norm0");
+        lcheck(qkl_reduce(partB.d(), NB, ncol, group, 1, 0, nrm.d(), 0, ncol, s), "reduce");
+        hip_check(hipStreamSynchronize(s), "sync");
+    }
+    lcheck(qkl_fill_zero(nel, P, s), "zero");
+    const double t0 = now_s();
+    for (int k = 0; k < n_steps; ++k) {
+        lcheck(qkl_apply_block(E.tables(), E.wide() ? 1 : 0, Cc, W, ncol, 0, ncol, k, P, nrm.d(), ncol, partA.d(), npA, s),
+               "apply_block");
+        lcheck(qkl_reduce(partA.d(), npA, ncol, group, 0, 0, alpha.d(), k, ncol, s), "reduce alpha");
+        lcheck(qkl_lanczos_update(0, D, ncol, ncol, W, Cc, alpha.d(), nrm.d(), k, ncol, partB.d(), s), "update");
+        lcheck(qkl_reduce(partB.d(), NB, ncol, group, 1, 0, nrm.d(), k + 1, ncol, s), "reduce beta");
+        double* t = P;
+        P = Cc;
+        Cc = W;
+        W = t;
+    }
+    hip_check(hipStreamSynchronize(s), "sync");
+    if (seconds) *seconds = now_s() - t0;
+    std::vector<double> ha((size_t)n_steps * ncol), hn((size_t)(n_steps + 1) * ncol);
+    hip_check(hipMemcpy(ha.data(), alpha.get(), ha.size() * sizeof(double), hipMemcpyDeviceToHost), "d2h");
+    hip_check(hipMemcpy(hn.data(), nrm.get(), hn.size() * sizeof(double), hipMemcpyDeviceToHost), "d2h");
+    for (int g = 0; g < ng; ++g) {
+        const int c = g * group;
+        out.norm0[g] = hn[c];
+        if (hn[c] < 1e-15) continue;
+        for (int k = 0; k < n_steps; ++k) {
+            out.alphas[g].push_back(ha[(size_t)k * ncol + c]);
+            const double b = hn[(size_t)(k + 1) * ncol + c];
+            if (b < 1e-15) break;
+            out.betas[g].push_back(b);
+        }
+    }
+    return out;
+}
+
+void ftlm_observables(const std::vector<std::vector<double>>& alphas, const std::vector<std::vector<double>>& betas,
+                      const std::vector<double>& norm2, const std::vector<double>& bT, std::vector<double>& Zo,
+                      std::vector<double>& Eo, std::vector<double>& Cvo)
+{
+    const size_t R = alphas.size(), nT = bT.size();
+    std::vector<double> Z(nT, 0.0), E(nT, 0.0), E2(nT, 0.0);
+    for (size_t r = 0; r < R; ++r) {
+        const int n = (int)alphas[r].size();
+        if (n == 0) continue;
+        std::vector<double> ev(n), fc(n);
+        if (n == 1) {
+            ev[0] = alphas[r][0];
+            fc[0] = 1.0;
+        } else {
+            std::vector<double> z((size_t)n * n);
+            tridiag_qr(alphas[r].data(), betas[r].data(), n, ev.data(), z.data());
+            for (int i = 0; i < n; ++i) fc[i] = z[i];   // z[0][i]
+        }
+        for (size_t t = 0; t < nT; ++t)
+            for (int i = 0; i < n; ++i) {
+                const double w = norm2[r] * fc[i] * fc[i] * std::exp(-bT[t] * ev[i]);
+                Z[t] += w;
+                E[t] += ev[i] * w;
+                E2[t] += ev[i] * ev[i] * w;
+            }
+    }
+    Zo.assign(nT, 0.0);
+    Eo.assign(nT, 0.0);
+    Cvo.assign(nT, 0.0);
+    for (size_t t = 0; t < nT; ++t) {
+        const double z = Z[t] / R, e = E[t] / R, e2 = E2[t] / R;
+        Zo[t] = z;
+        Eo[t] = e / z;
+        Cvo[t] = bT[t] * bT[t] * (e2 / z - (e / z) * (e / z));
+    }
+}
+
+double spectral_value(const double* alphas, const double* betas, size_t n, double norm_phi0, double omega, double E0,
+                      double eta)
+{
+    if (n == 0) return 0.0;
+    const std::complex<double> z(omega + E0, eta);
+    std::complex<double> f = 0.0;
+    for (int i = (int)n - 1; i >= 0; --i) {
+        if (i == (int)n - 1) f = 1.0 / (z - alphas[i]);
+        else f = 1.0 / (z - alphas[i] - betas[i] * betas[i] * f);
+    }
+    return -1.0 / M_PI * std::imag(norm_phi0 * norm_phi0 * f);
+}
+
+// ---------------------------------------------------------------- Davidson
+
+namespace {
+
+using cd = std::complex<double>;
+
+// Jacobi rotations for a small Hermitian matrix: the same rotation sequence
+// as the reference's solve_small_hermitian (src/solvers/davidson.cpp:22-82)
+void small_hermitian(int n, std::vector<cd> A, std::vector<double>& ev, std::vector<std::vector<cd>>& vecs)
+{
+    std::vector<cd> V((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    auto a = [&](int i, int j) -> cd& { return A[(size_t)i * n + j]; };
+    auto v = [&](int i, int j) -> cd& { return V[(size_t)i * n + j]; };
+    for (int it = 0; it < 1000; ++it) {
+        double mo = 0.0;
+        int p = 0, q = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j)
+                if (std::abs(a(i, j)) > mo) {
+                    mo = std::abs(a(i, j));
+                    p = i;
+                    q = j;
+                }
+        if (mo < 1e-15) break;
+        const cd app = a(p, p), aqq = a(q, q), apq = a(p, q);
+        const double phi = 0.5 * std::atan2(2.0 * std::abs(apq), std::real(app - aqq));
+        const cd c = std::cos(phi);
+        const cd s = std::sin(phi) * apq / std::abs(apq);
+        for (int i = 0; i < n; ++i) {
+            const cd vip = v(i, p), viq = v(i, q);
+            v(i, p) = c * vip + s * viq;
+            v(i, q) = -std::conj(s) * vip + c * viq;
+            if (i != p && i != q) {
+                const cd aip = a(i, p), aiq = a(i, q);
+                a(i, p) = a(p, i) = c * aip + s * aiq;
+                a(i, q) = a(q, i) = -std::conj(s) * aip + c * aiq;
+            }
+        }
+        a(p, p) = c * c * app + s * std::conj(s) * aqq + 2.0 * std::real(c * s * std::conj(apq));
+        a(q, q) = s * std::conj(s) * app + c * c * aqq - 2.0 * std::real(c * s * std::conj(apq));
+        a(p, q) = a(q, p) = 0.0;
+    }
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return std::real(a(i, i)) < std::real(a(j, j)); });
+    ev.clear();
+    vecs.clear();
+    for (int i : idx) {
+        ev.push_back(std::real(a(i, i)));
+        std::vector<cd> col(n);
+        for (int j = 0; j < n; ++j) col[j] = v(j, i);
+        vecs.push_back(col);
+    }
+}
+
+} // namespace
+
+DavidsonOutput davidson_run(const Engine& E, int n_eig, int max_subspace, double tol, bool want_vectors, hipStream_t s)
+{
+    DavidsonOutput out;
+    const size_t D = (size_t)E.dim();
+    if (D == 0) return out;
+    n_eig = (int)std::min<size_t>((size_t)std::max(n_eig, 1), D);
+    max_subspace = std::max(max_subspace, 2 * n_eig);
+    const size_t vb = 2 * D * sizeof(double);
+    DevBuf diag(vb), part(qkl_blas_partials() * 2 * sizeof(double)), tmp(4 * sizeof(double));
+    E.diagonal(diag.d(), true, s);
+    std::vector<double> dh(2 * D);
+    hip_check(hipMemcpyAsync(dh.data(), diag.get(), vb, hipMemcpyDeviceToHost, s), "d2h");
+    hip_check(hipStreamSynchronize(s), "sync");
+    auto dotc = [&](const double* a, const double* b) { return dot_host(true, D, a, b, part.d(), tmp.d(), s); };
+    auto axpy = [&](cd a, const double* x, double* y) { lcheck(qkl_axpy(1, D, a.real(), a.imag(), x, y, s), "axpy"); };
+    auto nrm = [&](const double* x) { return std::sqrt(dotc(x, x).real()); };
+    // n_eig smallest diagonal entries (ties by index)
+    std::vector<size_t> di(D);
+    std::iota(di.begin(), di.end(), 0);
+    std::partial_sort(di.begin(), di.begin() + n_eig, di.end(), [&](size_t i, size_t j) {
+        return dh[2 * i] < dh[2 * j] || (dh[2 * i] == dh[2 * j] && i < j);
+    });
+    std::vector<DevBuf> V, HV;
+    for (int i = 0; i < n_eig; ++i) {
+        DevBuf v(vb);
+        lcheck(qkl_fill_zero(2 * D, v.d(), s), "zero");
+        const double one = 1.0;
+        hip_check(hipMemcpyAsync(v.d() + 2 * di[i], &one, sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+        for (auto& vp : V) axpy(-dotc(vp.d(), v.d()), vp.d(), v.d());
+        double n = nrm(v.d());
+        if (n < 1e-10) {
+            lcheck(qkl_fill_random(0, 2 * D, 1234, 0, 1, 1, 0, v.d(), s), "random");
+            for (auto& vp : V) axpy(-dotc(vp.d(), v.d()), vp.d(), v.d());
+            n = nrm(v.d());
+        }
+        lcheck(qkl_scal(1, D, 1.0 / n, 0.0, v.d(), s), "scal");
+        DevBuf hv(vb);
+        E.apply(v.d(), hv.d(), true, s);
+        V.push_back(std::move(v));
+        HV.push_back(std::move(hv));
+    }
+    for (int iter = 0; iter < 100; ++iter) {
+        const int m = (int)V.size();
+        std::vector<cd> Hs((size_t)m * m);
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j <= i; ++j) {
+                Hs[(size_t)i * m + j] = dotc(V[i].d(), HV[j].d());
+                if (i != j) Hs[(size_t)j * m + i] = std::conj(Hs[(size_t)i * m + j]);
+            }
+        std::vector<double> ev;
+        std::vector<std::vector<cd>> es;
+        small_hermitian(m, Hs, ev, es);
+        bool all_conv = true;
+        std::vector<DevBuf> corr;
+        for (int k = 0; k < n_eig; ++k) {
+            const double lam = ev[k];
+            DevBuf r(vb);
+            lcheck(qkl_fill_zero(2 * D, r.d(), s), "zero");
+            for (int i = 0; i < m; ++i) {
+                axpy(es[k][i], HV[i].d(), r.d());
+                axpy(-lam * es[k][i], V[i].d(), r.d());
+            }
+            if (nrm(r.d()) > tol) {
+                all_conv = false;
+                DevBuf t(vb);
+                lcheck(qkl_davidson_precond(1, D, r.d(), diag.d(), lam, t.d(), s), "precond");
+                corr.push_back(std::move(t));
+            }
+        }
+        if (all_conv || iter == 99) {
+            out.iterations = iter + 1;
+            out.converged = all_conv;
+            for (int k = 0; k < n_eig; ++k) {
+                out.eigenvalues.push_back(ev[k]);
+                if (want_vectors) {
+                    DevBuf e(vb);
+                    lcheck(qkl_fill_zero(2 * D, e.d(), s), "zero");
+                    for (int i = 0; i < m; ++i) axpy(es[k][i], V[i].d(), e.d());
+                    std::vector<double> h(2 * D);
+                    hip_check(hipMemcpyAsync(h.data(), e.get(), vb, hipMemcpyDeviceToHost, s), "d2h");
+                    hip_check(hipStreamSynchronize(s), "sync");
+                    out.eigenvectors.push_back(std::move(h));
+                }
+            }
+            return out;
+        }
+        if (V.size() + corr.size() > (size_t)max_subspace) {
+            std::vector<DevBuf> nV, nHV;
+            for (int k = 0; k < n_eig; ++k) {
+                DevBuf rv(vb), rh(vb);
+                lcheck(qkl_fill_zero(2 * D, rv.d(), s), "zero");
+                lcheck(qkl_fill_zero(2 * D, rh.d(), s), "zero");
+                for (int i = 0; i < m; ++i) {
+                    axpy(es[k][i], V[i].d(), rv.d());
+                    axpy(es[k][i], HV[i].d(), rh.d());
+                }
+                nV.push_back(std::move(rv));
+                nHV.push_back(std::move(rh));
+            }
+            V = std::move(nV);
+            HV = std::move(nHV);
+        }
+        for (auto& t : corr) {
+            for (auto& v : V) axpy(-dotc(v.d(), t.d()), v.d(), t.d());
+            const double n = nrm(t.d());
+            if (n > 1e-10) {
+                lcheck(qkl_scal(1, D, 1.0 / n, 0.0, t.d(), s), "scal");
+                DevBuf ht(vb);
+                E.apply(t.d(), ht.d(), true, s);
+                V.push_back(std::move(t));
+                HV.push_back(std::move(ht));
+            }
+        }
+    }
+    return out;
+}
+
+} // namespace qk

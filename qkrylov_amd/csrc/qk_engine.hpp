@@ -1,0 +1,150 @@
+// qk_engine.hpp -- device-resident Hamiltonian engine and solver kernels
+// orchestration (host side). Calls the HIP kernels only through qk_launch.h.
+#pragma once
+#include "qk_launch.h"
+#include "qk_model.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace qk {
+
+void hip_check(hipError_t e, const char* what);
+
+// RAII device allocation (hipMalloc / hipFree)
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes);
+    ~DevBuf();
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; o.bytes_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept;
+    void* get() const { return p_; }
+    double* d() const { return static_cast<double*>(p_); }
+    size_t bytes() const { return bytes_; }
+    void reset();
+
+private:
+    void* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// one library stream per host thread (default priority)
+hipStream_t default_stream();
+void ensure_device(int id);
+
+struct EngineOptions {
+    int NL = -1;   // lo-bit split (-1: automatic)
+};
+
+class Engine {
+public:
+    Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpSum& ops, EngineOptions opt = {});
+
+    int64_t dim() const { return basis_->dim(); }
+    bool is_real() const { return comp_.is_real; }
+    bool wide() const { return basis_->nbits() > 32; }
+    const QkTables* tables() const { return &T_; }
+    const BasisSpec& basis() const { return *basis_; }
+    const Compiled& compiled() const { return comp_; }
+    int NL() const { return host_.NL; }
+    size_t table_bytes() const { return table_bytes_; }
+
+    // y = H^dagger-convention apply of the reference (y[a] = sum conj(amp) x[b])
+    void apply(const double* x, double* y, bool cplx, hipStream_t s) const;
+    void apply_block(const double* X, double* Y, int ld, int ncols, hipStream_t s) const;
+    void diagonal(double* d, bool cplx, hipStream_t s) const;
+    // y = O x for a diagonal operator compiled into this engine
+    void apply_diag_operator(const double* x, bool xcplx, double* y, hipStream_t s) const;
+
+private:
+    std::shared_ptr<const BasisSpec> basis_;
+    Compiled comp_;
+    HostTables host_;
+    DevBuf tiles_, hoff_, lrank_, lolist_, br_, dg_;
+    QkTables T_{};
+    size_t table_bytes_ = 0;
+};
+
+// ---------------------------------------------------------------- tridiagonal
+// Implicit symmetric QR with Wilkinson shift, deflation test
+// |e_i| < 1e-14 (|d_i| + |d_{i+1}|), at most 1000 sweeps; the same steps as
+// the reference's tridiag_ground_state_full (src/solvers/lanczos.cpp:25-98).
+// evals: all eigenvalues in QR order; z (n*n row-major, optional).
+void tridiag_qr(const double* alpha, const double* beta, int n, double* d, double* z);
+double tridiag_min(const double* alpha, const double* beta, int n, std::vector<double>* evec);
+
+// ---------------------------------------------------------------- Lanczos
+struct LanczosOptions {
+    int maxiter = 200;
+    double tol = 1e-12;
+    bool reorth = true;          // reference behaviour: full reorthogonalisation
+    bool want_vector = false;
+    bool fixed_steps = false;    // run min(maxiter, dim) steps, no convergence test
+    uint64_t seed = 1234;
+    int check_every = 0;         // 0: automatic
+    const double* start = nullptr;   // host start vector (real, or interleaved complex for complex H)
+    const double* start_dev = nullptr;   // device start vector (same layout)
+};
+
+struct LanczosOutput {
+    double energy = 0.0;
+    int iterations = 0;
+    bool converged = false;
+    std::vector<double> alphas, betas;
+    std::vector<double> vector;      // real, or interleaved complex for complex H
+    double loop_seconds = 0.0;       // wall time of the iteration loop
+    double ritz_seconds = 0.0;
+    int steps_run = 0;               // device steps computed (>= iterations)
+};
+
+LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t s);
+
+// three-term recurrence from a start vector without reorthogonalisation;
+// alphas/betas truncated at the first beta < 1e-15 as the reference's
+// compute_tridiag / continued_fraction_coeffs (ftlm.cpp:24-53, dynamics.cpp:12-59)
+struct TridiagOutput {
+    std::vector<double> alphas, betas;
+    double norm0 = 0.0;
+};
+TridiagOutput tridiag_run(const Engine& E, const double* start_host, bool start_cplx, int n_steps, hipStream_t s);
+// real H, real device start vector (not modified); seconds: recurrence wall time
+TridiagOutput tridiag_run_dev(const Engine& E, const double* start_dev, int n_steps, hipStream_t s,
+                              double* seconds = nullptr);
+
+// R independent recurrences on a row-major block (columns grouped `group`
+// at a time into one complex column pair when group == 2)
+struct BlockTridiag {
+    int ncol = 0, group = 1;
+    std::vector<std::vector<double>> alphas, betas;   // per group
+    std::vector<double> norm0;                         // per group
+};
+// X0: device block start (nrows x ncol, row-major, ld = ncol); consumed.
+BlockTridiag block_tridiag_run(const Engine& E, double* X0_dev, int ncol, int group, int n_steps, hipStream_t s,
+                               double* seconds = nullptr);
+
+// FTLM observables from tridiagonals (reference ftlm.cpp:122-170, for a grid
+// of inverse temperatures); weights use the squared start-vector norms.
+void ftlm_observables(const std::vector<std::vector<double>>& alphas, const std::vector<std::vector<double>>& betas,
+                      const std::vector<double>& norm2, const std::vector<double>& betas_T, std::vector<double>& Z,
+                      std::vector<double>& E, std::vector<double>& Cv);
+
+// ---------------------------------------------------------------- Davidson
+struct DavidsonOutput {
+    std::vector<double> eigenvalues;
+    std::vector<std::vector<double>> eigenvectors;   // interleaved complex
+    int iterations = 0;
+    bool converged = false;
+};
+DavidsonOutput davidson_run(const Engine& E, int n_eig, int max_subspace, double tol, bool want_vectors,
+                            hipStream_t s);
+
+double spectral_value(const double* alphas, const double* betas, size_t n, double norm_phi0, double omega, double E0,
+                      double eta);
+
+} // namespace qk

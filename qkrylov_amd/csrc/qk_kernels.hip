@@ -1,0 +1,988 @@
+// qk_kernels.hip -- CDNA4 (gfx950) kernels of the matrix-free engine.
+//
+// Pull form: every output row is produced by exactly one lane, so there are
+// no atomics and results are deterministic. A wavefront owns a tile: up to 64
+// consecutive basis rows that share the high bits `hi` of their state, so
+// everything that depends on hi alone (term conditions on high bits, the
+// target block offset hoff[hi ^ F_hi], the high part of the Jordan-Wigner
+// parity) is wave-uniform and runs on the scalar unit. Each lane decodes its
+// low bits from a coalesced lolist load, applies the compiled branches with
+// bit operations, ranks the target through the LDS-resident lo-rank table,
+// and gathers x. Reductions (Lanczos dot products and norms) are two-pass and
+// fixed-order: per-workgroup partials, then one workgroup sums them.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "qk_launch.h"
+
+using namespace qk;
+
+#define QK_TPB 256
+#define QK_WPB 4
+#define QK_BLAS_BLOCKS 1024
+#define QK_MAX_APPLY_BLOCKS 1024
+
+namespace {
+
+struct QkScal {
+    uint32_t lomask;
+    uint32_t ntiles;
+    int32_t nbr, ndg, NL, nlo;
+    double dcr, dci;
+};
+
+QkScal make_scal(const QkTables* T)
+{
+    QkScal s;
+    s.lomask = static_cast<uint32_t>(T->lomask);
+    s.ntiles = T->ntiles;
+    s.nbr = T->nbr;
+    s.ndg = T->ndg;
+    s.NL = T->NL;
+    s.nlo = T->nlo;
+    s.dcr = T->dconst_re;
+    s.dci = T->dconst_im;
+    return s;
+}
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <typename ST>
+__device__ __forceinline__ int popc(ST x);
+template <>
+__device__ __forceinline__ int popc<uint32_t>(uint32_t x) { return __builtin_popcount(x); }
+template <>
+__device__ __forceinline__ int popc<uint64_t>(uint64_t x) { return __builtin_popcountll(x); }
+
+// XCD-aware tile schedule: workgroups are dispatched round-robin over the 8
+// XCDs, so workgroup b runs on XCD b % 8. Each XCD gets one contiguous
+// segment of tiles, and its waves sweep that segment together, keeping the
+// rows (and most gather targets) in flight on one XCD inside one window of
+// its L2.
+__device__ __forceinline__ void tile_schedule(uint32_t ntiles, uint32_t& t0, uint32_t& tend, uint32_t& step)
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t w = rfl(threadIdx.x >> 6), wpb = blockDim.x >> 6;
+    if (nb >= 8 && (nb & 7u) == 0) {
+        const uint32_t xcd = b & 7u, loc = b >> 3, nloc = nb >> 3;
+        const uint32_t seg = (ntiles + 7u) / 8u;
+        const uint32_t s0 = min(xcd * seg, ntiles), s1 = min(s0 + seg, ntiles);
+        t0 = s0 + loc * wpb + w;
+        tend = s1;
+        step = nloc * wpb;
+    } else {
+        t0 = b * wpb + w;
+        tend = ntiles;
+        step = nb * wpb;
+    }
+}
+
+template <typename ST, bool CPLX>
+__device__ __forceinline__ void eval_diag(const QkDiagTerm* __restrict__ dg, int ndg, double dcr, double dci, ST s,
+                                          double& dre, double& dim)
+{
+    dre = dcr;
+    dim = dci;
+    for (int i = 0; i < ndg; ++i) {
+        const QkDiagTerm g = dg[i];
+        const ST m = static_cast<ST>(g.mask);
+        int v;
+        if (g.kind == 0) v = popc<ST>(s & (s >> g.d) & m);
+        else if (g.kind == 1) v = ((s & m) == m) ? 1 : 0;
+        else v = ((s & m) == static_cast<ST>(g.val)) ? 1 : 0;
+        dre = fma(g.re, static_cast<double>(v), dre);
+        if (CPLX) dim = fma(g.im, static_cast<double>(v), dim);
+    }
+}
+
+template <bool U16>
+__device__ __forceinline__ uint32_t lds_rank(const unsigned char* smem, uint32_t lo)
+{
+    if (U16) return reinterpret_cast<const uint16_t*>(smem)[lo];
+    return reinterpret_cast<const uint32_t*>(smem)[lo];
+}
+
+template <bool U16>
+__device__ __forceinline__ void load_lrank(unsigned char* smem, const uint32_t* __restrict__ lrank, int nlo)
+{
+    for (int i = threadIdx.x; i < nlo; i += blockDim.x) {
+        if (U16) reinterpret_cast<uint16_t*>(smem)[i] = static_cast<uint16_t>(lrank[i]);
+        else reinterpret_cast<uint32_t*>(smem)[i] = lrank[i];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------ apply
+template <typename ST, bool CPLX, bool U16, bool LZ>
+__global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
+                                                   const uint32_t* __restrict__ lrank,
+                                                   const uint32_t* __restrict__ lolist,
+                                                   const QkBranch* __restrict__ br, const QkDiagTerm* __restrict__ dg,
+                                                   QkScal sc, const double* __restrict__ x, double* __restrict__ y,
+                                                   int lz_k, const double* __restrict__ prev,
+                                                   const double* __restrict__ nrm, double* __restrict__ partials)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double wsum[QK_WPB];
+    load_lrank<U16>(smem, lrank, sc.nlo);
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const int NL = sc.NL;
+    const uint32_t lomask = sc.lomask;
+    double inv_n = 1.0, cprev = 0.0;
+    if (LZ) {
+        inv_n = 1.0 / nrm[lz_k];
+        if (lz_k > 0) cprev = nrm[lz_k] / nrm[lz_k - 1];
+    }
+    double part = 0.0;
+
+    uint32_t t0, tend, step;
+    tile_schedule(sc.ntiles, t0, tend, step);
+    for (uint32_t t = t0; t < tend; t += step) {
+        const QkTile tl = tiles[t];
+        const uint32_t hi = tl.hi;
+        const bool active = lane < tl.count;
+        const uint32_t base_self = hoff[hi];
+        const uint32_t lidx = active ? lane : 0u;
+        const uint32_t my_lr = tl.row_start - base_self + lidx;
+        const uint32_t row = tl.row_start + lidx;
+        const uint32_t lo = lolist[tl.lo_base + lidx];
+        const ST s = (static_cast<ST>(hi) << NL) | static_cast<ST>(lo);
+
+        double dre, dim;
+        eval_diag<ST, CPLX>(dg, sc.ndg, sc.dcr, sc.dci, s, dre, dim);
+        double ar, ai = 0.0, xr, xi = 0.0;
+        if (CPLX) {
+            xr = x[2 * (size_t)row];
+            xi = x[2 * (size_t)row + 1];
+            ar = dre * xr - dim * xi;
+            ai = dre * xi + dim * xr;
+        } else {
+            xr = x[row];
+            ar = dre * xr;
+        }
+
+        for (int b = 0; b < sc.nbr; ++b) {
+            const QkBranch B = br[b];
+            const uint32_t hcm = static_cast<uint32_t>(B.cm >> NL);
+            const uint32_t hcv = static_cast<uint32_t>(B.cv >> NL);
+            const uint32_t hm = hi & hcm;
+            const bool u0 = (hm == hcv);
+            const bool u1 = (B.alt != 0) && (hm == (hcv ^ static_cast<uint32_t>(B.alt >> NL)));
+            if (!(u0 || u1)) continue;
+            const uint32_t base_t = hoff[hi ^ static_cast<uint32_t>(B.F >> NL)];
+            const int hpar = __builtin_popcount(hi & static_cast<uint32_t>(B.M >> NL)) & 1;
+            const uint32_t lcm = static_cast<uint32_t>(B.cm) & lomask;
+            const uint32_t lcv = static_cast<uint32_t>(B.cv) & lomask;
+            const uint32_t lalt = static_cast<uint32_t>(B.alt) & lomask;
+            const uint32_t lF = static_cast<uint32_t>(B.F) & lomask;
+            const uint32_t lM = static_cast<uint32_t>(B.M) & lomask;
+            const uint32_t lf = lo & lcm;
+            const bool ok = (u0 && lf == lcv) || (u1 && lf == (lcv ^ lalt));
+            if (ok) {
+                const uint32_t lr = lF ? lds_rank<U16>(smem, lo ^ lF) : my_lr;
+                const uint32_t rt = base_t + lr;
+                const bool neg = ((__builtin_popcount(lo & lM) & 1) ^ hpar) != 0;
+                const double vr = neg ? -B.re : B.re;
+                if (CPLX) {
+                    const double vi = neg ? -B.im : B.im;
+                    const double tr = x[2 * (size_t)rt], ti = x[2 * (size_t)rt + 1];
+                    ar = fma(vr, tr, ar);
+                    ar = fma(-vi, ti, ar);
+                    ai = fma(vr, ti, ai);
+                    ai = fma(vi, tr, ai);
+                } else {
+                    ar = fma(vr, x[rt], ar);
+                }
+            }
+        }
+
+        if (active) {
+            if (LZ) {
+                double w = ar * inv_n;
+                if (lz_k > 0) w = fma(-cprev, prev[row], w);
+                y[row] = w;
+                part = fma(xr * inv_n, w, part);
+            } else if (CPLX) {
+                y[2 * (size_t)row] = ar;
+                y[2 * (size_t)row + 1] = ai;
+            } else {
+                y[row] = ar;
+            }
+        }
+    }
+    if (LZ) {
+        part = wave_sum(part);
+        if (lane == 0) wsum[threadIdx.x >> 6] = part;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += wsum[i];
+            partials[blockIdx.x] = s;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ block apply
+// Phase A (lane = row): decode up to 64 rows of a tile into per-row lists of
+// (target row, value) in LDS. Phase B (lane = column): for each row, stream
+// the list and gather a contiguous segment of the target row of X. Each
+// decoded matrix element is reused by all nc columns.
+struct BlockLds {
+    static constexpr int kHdr = 64 * 8 + 64 * 4 + 64 * 4;   // diag, off, cnt
+};
+
+template <typename ST, bool U16, bool LZ>
+__global__ __launch_bounds__(QK_TPB) void k_apply_block(
+    const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff, const uint32_t* __restrict__ lrank,
+    const uint32_t* __restrict__ lolist, const QkBranch* __restrict__ br, const QkDiagTerm* __restrict__ dg, QkScal sc,
+    const double* __restrict__ X, double* __restrict__ Y, int ld, int c0, int nc, int lz_k,
+    const double* __restrict__ P, const double* __restrict__ nrm, int ld_sc, double* __restrict__ partials,
+    int npart, int cap, int table_bytes)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double wsum[QK_WPB][64];
+    load_lrank<U16>(smem, lrank, sc.nlo);
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wv = threadIdx.x >> 6;
+    unsigned char* wbase = smem + table_bytes + (size_t)wv * (BlockLds::kHdr + (size_t)cap * 12);
+    double* l_diag = reinterpret_cast<double*>(wbase);
+    uint32_t* l_off = reinterpret_cast<uint32_t*>(wbase + 64 * 8);
+    uint32_t* l_cnt = reinterpret_cast<uint32_t*>(wbase + 64 * 8 + 64 * 4);
+    double* l_coef = reinterpret_cast<double*>(wbase + BlockLds::kHdr);
+    uint32_t* l_tgt = reinterpret_cast<uint32_t*>(wbase + BlockLds::kHdr + (size_t)cap * 8);
+
+    const int NL = sc.NL;
+    const uint32_t lomask = sc.lomask;
+    const bool colok = lane < (uint32_t)nc;
+    const int col = c0 + (int)(colok ? lane : 0u);
+    double inv_n = 1.0, cprev = 0.0;
+    if (LZ && colok) {
+        inv_n = 1.0 / nrm[(size_t)lz_k * ld_sc + col];
+        if (lz_k > 0) cprev = nrm[(size_t)lz_k * ld_sc + col] / nrm[(size_t)(lz_k - 1) * ld_sc + col];
+    }
+    double part = 0.0;
+
+    uint32_t t0, tend, step;
+    tile_schedule(sc.ntiles, t0, tend, step);
+    for (uint32_t t = t0; t < tend; t += step) {
+        const QkTile tl = tiles[t];
+        const uint32_t hi = tl.hi;
+        const uint32_t count = tl.count;
+        const bool active = lane < count;
+        const uint32_t base_self = hoff[hi];
+        const uint32_t lidx = active ? lane : 0u;
+        const uint32_t my_lr = tl.row_start - base_self + lidx;
+        const uint32_t lo = lolist[tl.lo_base + lidx];
+        const ST s = (static_cast<ST>(hi) << NL) | static_cast<ST>(lo);
+
+        // ---- phase A, pass 1: diagonal and entry counts
+        double dre, dim;
+        eval_diag<ST, false>(dg, sc.ndg, sc.dcr, sc.dci, s, dre, dim);
+        uint32_t n_j = 0;
+        for (int b = 0; b < sc.nbr; ++b) {
+            const QkBranch B = br[b];
+            const uint32_t hcm = static_cast<uint32_t>(B.cm >> NL), hcv = static_cast<uint32_t>(B.cv >> NL);
+            const uint32_t hm = hi & hcm;
+            const bool u0 = (hm == hcv);
+            const bool u1 = (B.alt != 0) && (hm == (hcv ^ static_cast<uint32_t>(B.alt >> NL)));
+            if (!(u0 || u1)) continue;
+            const uint32_t lf = lo & (static_cast<uint32_t>(B.cm) & lomask);
+            const uint32_t lcv = static_cast<uint32_t>(B.cv) & lomask;
+            const bool ok = (u0 && lf == lcv) || (u1 && lf == (lcv ^ (static_cast<uint32_t>(B.alt) & lomask)));
+            n_j += ok ? 1u : 0u;
+        }
+        if (!active) n_j = 0;
+        // exclusive scan over the wave
+        uint32_t incl = n_j;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, 64);
+            if ((int)lane >= off) incl += o;
+        }
+        const uint32_t excl = incl - n_j;
+        l_diag[lane] = dre;
+        l_off[lane] = excl;
+        l_cnt[lane] = n_j;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
+        uint32_t j0 = 0;
+        while (j0 < count) {
+            const uint32_t base = __shfl(excl, (int)j0, 64);
+            const bool inr = (lane >= j0) && active && (excl + n_j - base <= (uint32_t)cap);
+            const uint64_t bal = __ballot(inr);
+            const uint32_t j1 = j0 + (uint32_t)__popcll(bal);
+            // ---- phase A, pass 2: write the entries of rows [j0, j1)
+            if (inr) {
+                uint32_t k = excl - base;
+                for (int b = 0; b < sc.nbr; ++b) {
+                    const QkBranch B = br[b];
+                    const uint32_t hcm = static_cast<uint32_t>(B.cm >> NL), hcv = static_cast<uint32_t>(B.cv >> NL);
+                    const uint32_t hm = hi & hcm;
+                    const bool u0 = (hm == hcv);
+                    const bool u1 = (B.alt != 0) && (hm == (hcv ^ static_cast<uint32_t>(B.alt >> NL)));
+                    if (!(u0 || u1)) continue;
+                    const uint32_t lf = lo & (static_cast<uint32_t>(B.cm) & lomask);
+                    const uint32_t lcv = static_cast<uint32_t>(B.cv) & lomask;
+                    const bool ok =
+                        (u0 && lf == lcv) || (u1 && lf == (lcv ^ (static_cast<uint32_t>(B.alt) & lomask)));
+                    if (ok) {
+                        const uint32_t base_t = hoff[hi ^ static_cast<uint32_t>(B.F >> NL)];
+                        const int hpar = __builtin_popcount(hi & static_cast<uint32_t>(B.M >> NL)) & 1;
+                        const uint32_t lF = static_cast<uint32_t>(B.F) & lomask;
+                        const uint32_t lr = lF ? lds_rank<U16>(smem, lo ^ lF) : my_lr;
+                        const bool neg =
+                            ((__builtin_popcount(lo & (static_cast<uint32_t>(B.M) & lomask)) & 1) ^ hpar) != 0;
+                        l_coef[k] = neg ? -B.re : B.re;
+                        l_tgt[k] = base_t + lr;
+                        ++k;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // ---- phase B: lane = column
+            if (colok) {
+                for (uint32_t j = j0; j < j1; ++j) {
+                    const size_t row = (size_t)tl.row_start + j;
+                    const double xs = X[row * ld + col];
+                    double acc = l_diag[j] * xs;
+                    const uint32_t e0 = l_off[j] - base, e1 = e0 + l_cnt[j];
+                    for (uint32_t e = e0; e < e1; ++e) acc = fma(l_coef[e], X[(size_t)l_tgt[e] * ld + col], acc);
+                    if (LZ) {
+                        double w = acc * inv_n;
+                        if (lz_k > 0) w = fma(-cprev, P[row * ld + col], w);
+                        Y[row * ld + col] = w;
+                        part = fma(xs * inv_n, w, part);
+                    } else {
+                        Y[row * ld + col] = acc;
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            j0 = j1;
+        }
+    }
+    if (LZ) {
+        wsum[wv][lane] = part;
+        __syncthreads();
+        if (wv == 0 && colok) {
+            double s = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += wsum[i][lane];
+            partials[(size_t)lane * npart + blockIdx.x] = s;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Lanczos update
+// q_{k+1} = w - alpha_k q_k / nrm_k (in place on w); partial sums of |q_{k+1}|^2.
+// Single vector: thread-strided. Block: lane = column within a 64-wide row.
+template <bool CPLX>
+__global__ __launch_bounds__(QK_TPB) void k_update1(size_t n, double* __restrict__ w, const double* __restrict__ q,
+                                                     const double* __restrict__ alpha,
+                                                     const double* __restrict__ nrm, int k,
+                                                     double* __restrict__ partials)
+{
+    __shared__ double wsum[QK_WPB];
+    const double c = alpha[k] / nrm[k];
+    double part = 0.0;
+    const size_t nel = CPLX ? 2 * n : n;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nel; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = fma(-c, q[i], w[i]);
+        w[i] = v;
+        part = fma(v, v, part);
+    }
+    part = wave_sum(part);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += wsum[i];
+        partials[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(QK_TPB) void k_update_block(size_t nrows, int ncol, int ld, double* __restrict__ w,
+                                                          const double* __restrict__ q,
+                                                          const double* __restrict__ alpha,
+                                                          const double* __restrict__ nrm, int k, int ld_sc,
+                                                          double* __restrict__ partials)
+{
+    __shared__ double wsum[QK_WPB][64];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const bool colok = lane < (uint32_t)ncol;
+    const int c = colok ? (int)lane : 0;
+    const double coef = alpha[(size_t)k * ld_sc + c] / nrm[(size_t)k * ld_sc + c];
+    double part = 0.0;
+    if (colok) {
+        for (size_t r = (size_t)blockIdx.x * (blockDim.x >> 6) + wv; r < nrows; r += (size_t)gridDim.x * (blockDim.x >> 6)) {
+            const size_t i = r * ld + c;
+            const double v = fma(-coef, q[i], w[i]);
+            w[i] = v;
+            part = fma(v, v, part);
+        }
+    }
+    wsum[wv][lane] = part;
+    __syncthreads();
+    if (wv == 0 && colok) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += wsum[i][lane];
+        partials[(size_t)lane * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// ------------------------------------------------------------------ reductions
+// One workgroup per result group. Fixed order: thread-strided sums, then a
+// fixed tree, then columns of the group added in index order.
+__global__ __launch_bounds__(QK_TPB) void k_reduce(const double* __restrict__ partials, int npart, int group,
+                                                    int mode, int cplx, double* __restrict__ out, int idx, int ld_sc)
+{
+    __shared__ double red[QK_TPB];
+    __shared__ double red_i[QK_TPB];
+    const int g = blockIdx.x;
+    double tot = 0.0, tot_i = 0.0;
+    for (int cc = 0; cc < group; ++cc) {
+        const int c = g * group + cc;
+        const double* p = partials + (size_t)c * npart * (cplx ? 2 : 1);
+        double s = 0.0, si = 0.0;
+        for (int i = threadIdx.x; i < npart; i += blockDim.x) {
+            s += p[i];
+            if (cplx) si += p[npart + i];
+        }
+        red[threadIdx.x] = s;
+        red_i[threadIdx.x] = si;
+        __syncthreads();
+        for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) {
+                red[threadIdx.x] += red[threadIdx.x + off];
+                red_i[threadIdx.x] += red_i[threadIdx.x + off];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            tot += red[0];
+            tot_i += red_i[0];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int cc = 0; cc < group; ++cc) {
+            const int c = g * group + cc;
+            double* o = out + (size_t)idx * ld_sc + c * (mode == 2 ? 2 : 1);
+            if (mode == 0) o[0] = tot;
+            else if (mode == 1) o[0] = sqrt(tot);
+            else {
+                o[0] = tot;
+                o[1] = tot_i;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ BLAS-1
+template <bool CPLX>
+__global__ __launch_bounds__(QK_TPB) void k_dot(size_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                                 double* __restrict__ partials)
+{
+    __shared__ double wr[QK_WPB], wi[QK_WPB];
+    double sr = 0.0, si = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (CPLX) {
+            const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+            sr = fma(xr, yr, sr);
+            sr = fma(xi, yi, sr);
+            si = fma(xr, yi, si);
+            si = fma(-xi, yr, si);
+        } else {
+            sr = fma(x[i], y[i], sr);
+        }
+    }
+    sr = wave_sum(sr);
+    if (CPLX) si = wave_sum(si);
+    if ((threadIdx.x & 63) == 0) {
+        wr[threadIdx.x >> 6] = sr;
+        wi[threadIdx.x >> 6] = si;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+            a += wr[i];
+            b += wi[i];
+        }
+        partials[blockIdx.x] = a;
+        if (CPLX) partials[gridDim.x + blockIdx.x] = b;
+    }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(QK_TPB) void k_axpy(size_t n, double ar, double ai, const double* __restrict__ x,
+                                                  double* __restrict__ y)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (CPLX) {
+            const double xr = x[2 * i], xi = x[2 * i + 1];
+            y[2 * i] = fma(ar, xr, fma(-ai, xi, y[2 * i]));
+            y[2 * i + 1] = fma(ar, xi, fma(ai, xr, y[2 * i + 1]));
+        } else {
+            y[i] = fma(ar, x[i], y[i]);
+        }
+    }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(QK_TPB) void k_scal(size_t n, double ar, double ai, double* __restrict__ x)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (CPLX) {
+            const double xr = x[2 * i], xi = x[2 * i + 1];
+            x[2 * i] = ar * xr - ai * xi;
+            x[2 * i + 1] = ar * xi + ai * xr;
+        } else {
+            x[i] *= ar;
+        }
+    }
+}
+
+__global__ __launch_bounds__(QK_TPB) void k_copy_scaled(size_t n, const double* __restrict__ src,
+                                                         double* __restrict__ dst, const double* __restrict__ sc,
+                                                         int idx)
+{
+    const double inv = 1.0 / sc[idx];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] * inv;
+}
+
+__global__ __launch_bounds__(QK_TPB) void k_fill_zero(size_t n, double* __restrict__ x)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] = 0.0;
+}
+
+// splitmix64 finaliser (Steele, Lea, Flood 2014) over a (seed, counter) mix
+__device__ __forceinline__ uint64_t rng_u64(uint64_t seed, uint64_t counter)
+{
+    uint64_t z = seed * 0xD1B54A32D192ED03ULL + (counter + 1ULL) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(QK_TPB) void k_fill_random(int kind, size_t nrows, int ncol, int ld, int c0,
+                                                         uint64_t seed, uint64_t counter0, double* __restrict__ x)
+{
+    const size_t nel = nrows * (size_t)ncol;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nel; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t row = i / (size_t)ncol;
+        const int c = (int)(i - row * (size_t)ncol);
+        const uint64_t z = rng_u64(seed, counter0 + (uint64_t)(c0 + c) * nrows + row);
+        double v;
+        if (kind == 0) v = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        else v = (z >> 63) ? -1.0 : 1.0;
+        x[row * (size_t)ld + c0 + c] = v;
+    }
+}
+
+// h partials for classical Gram-Schmidt: partials[j * nb + b] = sum_i V_j[i] u[i]
+__global__ __launch_bounds__(QK_TPB) void k_multidot(size_t n, int m, const double* __restrict__ V, size_t ldv,
+                                                      const double* __restrict__ u, double* __restrict__ partials)
+{
+    __shared__ double wsum[QK_WPB];
+    const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    for (int j = 0; j < m; ++j) {
+        const double* v = V + (size_t)j * ldv;
+        double s = 0.0;
+        for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) s = fma(v[i], u[i], s);
+        s = wave_sum(s);
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) a += wsum[w];
+            partials[(size_t)j * gridDim.x + blockIdx.x] = a;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(QK_TPB) void k_multiaxpy(size_t n, int m, const double* __restrict__ V, size_t ldv,
+                                                       const double* __restrict__ coef, double sgn,
+                                                       double* __restrict__ u)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double a = u[i];
+        for (int j = 0; j < m; ++j) a = fma(sgn * coef[j], V[(size_t)j * ldv + i], a);
+        u[i] = a;
+    }
+}
+
+// ------------------------------------------------------------------ row helpers
+__device__ __forceinline__ uint32_t find_tile(const QkTile* __restrict__ tiles, uint32_t ntiles, uint32_t row)
+{
+    uint32_t lo = 0, hi = ntiles;   // last tile with row_start <= row
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tiles[mid].row_start <= row) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename ST>
+__global__ __launch_bounds__(QK_TPB) void k_diagonal(const QkTile* __restrict__ tiles,
+                                                      const uint32_t* __restrict__ lolist,
+                                                      const QkDiagTerm* __restrict__ dg, QkScal sc, int cplx,
+                                                      size_t dim, double* __restrict__ d)
+{
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += (size_t)gridDim.x * blockDim.x) {
+        const QkTile tl = tiles[find_tile(tiles, sc.ntiles, (uint32_t)r)];
+        const uint32_t lo = lolist[tl.lo_base + ((uint32_t)r - tl.row_start)];
+        const ST s = (static_cast<ST>(tl.hi) << sc.NL) | static_cast<ST>(lo);
+        double dre, dim_;
+        eval_diag<ST, true>(dg, sc.ndg, sc.dcr, sc.dci, s, dre, dim_);
+        if (cplx) {
+            d[2 * r] = dre;
+            d[2 * r + 1] = dim_;
+        } else {
+            d[r] = dre;
+        }
+    }
+}
+
+template <typename ST>
+__global__ __launch_bounds__(QK_TPB) void k_states(const QkTile* __restrict__ tiles,
+                                                    const uint32_t* __restrict__ lolist, QkScal sc, size_t r0,
+                                                    size_t n, uint64_t* __restrict__ out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)(r0 + i);
+        const QkTile tl = tiles[find_tile(tiles, sc.ntiles, r)];
+        const uint32_t lo = lolist[tl.lo_base + (r - tl.row_start)];
+        out[i] = (static_cast<uint64_t>(tl.hi) << sc.NL) | lo;
+    }
+}
+
+template <typename ST>
+__global__ __launch_bounds__(QK_TPB) void k_row_entries(const QkTile* __restrict__ tiles,
+                                                         const uint32_t* __restrict__ hoff,
+                                                         const uint32_t* __restrict__ lrank,
+                                                         const uint32_t* __restrict__ lolist,
+                                                         const QkBranch* __restrict__ br,
+                                                         const QkDiagTerm* __restrict__ dg, QkScal sc, size_t r0,
+                                                         size_t n, int cap, int64_t* __restrict__ cols,
+                                                         double* __restrict__ vals, int* __restrict__ counts)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)(r0 + i);
+        const QkTile tl = tiles[find_tile(tiles, sc.ntiles, r)];
+        const uint32_t hi = tl.hi;
+        const uint32_t lo = lolist[tl.lo_base + (r - tl.row_start)];
+        const ST s = (static_cast<ST>(hi) << sc.NL) | static_cast<ST>(lo);
+        const uint32_t my_lr = r - hoff[hi];
+        int k = 0;
+        double dre, dim_;
+        eval_diag<ST, true>(dg, sc.ndg, sc.dcr, sc.dci, s, dre, dim_);
+        if (dre != 0.0 || dim_ != 0.0) {
+            if (k < cap) {
+                cols[i * cap + k] = r;
+                vals[2 * (i * cap + k)] = dre;
+                vals[2 * (i * cap + k) + 1] = dim_;
+            }
+            ++k;
+        }
+        for (int b = 0; b < sc.nbr; ++b) {
+            const QkBranch B = br[b];
+            const uint64_t sm = (uint64_t)s & B.cm;
+            const bool okc = (sm == B.cv) || (B.alt != 0 && sm == (B.cv ^ B.alt));
+            if (!okc) continue;
+            const uint64_t t = (uint64_t)s ^ B.F;
+            const uint32_t thi = (uint32_t)(t >> sc.NL), tlo = (uint32_t)t & sc.lomask;
+            const uint32_t lr = (B.F & sc.lomask) ? lrank[tlo] : my_lr;
+            const bool neg = (__builtin_popcountll((uint64_t)s & B.M) & 1) != 0;
+            if (k < cap) {
+                cols[i * cap + k] = (int64_t)(hoff[thi] + lr);
+                vals[2 * (i * cap + k)] = neg ? -B.re : B.re;
+                vals[2 * (i * cap + k) + 1] = neg ? -B.im : B.im;
+            }
+            ++k;
+        }
+        counts[i] = k;
+    }
+}
+
+template <typename ST>
+__global__ __launch_bounds__(QK_TPB) void k_apply_diag_terms(const QkTile* __restrict__ tiles,
+                                                              const uint32_t* __restrict__ lolist,
+                                                              const QkDiagTerm* __restrict__ dg, QkScal sc,
+                                                              int xcplx, size_t dim, const double* __restrict__ x,
+                                                              double* __restrict__ y)
+{
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < dim; r += (size_t)gridDim.x * blockDim.x) {
+        const QkTile tl = tiles[find_tile(tiles, sc.ntiles, (uint32_t)r)];
+        const uint32_t lo = lolist[tl.lo_base + ((uint32_t)r - tl.row_start)];
+        const ST s = (static_cast<ST>(tl.hi) << sc.NL) | static_cast<ST>(lo);
+        double fr, fi;
+        eval_diag<ST, true>(dg, sc.ndg, sc.dcr, sc.dci, s, fr, fi);
+        // stored diagonal values are conj(amp); O x needs amp = conj(stored)
+        fi = -fi;
+        const double xr = xcplx ? x[2 * r] : x[r];
+        const double xi = xcplx ? x[2 * r + 1] : 0.0;
+        y[2 * r] = fr * xr - fi * xi;
+        y[2 * r + 1] = fr * xi + fi * xr;
+    }
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(QK_TPB) void k_precond(size_t n, const double* __restrict__ r,
+                                                     const double* __restrict__ dg, double lambda,
+                                                     double* __restrict__ t)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (CPLX) {
+            double dr = dg[2 * i] - lambda, di = dg[2 * i + 1];
+            if (sqrt(dr * dr + di * di) < 1e-10) {
+                dr = (dr >= 0) ? 1e-10 : -1e-10;
+                di = 0.0;
+            }
+            const double den = dr * dr + di * di;
+            const double rr = r[2 * i], ri = r[2 * i + 1];
+            t[2 * i] = (rr * dr + ri * di) / den;
+            t[2 * i + 1] = (ri * dr - rr * di) / den;
+        } else {
+            double d = dg[i] - lambda;
+            if (fabs(d) < 1e-10) d = (d >= 0) ? 1e-10 : -1e-10;
+            t[i] = r[i] / d;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launch helpers
+inline int apply_blocks(uint32_t ntiles)
+{
+    uint32_t nb = (ntiles + QK_WPB - 1) / QK_WPB;
+    if (nb > QK_MAX_APPLY_BLOCKS) nb = QK_MAX_APPLY_BLOCKS;
+    if (nb >= 8) nb = (nb + 7u) & ~7u;
+    if (nb == 0) nb = 1;
+    return (int)nb;
+}
+
+inline size_t table_bytes(const QkTables* T) { return (size_t)T->nlo * (T->lrank_u16 ? 2 : 4); }
+
+inline int block_cap(const QkTables* T) { return T->nbr > 256 ? T->nbr : 256; }
+
+inline size_t block_lds(const QkTables* T)
+{
+    return table_bytes(T) + (size_t)QK_WPB * (BlockLds::kHdr + (size_t)block_cap(T) * 12);
+}
+
+inline int grid_for(size_t n)
+{
+    size_t nb = (n + QK_TPB - 1) / QK_TPB;
+    if (nb > QK_BLAS_BLOCKS) nb = QK_BLAS_BLOCKS;
+    return nb == 0 ? 1 : (int)nb;
+}
+
+inline int rc(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
+
+template <typename ST, bool CPLX, bool U16, bool LZ>
+int launch_apply(const QkTables* T, const double* x, double* y, int k, const double* prev, const double* nrm,
+                 double* partials, hipStream_t s)
+{
+    const int nb = apply_blocks(T->ntiles);
+    hipLaunchKernelGGL((k_apply<ST, CPLX, U16, LZ>), dim3(nb), dim3(QK_TPB), table_bytes(T), s, T->tiles, T->hoff,
+                       T->lrank, T->lolist, T->br, T->dg, make_scal(T), x, y, k, prev, nrm, partials);
+    return rc(hipGetLastError());
+}
+
+template <typename ST, bool U16, bool LZ>
+int launch_block(const QkTables* T, const double* X, double* Y, int ld, int c0, int nc, int k, const double* P,
+                 const double* nrm, int ld_sc, double* partials, int npart, hipStream_t s)
+{
+    const int nb = apply_blocks(T->ntiles);
+    if (LZ && npart != nb) return -1;
+    hipLaunchKernelGGL((k_apply_block<ST, U16, LZ>), dim3(nb), dim3(QK_TPB), block_lds(T), s, T->tiles, T->hoff,
+                       T->lrank, T->lolist, T->br, T->dg, make_scal(T), X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
+                       npart, block_cap(T), (int)table_bytes(T));
+    return rc(hipGetLastError());
+}
+
+} // namespace
+
+extern "C" {
+
+int qkl_apply_partials(const QkTables* T) { return apply_blocks(T->ntiles); }
+int qkl_block_partials(const QkTables* T) { return apply_blocks(T->ntiles); }
+int qkl_blas_partials(void) { return QK_BLAS_BLOCKS; }
+
+int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y, int lz_k, const double* prev,
+              const double* nrm, double* partials, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (T->ntiles == 0) return 0;
+    const bool u16 = T->lrank_u16 != 0;
+    const bool lz = lz_k >= 0;
+    if (lz && cplx) return -1;
+#define QK_DISPATCH(ST, C)                                                                                   \
+    if (u16) return lz ? launch_apply<ST, C, true, true>(T, x, y, lz_k, prev, nrm, partials, s)             \
+                       : launch_apply<ST, C, true, false>(T, x, y, lz_k, prev, nrm, partials, s);           \
+    else return lz ? launch_apply<ST, C, false, true>(T, x, y, lz_k, prev, nrm, partials, s)                \
+                   : launch_apply<ST, C, false, false>(T, x, y, lz_k, prev, nrm, partials, s);
+    if (wide) {
+        if (cplx) { QK_DISPATCH(uint64_t, true) } else { QK_DISPATCH(uint64_t, false) }
+    } else {
+        if (cplx) { QK_DISPATCH(uint32_t, true) } else { QK_DISPATCH(uint32_t, false) }
+    }
+#undef QK_DISPATCH
+}
+
+int qkl_apply_block(const QkTables* T, int wide, const double* X, double* Y, int ld, int c0, int nc, int lz_k,
+                    const double* P, const double* nrm, int ld_sc, double* partials, int npart, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (nc <= 0 || nc > 64) return -1;
+    if (T->ntiles == 0) return 0;
+    if (block_lds(T) > 160 * 1024) return -2;
+    const bool u16 = T->lrank_u16 != 0, lz = lz_k >= 0;
+#define QK_BDISPATCH(ST)                                                                                      \
+    if (u16) return lz ? launch_block<ST, true, true>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s) \
+                       : launch_block<ST, true, false>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s); \
+    else return lz ? launch_block<ST, false, true>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s)   \
+                   : launch_block<ST, false, false>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s);
+    if (wide) { QK_BDISPATCH(uint64_t) } else { QK_BDISPATCH(uint32_t) }
+#undef QK_BDISPATCH
+}
+
+int qkl_lanczos_update(int cplx, size_t nrows, int ncol, int ld, double* w, const double* q, const double* alpha,
+                       const double* nrm, int k, int ld_sc, double* partials, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (ncol == 1 && ld == 1) {
+        if (cplx) hipLaunchKernelGGL(k_update1<true>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, alpha, nrm, k, partials);
+        else hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, alpha, nrm, k, partials);
+    } else {
+        if (cplx || ncol > 64) return -1;
+        hipLaunchKernelGGL(k_update_block, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, ncol, ld, w, q, alpha, nrm,
+                           k, ld_sc, partials);
+    }
+    return rc(hipGetLastError());
+}
+
+int qkl_reduce(const double* partials, int npart, int ncol, int group, int mode, int cplx, double* out, int idx,
+               int ld_sc, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (group <= 0 || ncol % group) return -1;
+    hipLaunchKernelGGL(k_reduce, dim3(ncol / group), dim3(QK_TPB), 0, s, partials, npart, group, mode, cplx, out, idx,
+                       ld_sc);
+    return rc(hipGetLastError());
+}
+
+int qkl_dot_partials(int cplx, size_t n, const double* x, const double* y, double* partials, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (cplx) hipLaunchKernelGGL(k_dot<true>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, n, x, y, partials);
+    else hipLaunchKernelGGL(k_dot<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, n, x, y, partials);
+    return rc(hipGetLastError());
+}
+
+int qkl_axpy(int cplx, size_t n, double ar, double ai, const double* x, double* y, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (cplx) hipLaunchKernelGGL(k_axpy<true>, dim3(grid_for(n)), dim3(QK_TPB), 0, s, n, ar, ai, x, y);
+    else hipLaunchKernelGGL(k_axpy<false>, dim3(grid_for(n)), dim3(QK_TPB), 0, s, n, ar, ai, x, y);
+    return rc(hipGetLastError());
+}
+
+int qkl_scal(int cplx, size_t n, double ar, double ai, double* x, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (cplx) hipLaunchKernelGGL(k_scal<true>, dim3(grid_for(n)), dim3(QK_TPB), 0, s, n, ar, ai, x);
+    else hipLaunchKernelGGL(k_scal<false>, dim3(grid_for(n)), dim3(QK_TPB), 0, s, n, ar, ai, x);
+    return rc(hipGetLastError());
+}
+
+int qkl_copy_scaled(size_t n, const double* src, double* dst, const double* sc, int idx, qkl_stream sv)
+{
+    hipLaunchKernelGGL(k_copy_scaled, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, src, dst, sc, idx);
+    return rc(hipGetLastError());
+}
+
+int qkl_fill_random(int kind, size_t nrows, uint64_t seed, uint64_t counter0, int ncol, int ld, int c0, double* x,
+                    qkl_stream sv)
+{
+    hipLaunchKernelGGL(k_fill_random, dim3(grid_for(nrows * (size_t)ncol)), dim3(QK_TPB), 0, (hipStream_t)sv, kind,
+                       nrows, ncol, ld, c0, seed, counter0, x);
+    return rc(hipGetLastError());
+}
+
+int qkl_fill_zero(size_t n, double* x, qkl_stream sv)
+{
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, x);
+    return rc(hipGetLastError());
+}
+
+int qkl_multidot(size_t n, int m, const double* V, size_t ldv, const double* u, double* partials, qkl_stream sv)
+{
+    if (m <= 0) return 0;
+    hipLaunchKernelGGL(k_multidot, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, (hipStream_t)sv, n, m, V, ldv, u, partials);
+    return rc(hipGetLastError());
+}
+
+int qkl_multiaxpy(size_t n, int m, const double* V, size_t ldv, const double* coef, double sgn, double* u,
+                  qkl_stream sv)
+{
+    if (m <= 0) return 0;
+    hipLaunchKernelGGL(k_multiaxpy, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, m, V, ldv, coef, sgn, u);
+    return rc(hipGetLastError());
+}
+
+int qkl_diagonal(const QkTables* T, int cplx, int wide, double* d, qkl_stream sv)
+{
+    const size_t dim = (size_t)T->dim;
+    if (dim == 0) return 0;
+    if (wide) hipLaunchKernelGGL(k_diagonal<uint64_t>, dim3(grid_for(dim)), dim3(QK_TPB), 0, (hipStream_t)sv, T->tiles, T->lolist, T->dg, make_scal(T), cplx, dim, d);
+    else hipLaunchKernelGGL(k_diagonal<uint32_t>, dim3(grid_for(dim)), dim3(QK_TPB), 0, (hipStream_t)sv, T->tiles, T->lolist, T->dg, make_scal(T), cplx, dim, d);
+    return rc(hipGetLastError());
+}
+
+int qkl_row_entries(const QkTables* T, int wide, uint64_t r0, uint64_t nrows, int cap, int64_t* cols, double* vals,
+                    int* counts, qkl_stream sv)
+{
+    if (nrows == 0) return 0;
+    if (wide) hipLaunchKernelGGL(k_row_entries<uint64_t>, dim3(grid_for(nrows)), dim3(QK_TPB), 0, (hipStream_t)sv, T->tiles, T->hoff, T->lrank, T->lolist, T->br, T->dg, make_scal(T), (size_t)r0, (size_t)nrows, cap, cols, vals, counts);
+    else hipLaunchKernelGGL(k_row_entries<uint32_t>, dim3(grid_for(nrows)), dim3(QK_TPB), 0, (hipStream_t)sv, T->tiles, T->hoff, T->lrank, T->lolist, T->br, T->dg, make_scal(T), (size_t)r0, (size_t)nrows, cap, cols, vals, counts);
+    return rc(hipGetLastError());
+}
+
+int qkl_states(const QkTables* T, int wide, uint64_t r0, uint64_t n, uint64_t* out, qkl_stream sv)
+{
+    if (n == 0) return 0;
+    if (wide) hipLaunchKernelGGL(k_states<uint64_t>, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, T->tiles, T->lolist, make_scal(T), (size_t)r0, (size_t)n, out);
+    else hipLaunchKernelGGL(k_states<uint32_t>, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, T->tiles, T->lolist, make_scal(T), (size_t)r0, (size_t)n, out);
+    return rc(hipGetLastError());
+}
+
+int qkl_apply_diag_terms(const QkTables* T, int wide, int xcplx, const double* x, double* y, qkl_stream sv)
+{
+    const size_t dim = (size_t)T->dim;
+    if (dim == 0) return 0;
+    if (wide) hipLaunchKernelGGL(k_apply_diag_terms<uint64_t>, dim3(grid_for(dim)), dim3(QK_TPB), 0, (hipStream_t)sv, T->tiles, T->lolist, T->dg, make_scal(T), xcplx, dim, x, y);
+    else hipLaunchKernelGGL(k_apply_diag_terms<uint32_t>, dim3(grid_for(dim)), dim3(QK_TPB), 0, (hipStream_t)sv, T->tiles, T->lolist, T->dg, make_scal(T), xcplx, dim, x, y);
+    return rc(hipGetLastError());
+}
+
+int qkl_davidson_precond(int cplx, size_t n, const double* r, const double* diag, double lambda, double* t,
+                         qkl_stream sv)
+{
+    if (cplx) hipLaunchKernelGGL(k_precond<true>, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, r, diag, lambda, t);
+    else hipLaunchKernelGGL(k_precond<false>, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, r, diag, lambda, t);
+    return rc(hipGetLastError());
+}
+
+} // extern "C"

@@ -1,0 +1,88 @@
+/* qk_launch.h -- the thin internal C ABI between the C++ host layer and the
+ * HIP kernels (qk_kernels.hip). Plain pointers, sizes and streams only. */
+#ifndef QK_LAUNCH_H
+#define QK_LAUNCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "qk_device.hpp"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* qkl_stream;
+
+/* Number of partial sums (per reduced quantity) the reducing kernels write. */
+int qkl_apply_partials(const qk::QkTables* T);
+int qkl_blas_partials(void);
+
+/* y = H x (pull form). cplx: interleaved complex x/y. wide: 64-bit states.
+ * Lanczos epilogue when lz_k >= 0 (real only):
+ *   w = (H x) / nrm[k] - (k > 0 ? nrm[k] * prev / nrm[k-1] : 0) -> y
+ *   partials[b] = sum_rows (x / nrm[k]) * w        (deterministic order)   */
+int qkl_apply(const qk::QkTables* T, int cplx, int wide, const double* x, double* y,
+              int lz_k, const double* prev, const double* nrm, double* partials, qkl_stream s);
+
+/* Block apply over nc <= 64 real columns: X[row * ld + c0 + c]. Lanczos
+ * epilogue per column when lz_k >= 0 (scalars per column, stride ld_sc):
+ *   w = HX / nrm[k][c] - nrm[k][c] * P / nrm[k-1][c];
+ *   partials[c * npart + b] = sum (X / nrm) * w                             */
+int qkl_apply_block(const qk::QkTables* T, int wide, const double* X, double* Y, int ld, int c0, int nc,
+                    int lz_k, const double* P, const double* nrm, int ld_sc, double* partials, int npart,
+                    qkl_stream s);
+int qkl_block_partials(const qk::QkTables* T);
+
+/* q_{k+1} = w - alpha[k] * q_k / nrm[k]; partials of |q_{k+1}|^2 (ncol columns
+ * interleaved with stride ld; ncol = 1, ld = 1 for a single vector; cplx for
+ * interleaved complex single vector). */
+int qkl_lanczos_update(int cplx, size_t nrows, int ncol, int ld, double* w_inout, const double* q,
+                       const double* alpha, const double* nrm, int k, int ld_sc, double* partials, qkl_stream s);
+
+/* Fixed-order sum of partials. mode 0: out[idx] = sum (real part);
+ * mode 1: out[idx] = sqrt(sum); mode 2: out[idx] = sum, out[idx+1] = sum of
+ * imaginary partials (cplx). For ncol columns (grouped by `group` columns
+ * which share one result): partials[c * npart + b]; results to
+ * out[idx * ld_sc + c]. */
+int qkl_reduce(const double* partials, int npart, int ncol, int group, int mode, int cplx,
+               double* out, int idx, int ld_sc, qkl_stream s);
+
+/* BLAS-1 with deterministic partials (npart = qkl_blas_partials()) */
+int qkl_dot_partials(int cplx, size_t n, const double* x, const double* y, double* partials, qkl_stream s);
+int qkl_axpy(int cplx, size_t n, double are, double aim, const double* x, double* y, qkl_stream s);
+int qkl_scal(int cplx, size_t n, double are, double aim, double* x, qkl_stream s);
+int qkl_copy_scaled(size_t n_doubles, const double* src, double* dst, const double* scale_inv, int idx, qkl_stream s);
+int qkl_fill_random(int kind, size_t n_doubles, uint64_t seed, uint64_t counter0, int ncol, int ld, int c0,
+                    double* x, qkl_stream s);
+int qkl_fill_zero(size_t n_doubles, double* x, qkl_stream s);
+
+/* h[j] partials: partials[j * npart + b] = sum_i V[j][i] u[i], j < m (real) */
+int qkl_multidot(size_t n, int m, const double* V, size_t ldv, const double* u, double* partials, qkl_stream s);
+/* u[i] += sum_j coef[j] * V[j][i]; coef on device (coef_scale: multiply by -1 when negate) */
+int qkl_multiaxpy(size_t n, int m, const double* V, size_t ldv, const double* coef, double coef_sign, double* u,
+                  qkl_stream s);
+
+/* diagonal of H into d (complex interleaved when cplx) */
+int qkl_diagonal(const qk::QkTables* T, int cplx, int wide, double* d, qkl_stream s);
+
+/* entries of rows [r0, r0 + nrows): cols[r * cap + j], vals (complex
+ * interleaved) and counts[r]; diagonal first (col = row), then branches */
+int qkl_row_entries(const qk::QkTables* T, int wide, uint64_t r0, uint64_t nrows, int cap, int64_t* cols,
+                    double* vals, int* counts, qkl_stream s);
+
+/* states of rows [r0, r0 + n) */
+int qkl_states(const qk::QkTables* T, int wide, uint64_t r0, uint64_t n, uint64_t* out, qkl_stream s);
+
+/* diagonal operator: y[row] = f(state) * x[row], f = sum of diag terms
+ * (used to build S^z_q seeds): cplx x/y interleaved. */
+int qkl_apply_diag_terms(const qk::QkTables* T, int wide, int xcplx, const double* x, double* y, qkl_stream s);
+
+/* Davidson preconditioner t = r / (diag - lambda) with the reference clamp */
+int qkl_davidson_precond(int cplx, size_t n, const double* r, const double* diag, double lambda, double* t,
+                         qkl_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

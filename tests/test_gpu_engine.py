@@ -1,0 +1,245 @@
+"""Parity of the HIP engine with the oracle, through the C ABI.
+
+Bit-exact: basis states of every row, and every row's column set and
+signed values against the oracle's CSR (which itself equals the reference's
+Phase-1 output on the fixtures). Floating point: applies within 1e-13
+relative (only the summation order differs), eigenvalues within 1e-10
+relative, FTLM / continued-fraction observables within 1e-8."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import qkrylov_amd as qa
+from qkrylov_amd import models
+
+pytestmark = pytest.mark.gpu
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_fixtures.json")))
+
+BASIS_CLS = {O.SPIN: qa.SpinHalfBasis, O.FERMION: qa.FermionBasis, O.HUBBARD: qa.HubbardBasis, O.TJ: qa.TJBasis}
+SITE_CLS = {O.SPIN: qa.SpinHalfSite, O.FERMION: qa.FermionSite, O.HUBBARD: qa.HubbardSite, O.TJ: qa.TJSite}
+
+
+def engine_basis(kind, n, sec):
+    if kind == O.SPIN:
+        return qa.SpinHalfBasis(n, sz2=sec["sz2"]) if sec.get("use_sz") else qa.SpinHalfBasis(n)
+    if kind == O.FERMION:
+        return qa.FermionBasis(n, n=sec["n"]) if sec.get("use_n") else qa.FermionBasis(n)
+    cls = BASIS_CLS[kind]
+    if sec.get("use_nup") or sec.get("use_ndn"):
+        if sec.get("use_nup") and sec.get("use_ndn"):
+            return cls(n, nup=sec["nup"], ndn=sec["ndn"])
+        pytest.skip("one-species sector is not reachable through the C ABI")
+    return cls(n)
+
+
+def pair(name, split_bits=-1):
+    m = FIX["models"][name]
+    terms = [(complex(*c), [tuple(f) for f in fs]) for c, fs in m["terms"]]
+    om = O.Model(m["bkind"], m["n"], O.sector(**m["sector"]), m["skind"], terms)
+    b = engine_basis(m["bkind"], m["n"], m["sector"])
+    H = qa.MatrixFreeHamiltonian(b, SITE_CLS[m["skind"]](), qa.OpSum(terms), split_bits=split_bits)
+    return H, om
+
+
+CASES = [n for n in sorted(FIX["models"]) if not FIX["models"][n]["sector"].get("use_ndn", 1) == 0 or True]
+
+
+def merged(cols, vals):
+    d = {}
+    for c, v in zip(cols.tolist(), vals.tolist()):
+        d[c] = d.get(c, 0.0) + v
+    return {c: v for c, v in d.items() if v != 0}
+
+
+@pytest.mark.parametrize("name", sorted(FIX["models"]))
+@pytest.mark.parametrize("split", [-1, 2])
+def test_states_and_row_entries_bit_exact(name, split):
+    m = FIX["models"][name]
+    if m["bkind"] in (O.HUBBARD, O.TJ) and split % 2:
+        split += 1
+    H, om = pair(name, split)
+    D = H.dimension
+    assert D == om.dim
+    st = H.states(0, D)
+    ref_states = O.enumerate_basis(m["bkind"], m["n"], O.sector(**m["sector"]))
+    assert np.array_equal(st, ref_states)
+    ro, cols, vals, _ = om.csr()
+    rows = H.row_entries(0, D)
+    for r in range(D):
+        a = merged(*rows[r])
+        b = merged(cols[ro[r]:ro[r + 1]], vals[ro[r]:ro[r + 1]])
+        assert set(a) == set(b), f"row {r}: columns differ"
+        for c in a:
+            assert abs(a[c] - b[c]) <= 1e-15 * max(1.0, abs(b[c])), f"row {r} col {c}: {a[c]} vs {b[c]}"
+
+
+@pytest.mark.parametrize("name", sorted(FIX["models"]))
+def test_apply_matches_oracle(name):
+    H, om = pair(name)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(H.dimension) + 1j * rng.standard_normal(H.dimension)
+    y = H.apply(x)
+    y_ref = om.apply(x)
+    assert np.linalg.norm(y - y_ref) <= 1e-13 * max(np.linalg.norm(y_ref), 1e-300)
+
+
+def test_apply_is_deterministic_and_split_invariant():
+    terms = models.j1j2_square(4, 4, 1.0, 0.5)
+    ys = []
+    for split in (-1, 6, 10, 16):
+        H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(16, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms), split_bits=split)
+        x = np.cos(np.arange(H.dimension) * 0.37) + 0j
+        ys.append(H.apply(x))
+        ys.append(H.apply(x))
+    for y in ys[1:]:
+        assert np.array_equal(y.view(np.float64), ys[0].view(np.float64))
+
+
+@pytest.mark.parametrize("kind,n,sec,terms", [
+    (O.SPIN, 16, dict(use_sz=1, sz2=0), models.j1j2_square(4, 4, 1.0, 0.5)),
+    (O.SPIN, 18, dict(use_sz=1, sz2=2), models.xxz_chain(18, 0.7)),
+    (O.HUBBARD, 8, dict(use_nup=1, nup=4, use_ndn=1, ndn=4), models.hubbard_chain(8, 1.0, 4.0)),
+    (O.HUBBARD, 8, dict(use_nup=1, nup=3, use_ndn=1, ndn=4), models.hubbard_chain(8, 1.0, 2.0, periodic=False)),
+    (O.TJ, 9, dict(use_nup=1, nup=4, use_ndn=1, ndn=3), models.tj_model(3, 3, 1.0, 0.4)),
+    (O.TJ, 8, dict(use_nup=1, nup=3, use_ndn=1, ndn=3), models.tj_model(2, 4, 1.0, 0.4, periodic=True)),
+])
+def test_larger_models_rows_and_apply(kind, n, sec, terms):
+    om = O.Model(kind, n, O.sector(**sec), kind, terms)
+    for split in (-1, 4):
+        H = qa.MatrixFreeHamiltonian(engine_basis(kind, n, sec), SITE_CLS[kind](), qa.OpSum(terms), split_bits=split)
+        assert H.dimension == om.dim
+        sample = np.unique(np.linspace(0, om.dim - 1, 300).astype(np.int64))
+        rows = H.row_entries(0, om.dim)
+        for r in sample:
+            c, v = om.row_entries(int(r))
+            assert merged(*rows[r]) == pytest.approx(merged(c, v), rel=1e-15, abs=1e-15)
+            assert set(merged(*rows[r])) == set(merged(c, v))
+        rng = np.random.default_rng(1)
+        x = rng.standard_normal(om.dim) + 0j
+        assert np.linalg.norm(H.apply(x) - om.apply(x)) <= 1e-13 * np.linalg.norm(om.apply(x))
+
+
+def test_lanczos_matches_oracle_trajectory_and_energy():
+    terms = models.heisenberg_chain(14)
+    om = O.Model(O.SPIN, 14, O.sector(use_sz=1, sz2=0), O.SPIN, terms)
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(14, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms))
+    start = O.start_vector(om.dim, 99)
+    for reorth in (True, False):
+        r = qa.lanczos_ground_state(H, 200, 1e-12, reorthogonalize=reorth, start=start, return_state=True)
+        rr = om.lanczos(200, 1e-12, start=start + 0j, reorth=reorth, want_vec=True)
+        assert abs(r.energy - rr["energy"]) < 1e-10 * abs(rr["energy"])
+        n = min(20, len(rr["alphas"]))
+        assert np.allclose(r.alphas[:n], rr["alphas"][:n], rtol=1e-11, atol=1e-11)
+        assert np.allclose(r.betas[:n], rr["betas"][:n], rtol=1e-10, atol=1e-11)
+        if reorth:
+            assert r.iterations == rr["iterations"] and r.converged == rr["converged"]
+        v = r.eigenvector
+        hv = H.apply(v + 0j).real
+        assert np.linalg.norm(hv - r.energy * v) < 1e-6
+        assert abs(np.linalg.norm(v) - 1) < 1e-12
+
+
+def test_lanczos_known_answers_and_maxiter():
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(4), qa.SpinHalfSite(), qa.OpSum(models.heisenberg_chain(4)))
+    r = qa.lanczos_ground_state(H, maxiter=50, tol=1e-12)
+    assert abs(r.energy + 2.0) < 1e-12 and r.converged
+    r = qa.lanczos_ground_state(H, maxiter=2, tol=1e-15)
+    assert r.iterations == 2 and not r.converged
+    H4 = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(4, sz2=0), qa.SpinHalfSite(),
+                                  qa.OpSum(models.heisenberg_chain(4, periodic=False)))
+    assert abs(qa.lanczos_ground_state(H4).energy + 1.6160254037844386) < 1e-12
+
+
+def test_lanczos_complex_hamiltonian():
+    terms = models.heisenberg_chain(10) + [(0.3, [("Sy", i), ("Sy", (i + 2) % 10)]) for i in range(10)] + \
+        [(0.2j, [("Sp", i), ("Sm", (i + 3) % 10)]) for i in range(10)] + \
+        [(-0.2j, [("Sm", i), ("Sp", (i + 3) % 10)]) for i in range(10)]
+    om = O.Model(O.SPIN, 10, O.sector(use_sz=1, sz2=0), O.SPIN, terms)
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(10, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms))
+    assert not H.is_real
+    ex = np.linalg.eigvalsh(om.dense())[0]
+    r = qa.lanczos_ground_state(H, 300, 1e-13)
+    assert abs(r.energy - ex) < 1e-10 * abs(ex)
+
+
+def _rademacher(seed, r, D):
+    L = O.lib()
+    return np.array([-1.0 if (L.qo_rng_u64(seed, r * D + i) >> 63) else 1.0 for i in range(D)])
+
+
+def test_ftlm_batched_vs_sequential_vs_oracle():
+    terms = models.tj_model(2, 4, 1.0, 0.4, periodic=False)
+    sec = dict(use_nup=1, nup=3, use_ndn=1, ndn=3)
+    H = qa.MatrixFreeHamiltonian(qa.TJBasis(8, nup=3, ndn=3), qa.TJSite(), qa.OpSum(terms))
+    om = O.Model(O.TJ, 8, O.sector(**sec), O.TJ, terms)
+    betas = np.linspace(0.2, 20, 16)
+    R, M = 12, 40
+    b = qa.ftlm_batched(H, R, M, betas, seed=42)
+    s = qa.ftlm_batched(H, R, M, betas, seed=42, sequential=True)
+    for key in ("Z", "E", "Cv"):
+        assert np.allclose(b[key], s[key], rtol=1e-8, atol=1e-10)
+    rv = np.array([_rademacher(42, r, om.dim) for r in range(R)]) + 0j
+    for i, beta in enumerate(betas[::5]):
+        Z, E, Cv = om.ftlm(beta, rv, M)
+        j = i * 5
+        assert abs(b["Z"][j] - Z) < 1e-8 * Z and abs(b["E"][j] - E) < 1e-8 * max(1, abs(E))
+        assert abs(b["Cv"][j] - Cv) < 1e-7 * max(1, abs(Cv))
+
+
+def test_ftlm_reference_entry_point():
+    """ftlm() keeps the reference's complex normal vectors; with H = Sz0 Sz1
+    on two sites the exact Z is 4 cosh(beta/4) times E|r|^2 / D = 2."""
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(2), qa.SpinHalfSite(), qa.OpSum([(1.0, [("Sz", 0), ("Sz", 1)])]))
+    r = qa.ftlm(H, 1.0, 200, 10)
+    assert r.partition_function > 0 and abs(r.internal_energy) < 1.0
+    assert abs(r.partition_function / (2 * 4 * math.cosh(0.25)) - 1) < 0.25
+
+
+def test_cf_single_and_batched_vs_oracle():
+    terms = models.heisenberg_chain(12)
+    om = O.Model(O.SPIN, 12, O.sector(use_sz=1, sz2=0), O.SPIN, terms)
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(12, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms))
+    rng = np.random.default_rng(7)
+    seeds = rng.standard_normal((5, om.dim)) + 1j * rng.standard_normal((5, om.dim))
+    batched, _ = qa.continued_fraction_batched(H, seeds, 60)
+    for k in range(5):
+        al, be, nrm = om.cf(seeds[k], 60)
+        single = qa.continued_fraction_coeffs(H, seeds[k], 60)
+        for res in (single, batched[k]):
+            assert abs(res.norm_phi0 - nrm) < 1e-12 * nrm
+            n = min(len(al), len(res.alphas), 30)
+            assert np.allclose(res.alphas[:n], al[:n], rtol=1e-9, atol=1e-9)
+            for om_ in (0.5, 1.5):
+                E0 = -5.0
+                s1 = qa.evaluate_spectral_function(res, om_, E0, 0.2)
+                s0 = O.spectral(al, be, nrm, om_, E0, 0.2)
+                assert abs(s1 - s0) < 1e-8 * max(1.0, abs(s0))
+
+
+def test_davidson_vs_dense():
+    terms = models.heisenberg_chain(10)
+    om = O.Model(O.SPIN, 10, O.sector(use_sz=1, sz2=0), O.SPIN, terms)
+    w = np.linalg.eigvalsh(om.dense())
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(10, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms))
+    r = qa.davidson_lowest(H, n_eig=2, max_subspace=20, tol=1e-9)
+    assert r.converged
+    assert abs(r.eigenvalues[0] - w[0]) < 1e-10 * abs(w[0])
+    v = r.eigenvectors[0]
+    assert np.linalg.norm(H.apply(v) - r.eigenvalues[0] * v) < 1e-6
+
+
+def test_operator_apply_is_O_not_Odagger():
+    terms = [(0.7 + 0.2j, [("Sp", 0), ("Sm", 2)]), (0.3, [("Sz", 1)])]
+    b = qa.SpinHalfBasis(4, sz2=0)
+    op = qa.MatrixFreeHamiltonian(b, qa.SpinHalfSite(), qa.OpSum(terms), operator=True)
+    hm = qa.MatrixFreeHamiltonian(b, qa.SpinHalfSite(), qa.OpSum(terms))
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(6) + 1j * rng.standard_normal(6)
+    y = rng.standard_normal(6) + 1j * rng.standard_normal(6)
+    # <y, O x> = <O^dagger y, x>
+    assert abs(np.vdot(y, op.apply(x)) - np.vdot(hm.apply(y), x)) < 1e-14
