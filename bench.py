@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calib-gib", type=float, default=0.0,
+                    help="also run one plain copy kernel of this many GiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part of this "
@@ -139,6 +141,14 @@ def main():
     from qkrylov_amd import _lib
     L = _lib.load()
     w = WORKLOADS[args.workload]
+    if args.copy_calib_gib > 0:
+        n = int(args.copy_calib_gib * (1 << 30) / 8)
+        a, b = qa.DeviceArray(n), qa.DeviceArray(n)
+        _lib.check(L.qkrylov_amd_fill_random(a.ptr, n, 3, 0), "fill")
+        _lib.check(L.qkrylov_amd_copy(b.ptr, a.ptr, n), "copy")
+        log(f"[bench] copy calibration: {n * 8} bytes read + {n * 8} bytes written by k_copy")
+        a.free()
+        b.free()
     t0 = time.perf_counter()
     H, terms = make_hamiltonian(args.workload)
     t_setup = time.perf_counter() - t0

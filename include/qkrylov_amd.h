@@ -28,6 +28,9 @@ int qkrylov_amd_synchronize(void);
 /* x[i] = uniform[-1,1) (kind 0) or +-1 (kind 1) from the counter RNG
  * (splitmix64 of seed and index i); on the library stream, synchronous */
 int qkrylov_amd_fill_random(double* x_dev, size_t n, uint64_t seed, int kind);
+/* dst[i] = src[i] for n doubles with a plain streaming kernel (synchronous);
+ * also the reference kernel for calibrating HBM byte counters */
+int qkrylov_amd_copy(double* dst_dev, const double* src_dev, size_t n);
 
 /* ---- Hamiltonian construction options and info ---- */
 /* NL: number of low state bits ranked per lane (-1: automatic) */

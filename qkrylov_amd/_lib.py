@@ -142,6 +142,7 @@ SIGNATURES = {
     "qkrylov_amd_memcpy_d2h": (c_int, [c_vp, c_vp, c_size]),
     "qkrylov_amd_synchronize": (c_int, []),
     "qkrylov_amd_fill_random": (c_int, [c_vp, c_size, c_u64, c_int]),
+    "qkrylov_amd_copy": (c_int, [c_vp, c_vp, c_size]),
     "qkrylov_hamiltonian_create_ex": (c_vp, [c_vp, c_vp, c_vp, c_int]),
     "qkrylov_operator_create": (c_vp, [c_vp, c_vp, c_vp]),
     "qkrylov_hamiltonian_info": (c_int, [c_vp, P(HamInfo)]),

@@ -575,6 +575,16 @@ int qkrylov_amd_synchronize(void)
 {
     return hipDeviceSynchronize() == hipSuccess ? QKRYLOV_SUCCESS : QKRYLOV_ERROR_EXCEPTION;
 }
+int qkrylov_amd_copy(double* dst, const double* src, size_t n)
+{
+    if ((!dst || !src) && n) return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    hipStream_t s = qk::default_stream();
+    if (qkl_copy(n, src, dst, s) != 0) throw std::runtime_error("copy launch failed");
+    qk::hip_check(hipStreamSynchronize(s), "sync");
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
 int qkrylov_amd_fill_random(double* x, size_t n, uint64_t seed, int kind)
 {
     if (!x || (kind != 0 && kind != 1)) return QKRYLOV_ERROR_INVALID_ARG;

@@ -559,6 +559,12 @@ __global__ __launch_bounds__(QK_TPB) void k_copy_scaled(size_t n, const double* 
         dst[i] = src[i] * inv;
 }
 
+__global__ __launch_bounds__(QK_TPB) void k_copy(size_t n, const double* __restrict__ src, double* __restrict__ dst)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(QK_TPB) void k_fill_zero(size_t n, double* __restrict__ x)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -918,6 +924,12 @@ int qkl_fill_random(int kind, size_t nrows, uint64_t seed, uint64_t counter0, in
 {
     hipLaunchKernelGGL(k_fill_random, dim3(grid_for(nrows * (size_t)ncol)), dim3(QK_TPB), 0, (hipStream_t)sv, kind,
                        nrows, ncol, ld, c0, seed, counter0, x);
+    return rc(hipGetLastError());
+}
+
+int qkl_copy(size_t n, const double* src, double* dst, qkl_stream sv)
+{
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, src, dst);
     return rc(hipGetLastError());
 }
 

@@ -56,6 +56,7 @@ int qkl_copy_scaled(size_t n_doubles, const double* src, double* dst, const doub
 int qkl_fill_random(int kind, size_t n_doubles, uint64_t seed, uint64_t counter0, int ncol, int ld, int c0,
                     double* x, qkl_stream s);
 int qkl_fill_zero(size_t n_doubles, double* x, qkl_stream s);
+int qkl_copy(size_t n_doubles, const double* src, double* dst, qkl_stream s);
 
 /* h[j] partials: partials[j * npart + b] = sum_i V[j][i] u[i], j < m (real) */
 int qkl_multidot(size_t n, int m, const double* V, size_t ldv, const double* u, double* partials, qkl_stream s);
