@@ -51,7 +51,7 @@ def build_terms(name):
     raise ValueError(name)
 
 
-def make_hamiltonian(name):
+def make_hamiltonian(name, split_bits=-1):
     import qkrylov_amd as qa
     w = WORKLOADS[name]
     terms = build_terms(name)
@@ -61,7 +61,7 @@ def make_hamiltonian(name):
         basis = qa.HubbardBasis(w["n"], nup=8, ndn=8)
     else:
         basis = qa.TJBasis(w["n"], nup=9, ndn=9)
-    return qa.MatrixFreeHamiltonian(basis, None, qa.OpSum(terms)), terms
+    return qa.MatrixFreeHamiltonian(basis, None, qa.OpSum(terms), split_bits=split_bits), terms
 
 
 def oracle_sector(name):
@@ -119,12 +119,100 @@ def cpu_baseline(name, terms, D, budget_s):
                 rows_per_s=round(rate_rows, 1))
 
 
+def bench_ftlm(args):
+    """Batched FTLM + S^zz(q, omega) on the t-J 4x5 open lattice (2 holes):
+    R = 64 Rademacher vectors, M = 100 steps, 64 temperatures T = 0.05..5,
+    one multi-vector recurrence vs the same 64 vectors run one at a time;
+    then the ground state and 20 plane-wave S^z_q seeds as one 40-column
+    block continued fraction (100 steps, eta = 0.05)."""
+    import numpy as np
+
+    import qkrylov_amd as qa
+    from qkrylov_amd import models
+    R, M = args.ftlm_R, 100
+    T = np.linspace(0.05, 5.0, 64)
+    betas = 1.0 / T
+    H, terms = make_hamiltonian("tj4x5")
+    D = H.dimension
+    log(f"[bench-ftlm] D={D} R={R} M={M}")
+    b = qa.ftlm_batched(H, R, M, betas, seed=42)
+    log(f"[bench-ftlm] batched: {b['seconds']:.3f}s")
+    seq = None
+    if not args.no_sequential:
+        seq = qa.ftlm_batched(H, R, M, betas, seed=42, sequential=True)
+        log(f"[bench-ftlm] sequential: {seq['seconds']:.3f}s")
+    rel = None
+    if seq is not None:
+        rel = max(float(np.max(np.abs(b[k] - seq[k]) / np.maximum(np.abs(seq[k]), 1e-300))) for k in ("Z", "E", "Cv"))
+    # ground state and S^zz(q, omega) seeds
+    g = qa.lanczos_ground_state(H, maxiter=300, tol=1e-10, reorthogonalize=False, return_state=True)
+    psi = g.eigenvector
+    basis = qa.TJBasis(20, nup=9, ndn=9)
+    seeds = []
+    for q in models.lattice_momenta(4, 5):
+        op = qa.MatrixFreeHamiltonian(basis, qa.TJSite(), qa.OpSum(models.szq_terms(4, 5, q)), operator=True)
+        seeds.append(op.apply(psi + 0j))
+        del op
+    cf, cf_s = qa.continued_fraction_batched(H, np.array(seeds), 100)
+    omega = np.linspace(0.0, 4.0, 201)
+    S = [[qa.evaluate_spectral_function(r, w, g.energy, 0.05) for w in omega] for r in cf]
+    log(f"[bench-ftlm] E0={g.energy:.10f} ({g.iterations} it), CF 20 q in {cf_s:.3f}s")
+    out = {
+        "metric": "batched FTLM solves/s (R=64 vectors x M=100 steps, FP64)",
+        "value": round(R / b["seconds"], 3), "unit": "random-vector solves/s", "n_gpus": 1, "steps": M,
+        "warmup": 0, "ms_per_step": round(1e3 * b["seconds"] / M, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (Rademacher vectors from the counter RNG)",
+        "config": {"workload": WORKLOADS["tj4x5"]["desc"], "D": D, "R": R, "M": M, "temperatures": 64},
+        "batched_seconds": round(b["seconds"], 4),
+        "sequential_seconds": round(seq["seconds"], 4) if seq else None,
+        "speedup_vs_sequential": round(seq["seconds"] / b["seconds"], 3) if seq else None,
+        "max_rel_diff_batched_vs_sequential": rel,
+        "Z_E_Cv_at_T=0.05,1,5": [[float(b[k][i]) for k in ("Z", "E", "Cv")] for i in (0, 12, 63)],
+        "ground_state": {"energy": g.energy, "iterations": g.iterations, "seconds": g.report["loop_seconds"]},
+        "szz_cf": {"q_points": 20, "steps": 100, "seconds": round(cf_s, 4), "eta": 0.05,
+                   "S_peak_q_pi": float(max(S[len(S) // 2]))},
+        "roofline": {"bound": "hbm", "achieved": round(16.0 * D * R * M / b["seconds"] / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(16.0 * D * R * M / b["seconds"] / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
+                     "note": "block recurrence wall time incl. reductions; 16 B x R per row per step"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Serving stream of XXZ-ring solves: Poisson arrivals at 50 req/s,
+    L in 12..28, Delta in [0.5, 1.5], |Sz| <= 2, 70 % ground states
+    (tol 1e-10), 30 % ground state + S^zz(pi, omega) (100 steps)."""
+    from qkrylov_amd import serving
+    reqs = serving.make_trace(args.serve_n, rate=50.0, seed=2024)
+    out, tot = serving.run(reqs, realtime=True, max_active=args.serve_active)
+    log(f"[bench-serve] {len(reqs)} requests: {tot}")
+    res = {
+        "metric": "serving solves/s with p50/p99 latency (FP64)",
+        "value": round(tot["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": len(reqs), "warmup": 0,
+        "ms_per_step": round(1e3 * tot["wall_s"] / len(reqs), 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic seeded Poisson trace",
+        "config": {"workload": "XXZ ring serving stream, 50 req/s Poisson, L 12..28", "requests": len(reqs),
+                   "max_active": args.serve_active},
+        "p50_latency_s": round(tot["p50_s"], 5), "p99_latency_s": round(tot["p99_s"], 5),
+        "wall_s": round(tot["wall_s"], 3),
+        "converged": int(sum(o["converged"] for o in out)),
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="j1j2_4x8", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="j1j2_4x8", choices=sorted(WORKLOADS) + ["ftlm_tj4x5", "serve_xxz"])
+    ap.add_argument("--split-bits", type=int, default=-1,
+                    help="low state bits ranked per lane (-1: the engine's cost model)")
+    ap.add_argument("--ftlm-R", type=int, default=64)
+    ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--serve-n", type=int, default=300)
+    ap.add_argument("--serve-active", type=int, default=32)
     ap.add_argument("--lanczos-iters", type=int, default=None)
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -140,6 +228,10 @@ def main():
     import qkrylov_amd as qa
     from qkrylov_amd import _lib
     L = _lib.load()
+    if args.workload == "ftlm_tj4x5":
+        return bench_ftlm(args)
+    if args.workload == "serve_xxz":
+        return bench_serve(args)
     w = WORKLOADS[args.workload]
     if args.copy_calib_gib > 0:
         n = int(args.copy_calib_gib * (1 << 30) / 8)
@@ -150,7 +242,7 @@ def main():
         a.free()
         b.free()
     t0 = time.perf_counter()
-    H, terms = make_hamiltonian(args.workload)
+    H, terms = make_hamiltonian(args.workload, args.split_bits)
     t_setup = time.perf_counter() - t0
     D = H.dimension
     log(f"[bench] {args.workload}: D={D} branches={H.info['n_offdiag_branches']} "

@@ -34,13 +34,26 @@ struct QkTile {
     uint32_t hi, lo_base, row_start, count;
 };
 
+// A branch split at the lo/hi boundary NL, as the apply kernels use it:
+// the h* fields act on hi = s >> NL (wave-uniform inside a tile), the l*
+// fields on lo = s & ((1 << NL) - 1) (per lane). Branches whose lo fields
+// are all zero ("hi-only") come first: their condition, target block and
+// sign are uniform, and their gather is one coalesced row segment.
+struct QkBranchK {
+    uint32_t hcm, hcv, halt, hF, hM;
+    uint32_t lcm, lcv, lalt, lF, lM;
+    double re, im;
+};
+
 struct QkTables {
     const QkTile* tiles;
     const uint32_t* hoff;     // [2^NH]  rows before the block of each hi
     const uint32_t* lrank;    // [2^NL]  rank of lo inside its class
     const uint32_t* lolist;   // ascending valid lo values, grouped by class
-    const QkBranch* br;       // off-diagonal branches
+    const QkBranch* br;       // off-diagonal branches (full-width form)
+    const QkBranchK* brk;     // the same branches split at NL: nA hi-only, then nB others (padded)
     const QkDiagTerm* dg;     // diagonal terms
+    int32_t nA, nB;           // padded counts (multiples of the kernel's unroll)
     uint64_t lomask;
     uint64_t dim;             // number of basis rows
     double dconst_re, dconst_im;

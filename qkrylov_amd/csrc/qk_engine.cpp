@@ -10,6 +10,8 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 
@@ -40,7 +42,10 @@ DevBuf::~DevBuf() { reset(); }
 
 void DevBuf::reset()
 {
-    if (p_) (void)hipFree(p_);
+    if (p_) {
+        if (async_) (void)hipFreeAsync(p_, stream_);
+        else (void)hipFree(p_);
+    }
     p_ = nullptr;
     bytes_ = 0;
 }
@@ -51,6 +56,8 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept
         reset();
         p_ = o.p_;
         bytes_ = o.bytes_;
+        stream_ = o.stream_;
+        async_ = o.async_;
         o.p_ = nullptr;
         o.bytes_ = 0;
     }
@@ -77,12 +84,61 @@ void ensure_device(int id)
     hip_check(hipSetDevice(id), "hipSetDevice");
 }
 
-template <class T>
-static DevBuf upload(const std::vector<T>& v)
+DevBuf::DevBuf(size_t bytes, hipStream_t async_stream) : bytes_(bytes), stream_(async_stream), async_(true)
 {
-    DevBuf b(std::max<size_t>(v.size() * sizeof(T), 8));
-    if (!v.empty()) hip_check(hipMemcpy(b.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    if (bytes == 0) return;
+    hipError_t e = hipMallocAsync(&p_, bytes, async_stream);
+    if (e != hipSuccess) {
+        p_ = nullptr;
+        throw std::runtime_error("stream-ordered allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                 hipGetErrorString(e));
+    }
+}
+
+template <class T>
+static DevBuf upload(const std::vector<T>& v, hipStream_t s)
+{
+    const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 8);
+    DevBuf b = s ? DevBuf(bytes, s) : DevBuf(bytes);
+    if (!v.empty()) {
+        if (s) {
+            hip_check(hipMemcpyAsync(b.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+            hip_check(hipStreamSynchronize(s), "upload sync");
+        } else {
+            hip_check(hipMemcpy(b.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+        }
+    }
     return b;
+}
+
+// ---------------------------------------------------------------- shared basis tables
+
+std::shared_ptr<const BasisTables> basis_tables(const BasisSpec& b, int NL)
+{
+    static std::mutex mu;
+    static std::map<std::string, std::weak_ptr<const BasisTables>> cache;
+    const Sector& s = b.sector();
+    const std::string key = std::to_string((int)b.kind()) + "/" + std::to_string(b.nsites()) + "/" +
+                            std::to_string(b.target_a()) + "/" + std::to_string(b.target_b()) + "/" +
+                            std::to_string(s.use_nup || s.use_ndn || s.use_sz || s.use_n) + "/" + std::to_string(NL);
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end())
+        if (auto sp = it->second.lock()) return sp;
+    auto t = std::make_shared<BasisTables>();
+    t->host = build_tables(b, NL);
+    t->tiles = upload(t->host.tiles, nullptr);
+    t->hoff = upload(t->host.hoff, nullptr);
+    t->lrank = upload(t->host.lrank, nullptr);
+    t->lolist = upload(t->host.lolist, nullptr);
+    // the host copies of the large arrays are not needed after upload
+    t->ntiles = t->host.tiles.size();
+    std::vector<QkTile>().swap(t->host.tiles);
+    std::vector<uint32_t>().swap(t->host.hoff);
+    std::vector<uint32_t>().swap(t->host.lolist);
+    std::vector<uint32_t>().swap(t->host.lrank);
+    cache[key] = t;
+    return t;
 }
 
 // ---------------------------------------------------------------- Engine
@@ -91,31 +147,60 @@ Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpS
     : basis_(std::move(basis))
 {
     if (basis_->dim() > 0) comp_ = compile_opsum(*basis_, site_kind, ops);
-    const int NL = opt.NL >= 0 ? opt.NL : choose_NL(*basis_);
-    host_ = build_tables(*basis_, NL);
-    tiles_ = upload(host_.tiles);
-    hoff_ = upload(host_.hoff);
-    lrank_ = upload(host_.lrank);
-    lolist_ = upload(host_.lolist);
-    br_ = upload(comp_.offdiag);
-    dg_ = upload(comp_.diag);
-    T_.tiles = static_cast<const QkTile*>(tiles_.get());
-    T_.hoff = static_cast<const uint32_t*>(hoff_.get());
-    T_.lrank = static_cast<const uint32_t*>(lrank_.get());
-    T_.lolist = static_cast<const uint32_t*>(lolist_.get());
+    const int NL = opt.NL >= 0 ? opt.NL : choose_NL_model(*basis_, comp_);
+    tables_ = basis_tables(*basis_, NL);
+    br_ = upload(comp_.offdiag, opt.stream);
+    dg_ = upload(comp_.diag, opt.stream);
+    // branches split at NL: hi-only first (padded to 4), then the rest
+    // (padded to 8); padding branches can never be taken (hcm = 0, hcv = 1)
+    {
+        const int nl = tables_->host.NL;
+        const uint64_t lm = (nl >= 64) ? ~0ULL : ((1ULL << nl) - 1ULL);
+        std::vector<QkBranchK> A, B;
+        for (const auto& b : comp_.offdiag) {
+            QkBranchK k;
+            k.hcm = (uint32_t)(b.cm >> nl);
+            k.hcv = (uint32_t)(b.cv >> nl);
+            k.halt = (uint32_t)(b.alt >> nl);
+            k.hF = (uint32_t)(b.F >> nl);
+            k.hM = (uint32_t)(b.M >> nl);
+            k.lcm = (uint32_t)(b.cm & lm);
+            k.lcv = (uint32_t)(b.cv & lm);
+            k.lalt = (uint32_t)(b.alt & lm);
+            k.lF = (uint32_t)(b.F & lm);
+            k.lM = (uint32_t)(b.M & lm);
+            k.re = b.re;
+            k.im = b.im;
+            const bool hi_only = (k.lcm | k.lcv | k.lalt | k.lF | k.lM) == 0;
+            (hi_only ? A : B).push_back(k);
+        }
+        const QkBranchK pad{0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0};
+        while (A.size() % 4) A.push_back(pad);
+        while (B.size() % 8) B.push_back(pad);
+        T_.nA = (int32_t)A.size();
+        T_.nB = (int32_t)B.size();
+        A.insert(A.end(), B.begin(), B.end());
+        brk_ = upload(A, opt.stream);
+    }
+    T_.tiles = static_cast<const QkTile*>(tables_->tiles.get());
+    T_.hoff = static_cast<const uint32_t*>(tables_->hoff.get());
+    T_.lrank = static_cast<const uint32_t*>(tables_->lrank.get());
+    T_.lolist = static_cast<const uint32_t*>(tables_->lolist.get());
     T_.br = static_cast<const QkBranch*>(br_.get());
+    T_.brk = static_cast<const QkBranchK*>(brk_.get());
     T_.dg = static_cast<const QkDiagTerm*>(dg_.get());
-    T_.lomask = (host_.NL >= 64) ? ~0ULL : ((1ULL << host_.NL) - 1ULL);
+    T_.lomask = (tables_->host.NL >= 64) ? ~0ULL : ((1ULL << tables_->host.NL) - 1ULL);
     T_.dim = static_cast<uint64_t>(basis_->dim());
     T_.dconst_re = comp_.dconst.real();
     T_.dconst_im = comp_.dconst.imag();
-    T_.ntiles = static_cast<uint32_t>(host_.tiles.size());
+    T_.ntiles = static_cast<uint32_t>(tables_->ntiles);
     T_.nbr = static_cast<int32_t>(comp_.offdiag.size());
     T_.ndg = static_cast<int32_t>(comp_.diag.size());
-    T_.NL = host_.NL;
-    T_.lrank_u16 = host_.lrank_u16 ? 1 : 0;
-    T_.nlo = 1 << host_.NL;
-    table_bytes_ = tiles_.bytes() + hoff_.bytes() + lrank_.bytes() + lolist_.bytes() + br_.bytes() + dg_.bytes();
+    T_.NL = tables_->host.NL;
+    T_.lrank_u16 = tables_->host.lrank_u16 ? 1 : 0;
+    T_.nlo = 1 << tables_->host.NL;
+    table_bytes_ = tables_->tiles.bytes() + tables_->hoff.bytes() + tables_->lrank.bytes() + tables_->lolist.bytes() +
+                   br_.bytes() + brk_.bytes() + dg_.bytes();
 }
 
 void Engine::apply(const double* x, double* y, bool cplx, hipStream_t s) const
@@ -249,110 +334,82 @@ std::complex<double> dot_host(bool cplx, size_t n, const double* x, const double
     return {v[0], cplx ? v[1] : 0.0};
 }
 
-// reference replay of the convergence logic of lanczos_ground_state
-// (src/solvers/lanczos.cpp:135-183)
-struct Replay {
-    int kmax;
-    double tol;
-    bool fixed;
-    std::vector<double> alphas, betas;
-    double e_old = 1e100;
-    int next = 0;
-    bool done = false, converged = false;
+} // namespace
 
-    // feed step `it` with alpha_it and beta_it (norm after step it)
-    void feed(double a, double b)
-    {
-        const int it = next++;
-        alphas.push_back(a);
-        if (b < 1e-15) {
+void LanczosReplay::feed(double a, double b)
+{
+    const int it = next++;
+    alphas.push_back(a);
+    if (b < 1e-15) {
+        converged = true;
+        done = true;
+        return;
+    }
+    if (it + 1 == kmax) {
+        done = true;
+        return;
+    }
+    betas.push_back(b);
+    if (!fixed && it > 0) {
+        const double e = tridiag_min(alphas.data(), betas.data(), (int)alphas.size(), nullptr);
+        if (std::abs(e - e_old) < tol) {
+            e_old = e;
             converged = true;
             done = true;
             return;
         }
-        if (it + 1 == kmax) {
-            done = true;
-            return;
-        }
-        betas.push_back(b);
-        if (!fixed && it > 0) {
-            const double e = tridiag_min(alphas.data(), betas.data(), (int)alphas.size(), nullptr);
-            if (std::abs(e - e_old) < tol) {
-                e_old = e;
-                converged = true;
-                done = true;
-                return;
-            }
-            e_old = e;
-        }
+        e_old = e;
     }
-};
+}
 
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// device state of the fused real recurrence
-struct RealCore {
-    const Engine& E;
-    hipStream_t s;
-    size_t D;
-    int kcap;
-    DevBuf Q[3];
-    DevBuf alpha, nrm, hcoef, partA, partB, partM;
-    int npA, NB;
-    int ip = 0, ic = 1, iw = 2;
-
-    RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth)
-        : E(e), s(st), D((size_t)e.dim()), kcap(kcap_)
-    {
-        for (auto& q : Q) q = DevBuf(D * sizeof(double));
-        alpha = DevBuf((kcap + 2) * sizeof(double));
-        nrm = DevBuf((kcap + 3) * sizeof(double));
-        npA = qkl_apply_partials(E.tables());
-        NB = qkl_blas_partials();
-        partA = DevBuf(npA * sizeof(double));
-        partB = DevBuf(NB * sizeof(double));
-        if (reorth) {
-            hcoef = DevBuf((kcap + 2) * sizeof(double));
-            partM = DevBuf((size_t)NB * (kcap + 2) * sizeof(double));
-        }
+RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool so)
+    : E(e), s(st), D((size_t)e.dim()), kcap(kcap_)
+{
+    auto mk = [&](size_t bytes) { return so ? DevBuf(bytes, st) : DevBuf(bytes); };
+    for (auto& q : Q) q = mk(D * sizeof(double));
+    alpha = mk((kcap + 2) * sizeof(double));
+    nrm = mk((kcap + 3) * sizeof(double));
+    npA = qkl_apply_partials(E.tables());
+    NB = qkl_blas_partials();
+    partA = mk(npA * sizeof(double));
+    partB = mk(NB * sizeof(double));
+    if (reorth) {
+        hcoef = mk((kcap + 2) * sizeof(double));
+        partM = mk((size_t)NB * (kcap + 2) * sizeof(double));
     }
+}
 
-    double* cur() { return Q[ic].d(); }
-    double* prv() { return Q[ip].d(); }
-    double* wrk() { return Q[iw].d(); }
+void RealCore::init_norm()
+{
+    lcheck(qkl_fill_zero(D, prv(), s), "zero");
+    lcheck(qkl_dot_partials(0, D, cur(), cur(), partB.d(), s), "dot");
+    lcheck(qkl_reduce(partB.d(), NB, 1, 1, 1, 0, nrm.d(), 0, 1, s), "reduce");
+}
 
-    void init_norm()
-    {
-        lcheck(qkl_fill_zero(D, prv(), s), "zero");
-        lcheck(qkl_dot_partials(0, D, cur(), cur(), partB.d(), s), "dot");
-        lcheck(qkl_reduce(partB.d(), NB, 1, 1, 1, 0, nrm.d(), 0, 1, s), "reduce");
+// one step k: w = H v_k - beta v_{k-1}; alpha_k; q_{k+1}; nrm[k+1]
+void RealCore::step(int k, const double* basis)
+{
+    lcheck(qkl_apply(E.tables(), 0, E.wide() ? 1 : 0, cur(), wrk(), k, prv(), nrm.d(), partA.d(), s), "apply");
+    lcheck(qkl_reduce(partA.d(), npA, 1, 1, 0, 0, alpha.d(), k, 1, s), "reduce alpha");
+    lcheck(qkl_lanczos_update(0, D, 1, 1, wrk(), cur(), alpha.d(), nrm.d(), k, 1, partB.d(), s), "update");
+    if (basis) {
+        // classical Gram-Schmidt against v_0..v_k (stored normalised)
+        lcheck(qkl_multidot(D, k + 1, basis, D, wrk(), partM.d(), s), "multidot");
+        lcheck(qkl_reduce(partM.d(), NB, k + 1, 1, 0, 0, hcoef.d(), 0, 0, s), "reduce h");
+        lcheck(qkl_multiaxpy(D, k + 1, basis, D, hcoef.d(), -1.0, wrk(), s), "multiaxpy");
+        lcheck(qkl_dot_partials(0, D, wrk(), wrk(), partB.d(), s), "dot");
     }
-
-    // one step k: w = H v_k - beta v_{k-1}; alpha_k; q_{k+1}; nrm[k+1]
-    void step(int k, const double* basis)
-    {
-        lcheck(qkl_apply(E.tables(), 0, E.wide() ? 1 : 0, cur(), wrk(), k, prv(), nrm.d(), partA.d(), s), "apply");
-        lcheck(qkl_reduce(partA.d(), npA, 1, 1, 0, 0, alpha.d(), k, 1, s), "reduce alpha");
-        lcheck(qkl_lanczos_update(0, D, 1, 1, wrk(), cur(), alpha.d(), nrm.d(), k, 1, partB.d(), s), "update");
-        if (basis) {
-            // classical Gram-Schmidt against v_0..v_k (stored normalised)
-            lcheck(qkl_multidot(D, k + 1, basis, D, wrk(), partM.d(), s), "multidot");
-            lcheck(qkl_reduce(partM.d(), NB, k + 1, 1, 0, 0, hcoef.d(), 0, 0, s), "reduce h");
-            lcheck(qkl_multiaxpy(D, k + 1, basis, D, hcoef.d(), -1.0, wrk(), s), "multiaxpy");
-            lcheck(qkl_dot_partials(0, D, wrk(), wrk(), partB.d(), s), "dot");
-        }
-        lcheck(qkl_reduce(partB.d(), NB, 1, 1, 1, 0, nrm.d(), k + 1, 1, s), "reduce beta");
-        const int t = ip;
-        ip = ic;
-        ic = iw;
-        iw = t;
-    }
-};
-
-} // namespace
+    lcheck(qkl_reduce(partB.d(), NB, 1, 1, 1, 0, nrm.d(), k + 1, 1, s), "reduce beta");
+    const int t = ip;
+    ip = ic;
+    ic = iw;
+    iw = t;
+}
 
 // ---------------------------------------------------------------- Lanczos (real, fused)
 
@@ -386,7 +443,7 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
     C.init_norm();
     if (store) lcheck(qkl_copy_scaled((size_t)D, C.cur(), basis.d(), C.nrm.d(), 0, s), "store v0");
 
-    Replay rp{kmax, o.tol, o.fixed_steps, {}, {}};
+    LanczosReplay rp{kmax, o.tol, o.fixed_steps, {}, {}};
     int check = o.check_every > 0 ? o.check_every : (D < (1 << 22) ? 8 : 1);
     if (o.fixed_steps) check = kmax;
     std::vector<double> ha(kmax + 2), hn(kmax + 3);
@@ -471,7 +528,7 @@ static LanczosOutput lanczos_complex(const Engine& E, const LanczosOptions& o, h
         hip_check(hipMemcpyAsync(basis.back().get(), v, 2 * D * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
     };
     if (store) push_basis(vc.d());
-    Replay rp{kmax, o.tol, o.fixed_steps, {}, {}};
+    LanczosReplay rp{kmax, o.tol, o.fixed_steps, {}, {}};
     const double t0 = now_s();
     for (int it = 0; it < kmax && !rp.done; ++it) {
         E.apply(vc.d(), w.d(), true, s);

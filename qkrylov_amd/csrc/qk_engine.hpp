@@ -19,10 +19,17 @@ class DevBuf {
 public:
     DevBuf() = default;
     explicit DevBuf(size_t bytes);
+    // stream-ordered allocation: hipMallocAsync now, hipFreeAsync on release,
+    // so freeing never waits for work queued on other streams
+    DevBuf(size_t bytes, hipStream_t async_stream);
     ~DevBuf();
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; o.bytes_ = 0; }
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_), stream_(o.stream_), async_(o.async_)
+    {
+        o.p_ = nullptr;
+        o.bytes_ = 0;
+    }
     DevBuf& operator=(DevBuf&& o) noexcept;
     void* get() const { return p_; }
     double* d() const { return static_cast<double*>(p_); }
@@ -32,14 +39,27 @@ public:
 private:
     void* p_ = nullptr;
     size_t bytes_ = 0;
+    hipStream_t stream_ = nullptr;
+    bool async_ = false;
 };
 
 // one library stream per host thread (default priority)
 hipStream_t default_stream();
 void ensure_device(int id);
 
+// Device copies of the rank / unrank tables of a basis (tiles, hoff, lrank,
+// lolist). They depend only on the basis and the bit split, so every
+// Hamiltonian over the same basis shares one copy.
+struct BasisTables {
+    HostTables host;   // NL, NH, lrank_u16 (the large host arrays are released)
+    DevBuf tiles, hoff, lrank, lolist;
+    size_t ntiles = 0;
+};
+std::shared_ptr<const BasisTables> basis_tables(const BasisSpec& b, int NL);
+
 struct EngineOptions {
-    int NL = -1;   // lo-bit split (-1: automatic)
+    int NL = -1;                   // lo-bit split (-1: automatic)
+    hipStream_t stream = nullptr;  // non-null: stream-ordered term-table allocation
 };
 
 class Engine {
@@ -52,7 +72,7 @@ public:
     const QkTables* tables() const { return &T_; }
     const BasisSpec& basis() const { return *basis_; }
     const Compiled& compiled() const { return comp_; }
-    int NL() const { return host_.NL; }
+    int NL() const { return tables_->host.NL; }
     size_t table_bytes() const { return table_bytes_; }
 
     // y = H^dagger-convention apply of the reference (y[a] = sum conj(amp) x[b])
@@ -65,10 +85,47 @@ public:
 private:
     std::shared_ptr<const BasisSpec> basis_;
     Compiled comp_;
-    HostTables host_;
-    DevBuf tiles_, hoff_, lrank_, lolist_, br_, dg_;
+    std::shared_ptr<const BasisTables> tables_;
+    DevBuf br_, brk_, dg_;
     QkTables T_{};
     size_t table_bytes_ = 0;
+};
+
+double now_s();
+
+// Replays the reference's convergence logic of lanczos_ground_state
+// (src/solvers/lanczos.cpp:135-183) on alpha_k and beta_k = |q_{k+1}|.
+struct LanczosReplay {
+    int kmax;
+    double tol;
+    bool fixed;
+    std::vector<double> alphas, betas;
+    double e_old = 1e100;
+    int next = 0;
+    bool done = false, converged = false;
+    void feed(double a, double b);
+};
+
+// Device state of the fused real three-term recurrence: three rotating
+// vectors q (stored unnormalised with their norms nrm[k] on the device),
+// alpha[k], and the partial-sum buffers. One step = fused apply + alpha
+// reduction + update + norm reduction, all on stream s, no host sync.
+struct RealCore {
+    const Engine& E;
+    hipStream_t s;
+    size_t D;
+    int kcap;
+    DevBuf Q[3];
+    DevBuf alpha, nrm, hcoef, partA, partB, partM;
+    int npA, NB;
+    int ip = 0, ic = 1, iw = 2;
+
+    RealCore(const Engine& e, int kcap, hipStream_t st, bool reorth, bool stream_ordered = false);
+    double* cur() { return Q[ic].d(); }
+    double* prv() { return Q[ip].d(); }
+    double* wrk() { return Q[iw].d(); }
+    void init_norm();                        // nrm[0] = |cur|, prv = 0
+    void step(int k, const double* basis);   // basis: normalised v_0..v_k for reorthogonalisation
 };
 
 // ---------------------------------------------------------------- tridiagonal

@@ -21,14 +21,14 @@ using namespace qk;
 #define QK_TPB 256
 #define QK_WPB 4
 #define QK_BLAS_BLOCKS 1024
-#define QK_MAX_APPLY_BLOCKS 1024
+#define QK_MAX_APPLY_BLOCKS 2048
 
 namespace {
 
 struct QkScal {
     uint32_t lomask;
     uint32_t ntiles;
-    int32_t nbr, ndg, NL, nlo;
+    int32_t nbr, ndg, NL, nlo, nA, nB;
     double dcr, dci;
 };
 
@@ -37,6 +37,8 @@ QkScal make_scal(const QkTables* T)
     QkScal s;
     s.lomask = static_cast<uint32_t>(T->lomask);
     s.ntiles = T->ntiles;
+    s.nA = T->nA;
+    s.nB = T->nB;
     s.nbr = T->nbr;
     s.ndg = T->ndg;
     s.NL = T->NL;
@@ -113,6 +115,35 @@ __device__ __forceinline__ void load_lrank(unsigned char* smem, const uint32_t* 
     __syncthreads();
 }
 
+// acc += sum_u c_u * x[rt_u], all U loads issued before the first FMA;
+// the FMAs run in u order, so the sum order is the branch order
+template <bool CPLX, int U>
+__device__ __forceinline__ void accumulate(const double* __restrict__ x, const uint32_t* rt, const double* cr,
+                                           const double* ci, double& ar, double& ai)
+{
+    if (CPLX) {
+        double tr[U], ti[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            tr[u] = x[2 * (size_t)rt[u]];
+            ti[u] = x[2 * (size_t)rt[u] + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ar = fma(cr[u], tr[u], ar);
+            ar = fma(-ci[u], ti[u], ar);
+            ai = fma(cr[u], ti[u], ai);
+            ai = fma(ci[u], tr[u], ai);
+        }
+    } else {
+        double t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) t[u] = x[rt[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) ar = fma(cr[u], t[u], ar);
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -124,10 +155,11 @@ template <typename ST, bool CPLX, bool U16, bool LZ>
 __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
                                                    const uint32_t* __restrict__ lrank,
                                                    const uint32_t* __restrict__ lolist,
-                                                   const QkBranch* __restrict__ br, const QkDiagTerm* __restrict__ dg,
-                                                   QkScal sc, const double* __restrict__ x, double* __restrict__ y,
-                                                   int lz_k, const double* __restrict__ prev,
-                                                   const double* __restrict__ nrm, double* __restrict__ partials)
+                                                   const QkBranchK* __restrict__ brk,
+                                                   const QkDiagTerm* __restrict__ dg, QkScal sc,
+                                                   const double* __restrict__ x, double* __restrict__ y, int lz_k,
+                                                   const double* __restrict__ prev, const double* __restrict__ nrm,
+                                                   double* __restrict__ partials)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double wsum[QK_WPB];
@@ -169,52 +201,62 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
             ar = dre * xr;
         }
 
-        for (int b = 0; b < sc.nbr; ++b) {
-            const QkBranch B = br[b];
-            const uint32_t hcm = static_cast<uint32_t>(B.cm >> NL);
-            const uint32_t hcv = static_cast<uint32_t>(B.cv >> NL);
-            const uint32_t hm = hi & hcm;
-            const bool u0 = (hm == hcv);
-            const bool u1 = (B.alt != 0) && (hm == (hcv ^ static_cast<uint32_t>(B.alt >> NL)));
-            if (!(u0 || u1)) continue;
-            const uint32_t base_t = hoff[hi ^ static_cast<uint32_t>(B.F >> NL)];
-            const int hpar = __builtin_popcount(hi & static_cast<uint32_t>(B.M >> NL)) & 1;
-            const uint32_t lcm = static_cast<uint32_t>(B.cm) & lomask;
-            const uint32_t lcv = static_cast<uint32_t>(B.cv) & lomask;
-            const uint32_t lalt = static_cast<uint32_t>(B.alt) & lomask;
-            const uint32_t lF = static_cast<uint32_t>(B.F) & lomask;
-            const uint32_t lM = static_cast<uint32_t>(B.M) & lomask;
-            const uint32_t lf = lo & lcm;
-            const bool ok = (u0 && lf == lcv) || (u1 && lf == (lcv ^ lalt));
-            if (ok) {
-                const uint32_t lr = lF ? lds_rank<U16>(smem, lo ^ lF) : my_lr;
-                const uint32_t rt = base_t + lr;
-                const bool neg = ((__builtin_popcount(lo & lM) & 1) ^ hpar) != 0;
-                const double vr = neg ? -B.re : B.re;
-                if (CPLX) {
-                    const double vi = neg ? -B.im : B.im;
-                    const double tr = x[2 * (size_t)rt], ti = x[2 * (size_t)rt + 1];
-                    ar = fma(vr, tr, ar);
-                    ar = fma(-vi, ti, ar);
-                    ai = fma(vr, ti, ai);
-                    ai = fma(vi, tr, ai);
-                } else {
-                    ar = fma(vr, x[rt], ar);
-                }
+        // hi-only branches: condition, target block and sign are uniform;
+        // the gather is the tile's own lo-rank range in the target block.
+        // Groups of 4 issue their loads before accumulating; an untaken
+        // branch reads the row's own x with a zero coefficient.
+        for (int g = 0; g < sc.nA; g += 4) {
+            uint32_t rt[4];
+            double cr[4], ci[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const QkBranchK B = brk[g + u];
+                const uint32_t hm = hi & B.hcm;
+                const bool take = (hm == B.hcv) || (B.halt != 0 && hm == (B.hcv ^ B.halt));
+                const bool neg = (__builtin_popcount(hi & B.hM) & 1) != 0;
+                rt[u] = take ? hoff[hi ^ B.hF] + my_lr : row;
+                cr[u] = take ? (neg ? -B.re : B.re) : 0.0;
+                ci[u] = take ? (neg ? -B.im : B.im) : 0.0;
             }
+            accumulate<CPLX, 4>(x, rt, cr, ci, ar, ai);
+        }
+        // remaining branches: per-lane condition on lo, lo rank from LDS
+        // (lF == 0 reads lrank[lo] = the row's own rank), groups of 8
+        for (int g = sc.nA; g < sc.nA + sc.nB; g += 8) {
+            uint32_t rt[8];
+            double cr[8], ci[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const QkBranchK B = brk[g + u];
+                const uint32_t hm = hi & B.hcm;
+                const bool u0 = (hm == B.hcv);
+                const bool u1 = (B.halt != 0) && (hm == (B.hcv ^ B.halt));
+                const uint32_t base_t = hoff[hi ^ B.hF];
+                const int hpar = __builtin_popcount(hi & B.hM) & 1;
+                const uint32_t lf = lo & B.lcm;
+                const bool ok = (u0 && lf == B.lcv) || (u1 && lf == (B.lcv ^ B.lalt));
+                const uint32_t lr = lds_rank<U16>(smem, lo ^ B.lF);
+                const bool neg = ((__builtin_popcount(lo & B.lM) & 1) ^ hpar) != 0;
+                rt[u] = ok ? base_t + lr : row;
+                cr[u] = ok ? (neg ? -B.re : B.re) : 0.0;
+                ci[u] = ok ? (neg ? -B.im : B.im) : 0.0;
+            }
+            accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
         }
 
+        // y is written once and not read again by this kernel: stream it
+        // past the caches so L2 keeps x
         if (active) {
             if (LZ) {
                 double w = ar * inv_n;
-                if (lz_k > 0) w = fma(-cprev, prev[row], w);
-                y[row] = w;
+                if (lz_k > 0) w = fma(-cprev, __builtin_nontemporal_load(prev + row), w);
+                __builtin_nontemporal_store(w, y + row);
                 part = fma(xr * inv_n, w, part);
             } else if (CPLX) {
-                y[2 * (size_t)row] = ar;
-                y[2 * (size_t)row + 1] = ai;
+                __builtin_nontemporal_store(ar, y + 2 * (size_t)row);
+                __builtin_nontemporal_store(ai, y + 2 * (size_t)row + 1);
             } else {
-                y[row] = ar;
+                __builtin_nontemporal_store(ar, y + row);
             }
         }
     }
@@ -802,7 +844,7 @@ int launch_apply(const QkTables* T, const double* x, double* y, int k, const dou
 {
     const int nb = apply_blocks(T->ntiles);
     hipLaunchKernelGGL((k_apply<ST, CPLX, U16, LZ>), dim3(nb), dim3(QK_TPB), table_bytes(T), s, T->tiles, T->hoff,
-                       T->lrank, T->lolist, T->br, T->dg, make_scal(T), x, y, k, prev, nrm, partials);
+                       T->lrank, T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials);
     return rc(hipGetLastError());
 }
 

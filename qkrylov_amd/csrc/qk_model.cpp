@@ -2,6 +2,7 @@
 #include "qk_model.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <tuple>
 #include <unordered_map>
@@ -427,6 +428,64 @@ Compiled compile_opsum(const BasisSpec& basis, Kind sk, const OpSum& ops)
 }
 
 // ---------------------------------------------------------------- tables
+
+double tile_utilisation(const BasisSpec& b, int NL)
+{
+    const int nb = b.nbits(), NH = nb - NL;
+    const bool two = !b.spinless(), tj = b.no_double();
+    auto cnt = [&](int bits, int a, int c) -> double {
+        if (a < 0 || c < 0) return 0.0;
+        if (!two) return c == 0 ? (double)C(bits, a) : 0.0;
+        const int n = bits / 2;
+        if (tj) return (double)C(n, a) * (double)C(n - a, c);
+        return (double)C(n, a) * (double)C(n, c);
+    };
+    const int amax = two ? NL / 2 : NL, hmax = two ? NH / 2 : NH;
+    auto size_lo = [&](int ar, int br) {
+        double s = 0.0;
+        for (int a = 0; a <= amax; ++a)
+            for (int c = 0; c <= (two ? amax : 0); ++c)
+                if ((ar == -1 || a == ar) && (br == -1 || c == br)) s += cnt(NL, a, c);
+        return s;
+    };
+    const int A = b.target_a(), B = b.target_b();
+    double rows = 0.0, tiles = 0.0;
+    for (int a = 0; a <= hmax; ++a)
+        for (int c = 0; c <= (two ? hmax : 0); ++c) {
+            const double nh = cnt(NH, a, c);
+            if (nh == 0.0) continue;
+            const int ar = A == -1 ? -1 : A - a, br = B == -1 ? -1 : B - c;
+            if ((A != -1 && (ar < 0 || ar > amax)) || (B != -1 && (br < 0 || br > amax))) continue;
+            const double s = size_lo(ar, br);
+            rows += nh * s;
+            tiles += nh * std::ceil(s / 64.0);
+        }
+    return tiles > 0 ? rows / (64.0 * tiles) : 1.0;
+}
+
+int choose_NL_model(const BasisSpec& basis, const Compiled& comp)
+{
+    const int nb = basis.nbits();
+    if (nb <= 8) return nb - ((!basis.spinless() && (nb & 1)) ? 1 : 0);
+    int best = choose_NL(basis);
+    double best_cost = 1e300;
+    for (int NL = std::min(nb, 8); NL <= std::min(nb, 15); ++NL) {
+        if (!basis.spinless() && (NL & 1)) continue;
+        if (nb - NL > 28) continue;
+        const uint64_t lomask = (1ULL << NL) - 1ULL;
+        double w = 0.5 * comp.diag.size();
+        for (const auto& br : comp.offdiag) {
+            const bool hi_only = ((br.cm | br.cv | br.alt | br.F | br.M) & lomask) == 0;
+            w += hi_only ? 1.0 : 5.0;
+        }
+        const double cost = w / tile_utilisation(basis, NL);
+        if (cost < best_cost * 0.999) {
+            best_cost = cost;
+            best = NL;
+        }
+    }
+    return best;
+}
 
 int choose_NL(const BasisSpec& basis)
 {

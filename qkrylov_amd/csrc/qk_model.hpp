@@ -115,5 +115,9 @@ struct HostTables {
 
 HostTables build_tables(const BasisSpec& basis, int NL);
 int choose_NL(const BasisSpec& basis);
+// fraction of lanes holding a row when rows are tiled per hi block
+double tile_utilisation(const BasisSpec& basis, int NL);
+// split minimising (uniform + 5 x per-lane branch work) / utilisation
+int choose_NL_model(const BasisSpec& basis, const Compiled& comp);
 
 } // namespace qk

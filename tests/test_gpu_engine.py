@@ -94,10 +94,13 @@ def test_apply_is_deterministic_and_split_invariant():
     for split in (-1, 6, 10, 16):
         H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(16, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms), split_bits=split)
         x = np.cos(np.arange(H.dimension) * 0.37) + 0j
-        ys.append(H.apply(x))
-        ys.append(H.apply(x))
+        y1, y2 = H.apply(x), H.apply(x)
+        # same configuration: bitwise identical
+        assert np.array_equal(y1.view(np.float64), y2.view(np.float64))
+        ys.append(y1)
+    # different splits order the branch sums differently: equal to rounding
     for y in ys[1:]:
-        assert np.array_equal(y.view(np.float64), ys[0].view(np.float64))
+        assert np.linalg.norm(y - ys[0]) <= 1e-14 * np.linalg.norm(ys[0])
 
 
 @pytest.mark.parametrize("kind,n,sec,terms", [
