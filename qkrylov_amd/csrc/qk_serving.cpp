@@ -124,6 +124,7 @@ void start_job(Job& j)
     j.basis = std::make_shared<BasisSpec>(Kind::SpinHalf, j.rq.L, sec);
     EngineOptions eo;
     eo.stream = j.s;
+    eo.NL = choose_NL(*j.basis);   // the split whose tables the run keeps resident
     j.H = std::make_unique<Engine>(j.basis, Kind::SpinHalf, xxz_ring(j.rq.L, j.rq.delta), eo);
     j.kcap = (int)std::min<int64_t>(j.rq.maxiter, j.basis->dim());
     const int cap = std::max(j.kcap, j.rq.cf_steps) + 2;
@@ -183,6 +184,7 @@ void advance(Job& j, qkrylov_serve_result_t& res, double* cf_a, double* cf_b, in
         sec.sz2 = j.rq.sz2;
         EngineOptions eo;
         eo.stream = j.s;
+        eo.NL = choose_NL(*j.basis);
         j.Op = std::make_unique<Engine>(j.basis, Kind::SpinHalf, szpi(j.rq.L), eo);
         j.Op->apply(j.ritz.d(), j.C->cur(), false, j.s);
         if (qkl_dot_partials(0, D, j.ritz.d(), j.ritz.d(), j.C->partB.d(), j.s) != 0) throw std::runtime_error("dot");
