@@ -230,7 +230,8 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
                 const QkBranchK B = brk[g + u];
                 const uint32_t hm = hi & B.hcm;
                 const bool u0 = (hm == B.hcv);
-                const bool u1 = (B.halt != 0) && (hm == (B.hcv ^ B.halt));
+                // a merged pair may have its alternative entirely in lo
+                const bool u1 = ((B.halt | B.lalt) != 0) && (hm == (B.hcv ^ B.halt));
                 const uint32_t base_t = hoff[hi ^ B.hF];
                 const int hpar = __builtin_popcount(hi & B.hM) & 1;
                 const uint32_t lf = lo & B.lcm;
@@ -274,36 +275,66 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
 
 // ------------------------------------------------------------------ block apply
 // Phase A (lane = row): decode up to 64 rows of a tile into per-row lists of
-// (target row, value) in LDS. Phase B (lane = column): for each row, stream
-// the list and gather a contiguous segment of the target row of X. Each
-// decoded matrix element is reused by all nc columns.
+// (target row, value id) in LDS; value ids index a table of the signed
+// branch values. Phase B (lane = column): for each row, stream the list and
+// gather a contiguous segment of the target row of X, four entries in
+// flight. Each decoded matrix element is reused by all nc columns.
 struct BlockLds {
     static constexpr int kHdr = 64 * 8 + 64 * 4 + 64 * 4;   // diag, off, cnt
+    static constexpr int kEntry = 6;                         // uint32 target + uint16 value id
 };
+
+// per-lane branch test shared by both passes; returns the target row and
+// signed value id when branch g+u applies to the lane's state
+template <bool U16>
+__device__ __forceinline__ bool branch_hit(const QkBranchK& B, uint32_t hi, uint32_t lo, uint32_t my_lr,
+                                           const uint32_t* __restrict__ hoff, const unsigned char* smem, int b,
+                                           uint32_t& tgt, uint32_t& vid)
+{
+    const uint32_t hm = hi & B.hcm;
+    const bool u0 = (hm == B.hcv);
+    const bool u1 = ((B.halt | B.lalt) != 0) && (hm == (B.hcv ^ B.halt));
+    const uint32_t lf = lo & B.lcm;
+    const bool ok = (u0 && lf == B.lcv) || (u1 && lf == (B.lcv ^ B.lalt));
+    if (ok) {
+        const uint32_t lr = B.lF ? lds_rank<U16>(smem, lo ^ B.lF) : my_lr;
+        tgt = hoff[hi ^ B.hF] + lr;
+        const int neg = (__builtin_popcount(hi & B.hM) + __builtin_popcount(lo & B.lM)) & 1;
+        vid = 2u * (uint32_t)b + (uint32_t)neg;
+    }
+    return ok;
+}
 
 template <typename ST, bool U16, bool LZ>
 __global__ __launch_bounds__(QK_TPB) void k_apply_block(
     const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff, const uint32_t* __restrict__ lrank,
-    const uint32_t* __restrict__ lolist, const QkBranch* __restrict__ br, const QkDiagTerm* __restrict__ dg, QkScal sc,
-    const double* __restrict__ X, double* __restrict__ Y, int ld, int c0, int nc, int lz_k,
+    const uint32_t* __restrict__ lolist, const QkBranchK* __restrict__ brk, const QkDiagTerm* __restrict__ dg,
+    QkScal sc, const double* __restrict__ X, double* __restrict__ Y, int ld, int c0, int nc, int lz_k,
     const double* __restrict__ P, const double* __restrict__ nrm, int ld_sc, double* __restrict__ partials,
     int npart, int cap, int table_bytes)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double wsum[QK_WPB][64];
+    const int nbk = sc.nA + sc.nB;
+    double* l_vals = reinterpret_cast<double*>(smem + table_bytes);
+    for (int i = threadIdx.x; i < nbk; i += blockDim.x) {
+        l_vals[2 * i] = brk[i].re;
+        l_vals[2 * i + 1] = -brk[i].re;
+    }
     load_lrank<U16>(smem, lrank, sc.nlo);
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = threadIdx.x >> 6;
-    unsigned char* wbase = smem + table_bytes + (size_t)wv * (BlockLds::kHdr + (size_t)cap * 12);
+    const size_t vals_bytes = ((size_t)nbk * 16 + 15) & ~(size_t)15;
+    unsigned char* wbase = smem + table_bytes + vals_bytes +
+                           (size_t)wv * (BlockLds::kHdr + (((size_t)cap * BlockLds::kEntry + 15) & ~(size_t)15));
     double* l_diag = reinterpret_cast<double*>(wbase);
     uint32_t* l_off = reinterpret_cast<uint32_t*>(wbase + 64 * 8);
     uint32_t* l_cnt = reinterpret_cast<uint32_t*>(wbase + 64 * 8 + 64 * 4);
-    double* l_coef = reinterpret_cast<double*>(wbase + BlockLds::kHdr);
-    uint32_t* l_tgt = reinterpret_cast<uint32_t*>(wbase + BlockLds::kHdr + (size_t)cap * 8);
+    uint32_t* l_tgt = reinterpret_cast<uint32_t*>(wbase + BlockLds::kHdr);
+    uint16_t* l_vid = reinterpret_cast<uint16_t*>(wbase + BlockLds::kHdr + (size_t)cap * 4);
 
     const int NL = sc.NL;
-    const uint32_t lomask = sc.lomask;
     const bool colok = lane < (uint32_t)nc;
     const int col = c0 + (int)(colok ? lane : 0u);
     double inv_n = 1.0, cprev = 0.0;
@@ -330,20 +361,11 @@ __global__ __launch_bounds__(QK_TPB) void k_apply_block(
         double dre, dim;
         eval_diag<ST, false>(dg, sc.ndg, sc.dcr, sc.dci, s, dre, dim);
         uint32_t n_j = 0;
-        for (int b = 0; b < sc.nbr; ++b) {
-            const QkBranch B = br[b];
-            const uint32_t hcm = static_cast<uint32_t>(B.cm >> NL), hcv = static_cast<uint32_t>(B.cv >> NL);
-            const uint32_t hm = hi & hcm;
-            const bool u0 = (hm == hcv);
-            const bool u1 = (B.alt != 0) && (hm == (hcv ^ static_cast<uint32_t>(B.alt >> NL)));
-            if (!(u0 || u1)) continue;
-            const uint32_t lf = lo & (static_cast<uint32_t>(B.cm) & lomask);
-            const uint32_t lcv = static_cast<uint32_t>(B.cv) & lomask;
-            const bool ok = (u0 && lf == lcv) || (u1 && lf == (lcv ^ (static_cast<uint32_t>(B.alt) & lomask)));
-            n_j += ok ? 1u : 0u;
+        for (int b = 0; b < nbk; ++b) {
+            uint32_t tg, vd;
+            n_j += branch_hit<U16>(brk[b], hi, lo, my_lr, hoff, smem, b, tg, vd) ? 1u : 0u;
         }
         if (!active) n_j = 0;
-        // exclusive scan over the wave
         uint32_t incl = n_j;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t o = __shfl_up(incl, off, 64);
@@ -364,26 +386,11 @@ __global__ __launch_bounds__(QK_TPB) void k_apply_block(
             // ---- phase A, pass 2: write the entries of rows [j0, j1)
             if (inr) {
                 uint32_t k = excl - base;
-                for (int b = 0; b < sc.nbr; ++b) {
-                    const QkBranch B = br[b];
-                    const uint32_t hcm = static_cast<uint32_t>(B.cm >> NL), hcv = static_cast<uint32_t>(B.cv >> NL);
-                    const uint32_t hm = hi & hcm;
-                    const bool u0 = (hm == hcv);
-                    const bool u1 = (B.alt != 0) && (hm == (hcv ^ static_cast<uint32_t>(B.alt >> NL)));
-                    if (!(u0 || u1)) continue;
-                    const uint32_t lf = lo & (static_cast<uint32_t>(B.cm) & lomask);
-                    const uint32_t lcv = static_cast<uint32_t>(B.cv) & lomask;
-                    const bool ok =
-                        (u0 && lf == lcv) || (u1 && lf == (lcv ^ (static_cast<uint32_t>(B.alt) & lomask)));
-                    if (ok) {
-                        const uint32_t base_t = hoff[hi ^ static_cast<uint32_t>(B.F >> NL)];
-                        const int hpar = __builtin_popcount(hi & static_cast<uint32_t>(B.M >> NL)) & 1;
-                        const uint32_t lF = static_cast<uint32_t>(B.F) & lomask;
-                        const uint32_t lr = lF ? lds_rank<U16>(smem, lo ^ lF) : my_lr;
-                        const bool neg =
-                            ((__builtin_popcount(lo & (static_cast<uint32_t>(B.M) & lomask)) & 1) ^ hpar) != 0;
-                        l_coef[k] = neg ? -B.re : B.re;
-                        l_tgt[k] = base_t + lr;
+                for (int b = 0; b < nbk; ++b) {
+                    uint32_t tg, vd;
+                    if (branch_hit<U16>(brk[b], hi, lo, my_lr, hoff, smem, b, tg, vd)) {
+                        l_tgt[k] = tg;
+                        l_vid[k] = (uint16_t)vd;
                         ++k;
                     }
                 }
@@ -396,7 +403,17 @@ __global__ __launch_bounds__(QK_TPB) void k_apply_block(
                     const double xs = X[row * ld + col];
                     double acc = l_diag[j] * xs;
                     const uint32_t e0 = l_off[j] - base, e1 = e0 + l_cnt[j];
-                    for (uint32_t e = e0; e < e1; ++e) acc = fma(l_coef[e], X[(size_t)l_tgt[e] * ld + col], acc);
+                    uint32_t e = e0;
+                    for (; e + 4 <= e1; e += 4) {
+                        const uint32_t ta = l_tgt[e], tb = l_tgt[e + 1], tc = l_tgt[e + 2], td = l_tgt[e + 3];
+                        const double xa = X[(size_t)ta * ld + col], xb = X[(size_t)tb * ld + col];
+                        const double xc = X[(size_t)tc * ld + col], xd = X[(size_t)td * ld + col];
+                        acc = fma(l_vals[l_vid[e]], xa, acc);
+                        acc = fma(l_vals[l_vid[e + 1]], xb, acc);
+                        acc = fma(l_vals[l_vid[e + 2]], xc, acc);
+                        acc = fma(l_vals[l_vid[e + 3]], xd, acc);
+                    }
+                    for (; e < e1; ++e) acc = fma(l_vals[l_vid[e]], X[(size_t)l_tgt[e] * ld + col], acc);
                     if (LZ) {
                         double w = acc * inv_n;
                         if (lz_k > 0) w = fma(-cprev, P[row * ld + col], w);
@@ -820,13 +837,27 @@ inline int apply_blocks(uint32_t ntiles)
     return (int)nb;
 }
 
-inline size_t table_bytes(const QkTables* T) { return (size_t)T->nlo * (T->lrank_u16 ? 2 : 4); }
+// LDS bytes of the lo-rank table, rounded to 16 so what follows it is aligned
+inline size_t table_bytes(const QkTables* T) { return ((size_t)T->nlo * (T->lrank_u16 ? 2 : 4) + 15) & ~(size_t)15; }
 
-inline int block_cap(const QkTables* T) { return T->nbr > 256 ? T->nbr : 256; }
+inline size_t block_vals_bytes(const QkTables* T) { return ((size_t)(T->nA + T->nB) * 16 + 15) & ~(size_t)15; }
+
+// entry capacity per wave: as many as fit two workgroups per CU (160 KB LDS),
+// never fewer than one row's worth of branches
+inline int block_cap(const QkTables* T)
+{
+    const long budget = 78 * 1024 - (long)table_bytes(T) - (long)block_vals_bytes(T) - QK_WPB * BlockLds::kHdr;
+    long cap = budget / (QK_WPB * BlockLds::kEntry);
+    cap = cap > 2048 ? 2048 : cap;
+    const long need = T->nA + T->nB;
+    if (cap < need) cap = need;
+    return (int)(cap & ~7L) + (int)((cap & 7L) ? 8 : 0);
+}
 
 inline size_t block_lds(const QkTables* T)
 {
-    return table_bytes(T) + (size_t)QK_WPB * (BlockLds::kHdr + (size_t)block_cap(T) * 12);
+    return table_bytes(T) + block_vals_bytes(T) +
+           (size_t)QK_WPB * (BlockLds::kHdr + (((size_t)block_cap(T) * BlockLds::kEntry + 15) & ~(size_t)15));
 }
 
 inline int grid_for(size_t n)
@@ -855,7 +886,7 @@ int launch_block(const QkTables* T, const double* X, double* Y, int ld, int c0, 
     const int nb = apply_blocks(T->ntiles);
     if (LZ && npart != nb) return -1;
     hipLaunchKernelGGL((k_apply_block<ST, U16, LZ>), dim3(nb), dim3(QK_TPB), block_lds(T), s, T->tiles, T->hoff,
-                       T->lrank, T->lolist, T->br, T->dg, make_scal(T), X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
+                       T->lrank, T->lolist, T->brk, T->dg, make_scal(T), X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
                        npart, block_cap(T), (int)table_bytes(T));
     return rc(hipGetLastError());
 }
@@ -896,6 +927,7 @@ int qkl_apply_block(const QkTables* T, int wide, const double* X, double* Y, int
     if (nc <= 0 || nc > 64) return -1;
     if (T->ntiles == 0) return 0;
     if (block_lds(T) > 160 * 1024) return -2;
+    if (2 * (T->nA + T->nB) > 65535) return -3;   // value ids are 16-bit
     const bool u16 = T->lrank_u16 != 0, lz = lz_k >= 0;
 #define QK_BDISPATCH(ST)                                                                                      \
     if (u16) return lz ? launch_block<ST, true, true>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s) \
