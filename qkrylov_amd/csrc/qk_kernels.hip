@@ -151,6 +151,21 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // ------------------------------------------------------------------ apply
+// Per-wave LDS scratch of the apply kernel: for every branch the tile's
+// uniform data (target block offset, hi-condition flags, hi parity), and the
+// compacted list of the hi-only branches the tile's hi takes.
+__host__ __device__ __forceinline__ size_t apply_wave_bytes(int nbk, int nA)
+{
+    return (((size_t)nbk * 8 + (size_t)nA * 20) + 15) & ~(size_t)15;
+}
+
+// Per-workgroup LDS copy of the lo side of the per-lane branches: 16 B of
+// {lcm, lcv, lalt, lF}, then lM, re, im; read as broadcasts in the hot loop.
+__host__ __device__ __forceinline__ size_t apply_branch_bytes(int nB)
+{
+    return (((size_t)nB * (16 + 4 + 8 + 8)) + 15) & ~(size_t)15;
+}
+
 template <typename ST, bool CPLX, bool U16, bool LZ>
 __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
                                                    const uint32_t* __restrict__ lrank,
@@ -159,15 +174,33 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
                                                    const QkDiagTerm* __restrict__ dg, QkScal sc,
                                                    const double* __restrict__ x, double* __restrict__ y, int lz_k,
                                                    const double* __restrict__ prev, const double* __restrict__ nrm,
-                                                   double* __restrict__ partials)
+                                                   double* __restrict__ partials, int table_bytes)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double wsum[QK_WPB];
+    const int nA = sc.nA, nB = sc.nB, nbk = sc.nA + sc.nB;
+    uint4* lb4 = reinterpret_cast<uint4*>(smem + table_bytes);
+    uint32_t* lbm = reinterpret_cast<uint32_t*>(lb4 + nB);
+    double* lbre = reinterpret_cast<double*>(smem + table_bytes + ((size_t)nB * 20 + 7) / 8 * 8);
+    double* lbim = lbre + nB;
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+        const QkBranchK B = brk[nA + i];
+        lb4[i] = make_uint4(B.lcm, B.lcv, B.lalt, B.lF);
+        lbm[i] = B.lM;
+        lbre[i] = B.re;
+        lbim[i] = B.im;
+    }
     load_lrank<U16>(smem, lrank, sc.nlo);
 
     const uint32_t lane = threadIdx.x & 63u;
     const int NL = sc.NL;
-    const uint32_t lomask = sc.lomask;
+    unsigned char* wb = smem + table_bytes + apply_branch_bytes(nB) + (threadIdx.x >> 6) * apply_wave_bytes(nbk, nA);
+    uint2* tb = reinterpret_cast<uint2*>(wb);                          // [nbk] {block offset, flags}
+    double* a_cr = reinterpret_cast<double*>(wb + (size_t)nbk * 8);    // [nA] taken hi-only values
+    double* a_ci = a_cr + nA;
+    uint32_t* a_base = reinterpret_cast<uint32_t*>(a_ci + nA);         // [nA] their block offsets
+    const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+
     double inv_n = 1.0, cprev = 0.0;
     if (LZ) {
         inv_n = 1.0 / nrm[lz_k];
@@ -188,6 +221,37 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
         const uint32_t lo = lolist[tl.lo_base + lidx];
         const ST s = (static_cast<ST>(hi) << NL) | static_cast<ST>(lo);
 
+        // (1) the uniform part of every branch, one branch per lane
+        uint32_t ntaken = 0;
+        for (int b0 = 0; b0 < nbk; b0 += 64) {
+            const int b = b0 + (int)lane;
+            bool takeA = false;
+            uint32_t base = 0;
+            double vr = 0.0, vi = 0.0;
+            if (b < nbk) {
+                const QkBranchK B = brk[b];
+                const uint32_t hm = hi & B.hcm;
+                const uint32_t u0 = (hm == B.hcv) ? 1u : 0u;
+                const uint32_t u1 = (((B.halt | B.lalt) != 0) && (hm == (B.hcv ^ B.halt))) ? 2u : 0u;
+                const uint32_t hpar = (uint32_t)(__builtin_popcount(hi & B.hM) & 1);
+                base = hoff[hi ^ B.hF];
+                tb[b] = make_uint2(base, u0 | u1 | (hpar << 2));
+                takeA = (b < nA) && (u0 | u1);
+                vr = hpar ? -B.re : B.re;
+                vi = hpar ? -B.im : B.im;
+            }
+            // (2) compact the taken hi-only branches
+            const uint64_t m = __ballot(takeA);
+            if (takeA) {
+                const uint32_t pos = ntaken + (uint32_t)__popcll(m & lt_mask);
+                a_base[pos] = base;
+                a_cr[pos] = vr;
+                a_ci[pos] = vi;
+            }
+            ntaken += (uint32_t)__popcll(m);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
         double dre, dim;
         eval_diag<ST, CPLX>(dg, sc.ndg, sc.dcr, sc.dci, s, dre, dim);
         double ar, ai = 0.0, xr, xi = 0.0;
@@ -201,49 +265,44 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
             ar = dre * xr;
         }
 
-        // hi-only branches: condition, target block and sign are uniform;
-        // the gather is the tile's own lo-rank range in the target block.
-        // Groups of 4 issue their loads before accumulating; an untaken
-        // branch reads the row's own x with a zero coefficient.
-        for (int g = 0; g < sc.nA; g += 4) {
-            uint32_t rt[4];
-            double cr[4], ci[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const QkBranchK B = brk[g + u];
-                const uint32_t hm = hi & B.hcm;
-                const bool take = (hm == B.hcv) || (B.halt != 0 && hm == (B.hcv ^ B.halt));
-                const bool neg = (__builtin_popcount(hi & B.hM) & 1) != 0;
-                rt[u] = take ? hoff[hi ^ B.hF] + my_lr : row;
-                cr[u] = take ? (neg ? -B.re : B.re) : 0.0;
-                ci[u] = take ? (neg ? -B.im : B.im) : 0.0;
-            }
-            accumulate<CPLX, 4>(x, rt, cr, ci, ar, ai);
-        }
-        // remaining branches: per-lane condition on lo, lo rank from LDS
-        // (lF == 0 reads lrank[lo] = the row's own rank), groups of 8
-        for (int g = sc.nA; g < sc.nA + sc.nB; g += 8) {
+        // (3) taken hi-only branches: coalesced gathers of the tile's own
+        // lo-rank range in each target block, four in flight
+        for (uint32_t i = 0; i < ntaken; i += 8) {
             uint32_t rt[8];
             double cr[8], ci[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const QkBranchK B = brk[g + u];
-                const uint32_t hm = hi & B.hcm;
-                const bool u0 = (hm == B.hcv);
-                // a merged pair may have its alternative entirely in lo
-                const bool u1 = ((B.halt | B.lalt) != 0) && (hm == (B.hcv ^ B.halt));
-                const uint32_t base_t = hoff[hi ^ B.hF];
-                const int hpar = __builtin_popcount(hi & B.hM) & 1;
-                const uint32_t lf = lo & B.lcm;
-                const bool ok = (u0 && lf == B.lcv) || (u1 && lf == (B.lcv ^ B.lalt));
-                const uint32_t lr = lds_rank<U16>(smem, lo ^ B.lF);
-                const bool neg = ((__builtin_popcount(lo & B.lM) & 1) ^ hpar) != 0;
-                rt[u] = ok ? base_t + lr : row;
-                cr[u] = ok ? (neg ? -B.re : B.re) : 0.0;
-                ci[u] = ok ? (neg ? -B.im : B.im) : 0.0;
+                const bool in = (i + u) < ntaken;
+                const uint32_t k = in ? i + u : 0u;
+                rt[u] = in ? a_base[k] + my_lr : row;
+                cr[u] = in ? a_cr[k] : 0.0;
+                ci[u] = in ? a_ci[k] : 0.0;
             }
             accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
         }
+
+        // (4) remaining branches: per-lane condition on lo, lo rank from LDS
+        // (lF == 0 reads lrank[lo] = the row's own rank), eight in flight
+        for (int g = 0; g < nB; g += 8) {
+            uint32_t rt[8];
+            double cr[8], ci[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint4 L = lb4[g + u];            // lcm, lcv, lalt, lF
+                const uint2 tv = tb[nA + g + u];       // block offset, flags
+                const uint32_t lf = lo & L.x;
+                const uint32_t ok = ((tv.y & 1u) & (uint32_t)(lf == L.y)) |
+                                    ((tv.y >> 1) & 1u & (uint32_t)(lf == (L.y ^ L.z)));
+                const uint32_t lr = lds_rank<U16>(smem, lo ^ L.w);
+                const uint32_t neg = ((uint32_t)__builtin_popcount(lo & lbm[g + u]) ^ (tv.y >> 2)) & 1u;
+                const double vr = lbre[g + u], vi = CPLX ? lbim[g + u] : 0.0;
+                rt[u] = ok ? tv.x + lr : row;
+                cr[u] = ok ? (neg ? -vr : vr) : 0.0;
+                ci[u] = ok ? (neg ? -vi : vi) : 0.0;
+            }
+            accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
         // y is written once and not read again by this kernel: stream it
         // past the caches so L2 keeps x
@@ -874,8 +933,10 @@ int launch_apply(const QkTables* T, const double* x, double* y, int k, const dou
                  double* partials, hipStream_t s)
 {
     const int nb = apply_blocks(T->ntiles);
-    hipLaunchKernelGGL((k_apply<ST, CPLX, U16, LZ>), dim3(nb), dim3(QK_TPB), table_bytes(T), s, T->tiles, T->hoff,
-                       T->lrank, T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials);
+    const size_t lds = table_bytes(T) + apply_branch_bytes(T->nB) + (size_t)QK_WPB * apply_wave_bytes(T->nA + T->nB, T->nA);
+    if (lds > 160 * 1024) return -2;
+    hipLaunchKernelGGL((k_apply<ST, CPLX, U16, LZ>), dim3(nb), dim3(QK_TPB), lds, s, T->tiles, T->hoff, T->lrank,
+                       T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T));
     return rc(hipGetLastError());
 }
 
