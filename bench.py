@@ -201,6 +201,45 @@ def bench_serve(args):
     print(json.dumps(res), flush=True)
 
 
+def measure_traffic(args):
+    """HBM-side bytes per apply launch from two child runs of this script
+    under `rocprofv3 --pmc` (FETCH_SIZE, then WRITE_SIZE), each also running
+    one plain copy kernel of known size whose counter reading gives the
+    scale factor.  Runs before this process touches the GPU.  Returns
+    (bytes per launch, detail dict) or (None, reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+    from qkrylov_amd import profiling
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not on PATH"
+    calib_gib = 1.0
+    tmp = tempfile.mkdtemp(prefix="qk_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    dbs = {}
+    try:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(tmp, ctr)
+            cmd = ["timeout", "-s", "KILL", "300", prof, "--pmc", ctr, "-d", out, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--workload", args.workload,
+                   "--split-bits", str(args.split_bits), "--steps", "2", "--warmup", "0", "--no-lanczos",
+                   "--no-cpu", "--no-traffic", "--copy-calib-gib", str(calib_gib)]
+            log(f"[bench] traffic pass {ctr}: {' '.join(cmd[4:8])} ...")
+            r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                               text=True)
+            db = os.path.join(out, "run_results.db")
+            if r.returncode != 0 or not os.path.exists(db):
+                return None, f"rocprofv3 --pmc {ctr} exited {r.returncode}: {r.stderr.strip()[-300:]}"
+            dbs[ctr] = db
+        res = profiling.calibrated_traffic(dbs["FETCH_SIZE"], dbs["WRITE_SIZE"], calib_gib * (1 << 30), "k_apply<")
+        return res["bytes_per_dispatch"], res
+    except Exception as e:          # a counter run that cannot be read leaves traffic unmeasured
+        return None, f"{type(e).__name__}: {e}"
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +256,8 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per apply")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="also run one plain copy kernel of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -224,6 +265,11 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part of this "
                                    "build (see DESIGN.md for how the apply would shard)"}))
         sys.exit(2)
+
+    traffic, traffic_info = None, "not measured (--no-traffic)"
+    if not args.no_traffic and args.workload in WORKLOADS:
+        traffic, traffic_info = measure_traffic(args)
+        log(f"[bench] traffic: {traffic_info if traffic is None else f'{traffic / 1e9:.3f} GB per apply'}")
 
     import qkrylov_amd as qa
     from qkrylov_amd import _lib
@@ -274,9 +320,21 @@ def main():
         "config": {"workload": w["desc"], "D": D, "terms": len(terms),
                    "offdiag_branches": H.info["n_offdiag_branches"], "setup_s": round(t_setup, 3)},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 5),
+                     "traffic": None if traffic is None else int(traffic),
                      "algorithmic_bytes_per_launch": int(alg_bytes)},
     }
+
+    if traffic is not None:
+        out["roofline"]["traffic_detail"] = {
+            "source": "rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, mean over the apply launches)",
+            "fetch_factor": round(traffic_info["FETCH_SIZE"]["factor"], 4),
+            "write_factor": round(traffic_info["WRITE_SIZE"]["factor"], 4),
+            "fetch_bytes": int(traffic_info["FETCH_SIZE"]["bytes_per_dispatch"]),
+            "write_bytes": int(traffic_info["WRITE_SIZE"]["bytes_per_dispatch"]),
+            "traffic_over_algorithmic": round(traffic / alg_bytes, 2)}
+    else:
+        out["roofline"]["traffic_detail"] = {"source": traffic_info}
 
     if not args.no_lanczos:
         iters = args.lanczos_iters or w["lanczos"]

@@ -17,7 +17,7 @@
  * std::mt19937 inside the solver); everything downstream of the start vector
  * follows the reference step by step.
  *
- * Pinning (tests/test_oracle_*.py): basis lists, site actions and CSR triplets
+ * Pinning (tests/test_oracle.py): basis lists, site actions and CSR triplets
  * against fixtures emitted by the reference's own compiled basis/site/opsum
  * sources (oracle/_ref); closed-form dimensions; the reference tests' known
  * answers (-0.75, -1.6160254038, -2.0, 0.3125, JW signs); dense numpy ED built
