@@ -662,6 +662,7 @@ BlockTridiag block_tridiag_run(const Engine& E, double* X0, int ncol, int group,
     double* Cc = X0;
     double* W = Wb.d();
     const int npA = qkl_block_partials(E.tables()), NB = qkl_blas_partials();
+    if (npA <= 0) throw std::runtime_error("block recurrence: the operator tables do not fit in LDS");
     DevBuf partA((size_t)npA * ncol * sizeof(double)), partB((size_t)NB * ncol * sizeof(double));
     DevBuf alpha((size_t)(n_steps + 1) * ncol * sizeof(double)), nrm((size_t)(n_steps + 2) * ncol * sizeof(double));
     // initial norms: an update with zero coefficient leaves X unchanged and

@@ -333,39 +333,61 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
 }
 
 // ------------------------------------------------------------------ block apply
-// Phase A (lane = row): decode up to 64 rows of a tile into per-row lists of
-// (target row, value id) in LDS; value ids index a table of the signed
-// branch values. Phase B (lane = column): for each row, stream the list and
-// gather a contiguous segment of the target row of X, four entries in
-// flight. Each decoded matrix element is reused by all nc columns.
-struct BlockLds {
-    static constexpr int kHdr = 64 * 8 + 64 * 4 + 64 * 4;   // diag, off, cnt
-    static constexpr int kEntry = 6;                         // uint32 target + uint16 value id
-};
+// Y = H X for up to 64 real columns of a row-major block (row stride ld).
+// One wave per tile, up to 16 waves per workgroup sharing the lo-rank table.
+// Phase A (lane = row) decodes the rows of the tile into one flat LDS list
+// of (target row, value id) entries in row order: the row's diagonal first
+// (target = the row itself, id = kDiag + j), then, in the Lanczos form, the
+// row of the previous vector (id = kPrev), then the off-diagonal entries in
+// branch order. Phase B (lane = column) walks the list eight entries at a
+// time with the next eight gathers already in flight (two register sets
+// used in turn), so row boundaries do not drain the memory pipeline. Every
+// decoded entry is reused by all columns, and every gather is one
+// contiguous segment of a row of X.
 
-// per-lane branch test shared by both passes; returns the target row and
-// signed value id when branch g+u applies to the lane's state
-template <bool U16>
-__device__ __forceinline__ bool branch_hit(const QkBranchK& B, uint32_t hi, uint32_t lo, uint32_t my_lr,
-                                           const uint32_t* __restrict__ hoff, const unsigned char* smem, int b,
-                                           uint32_t& tgt, uint32_t& vid)
+// per-workgroup LDS after the lo-rank table: lo fields of the per-lane
+// branches ({lcm, lcv, lalt, lF} as 16 B, then lM), then +-value of every
+// branch (value id 2b + sign)
+__host__ __device__ __forceinline__ size_t block_branch_bytes(int nB) { return ((size_t)nB * 20 + 15) & ~(size_t)15; }
+
+// per-wave LDS (cap a multiple of 8): entry targets [cap] uint32, entry
+// value ids [cap] uint16, the rows' diagonals [64] double, the tile data of
+// every branch [nbk] uint2, the taken hi-only list [nA] x (block offset,
+// value id)
+__host__ __device__ __forceinline__ size_t block_wave_bytes(int nbk, int nA, int cap)
 {
-    const uint32_t hm = hi & B.hcm;
-    const bool u0 = (hm == B.hcv);
-    const bool u1 = ((B.halt | B.lalt) != 0) && (hm == (B.hcv ^ B.halt));
-    const uint32_t lf = lo & B.lcm;
-    const bool ok = (u0 && lf == B.lcv) || (u1 && lf == (B.lcv ^ B.lalt));
-    if (ok) {
-        const uint32_t lr = B.lF ? lds_rank<U16>(smem, lo ^ B.lF) : my_lr;
-        tgt = hoff[hi ^ B.hF] + lr;
-        const int neg = (__builtin_popcount(hi & B.hM) + __builtin_popcount(lo & B.lM)) & 1;
-        vid = 2u * (uint32_t)b + (uint32_t)neg;
+    return ((size_t)cap * 6 + 64 * 8 + (size_t)nbk * 8 + (size_t)nA * 8 + 15) & ~(size_t)15;
+}
+
+// issue the gathers of entries [e0, e0 + 8) of the list (entries at or past
+// `total` gather row `tfill` of X and carry id kNop)
+__device__ __forceinline__ void block_gather8(const uint32_t* l_tgt, const uint16_t* l_vid, uint32_t e0,
+                                              uint32_t total, uint32_t tfill, const double* __restrict__ Xc,
+                                              const double* __restrict__ Pc, size_t ld, uint32_t kPrev,
+                                              uint32_t kNop, double* xg, uint32_t* vg)
+{
+    const uint32_t eb = e0 < total ? e0 : 0u;
+    const uint4 ta = *reinterpret_cast<const uint4*>(l_tgt + eb);
+    const uint4 tc = *reinterpret_cast<const uint4*>(l_tgt + eb + 4);
+    const uint4 vv = *reinterpret_cast<const uint4*>(l_vid + eb);
+    const uint32_t tg[8] = {ta.x, ta.y, ta.z, ta.w, tc.x, tc.y, tc.z, tc.w};
+    const uint32_t vd[8] = {vv.x & 0xFFFFu, vv.x >> 16, vv.y & 0xFFFFu, vv.y >> 16,
+                            vv.z & 0xFFFFu, vv.z >> 16, vv.w & 0xFFFFu, vv.w >> 16};
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const bool in = e0 + (uint32_t)u < total;
+        uint32_t t = __builtin_amdgcn_readfirstlane(tg[u]);
+        uint32_t v = __builtin_amdgcn_readfirstlane(vd[u]);
+        t = in ? t : tfill;
+        v = in ? v : kNop;
+        const double* src = (v == kPrev) ? Pc : Xc;
+        xg[u] = src[(size_t)t * ld];
+        vg[u] = v;
     }
-    return ok;
 }
 
 template <typename ST, bool U16, bool LZ>
-__global__ __launch_bounds__(QK_TPB) void k_apply_block(
+__global__ __launch_bounds__(QK_TPB * 4) void k_apply_block(
     const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff, const uint32_t* __restrict__ lrank,
     const uint32_t* __restrict__ lolist, const QkBranchK* __restrict__ brk, const QkDiagTerm* __restrict__ dg,
     QkScal sc, const double* __restrict__ X, double* __restrict__ Y, int ld, int c0, int nc, int lz_k,
@@ -373,34 +395,48 @@ __global__ __launch_bounds__(QK_TPB) void k_apply_block(
     int npart, int cap, int table_bytes)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double wsum[QK_WPB][64];
-    const int nbk = sc.nA + sc.nB;
-    double* l_vals = reinterpret_cast<double*>(smem + table_bytes);
+    const int nA = sc.nA, nB = sc.nB, nbk = sc.nA + sc.nB;
+    uint4* lb4 = reinterpret_cast<uint4*>(smem + table_bytes);
+    uint32_t* lbm = reinterpret_cast<uint32_t*>(lb4 + nB);
+    double* l_vals = reinterpret_cast<double*>(smem + table_bytes + block_branch_bytes(nB));
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+        const QkBranchK B = brk[nA + i];
+        lb4[i] = make_uint4(B.lcm, B.lcv, B.lalt, B.lF);
+        lbm[i] = B.lM;
+    }
     for (int i = threadIdx.x; i < nbk; i += blockDim.x) {
-        l_vals[2 * i] = brk[i].re;
-        l_vals[2 * i + 1] = -brk[i].re;
+        const double v = brk[i].re;
+        l_vals[2 * i] = v;
+        l_vals[2 * i + 1] = -v;
     }
     load_lrank<U16>(smem, lrank, sc.nlo);
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = threadIdx.x >> 6;
-    const size_t vals_bytes = ((size_t)nbk * 16 + 15) & ~(size_t)15;
-    unsigned char* wbase = smem + table_bytes + vals_bytes +
-                           (size_t)wv * (BlockLds::kHdr + (((size_t)cap * BlockLds::kEntry + 15) & ~(size_t)15));
-    double* l_diag = reinterpret_cast<double*>(wbase);
-    uint32_t* l_off = reinterpret_cast<uint32_t*>(wbase + 64 * 8);
-    uint32_t* l_cnt = reinterpret_cast<uint32_t*>(wbase + 64 * 8 + 64 * 4);
-    uint32_t* l_tgt = reinterpret_cast<uint32_t*>(wbase + BlockLds::kHdr);
-    uint16_t* l_vid = reinterpret_cast<uint16_t*>(wbase + BlockLds::kHdr + (size_t)cap * 4);
+    const size_t wave0 = (size_t)table_bytes + block_branch_bytes(nB) + (size_t)nbk * 16;
+    unsigned char* wb = smem + wave0 + (size_t)wv * block_wave_bytes(nbk, nA, cap);
+    uint32_t* l_tgt = reinterpret_cast<uint32_t*>(wb);                                   // [cap]
+    uint16_t* l_vid = reinterpret_cast<uint16_t*>(wb + (size_t)cap * 4);                 // [cap]
+    double* l_diag = reinterpret_cast<double*>(wb + (size_t)cap * 6);                    // [64]
+    uint2* tb = reinterpret_cast<uint2*>(l_diag + 64);                                   // [nbk]
+    uint32_t* a_base = reinterpret_cast<uint32_t*>(tb + nbk);                            // [nA]
+    uint32_t* a_vid = a_base + nA;                                                       // [nA]
+    const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    const uint32_t kDiag = 2u * (uint32_t)nbk, kPrev = kDiag + 64u, kNop = 0xFFFFu, kNone = 0xFFFFFFFFu;
 
     const int NL = sc.NL;
     const bool colok = lane < (uint32_t)nc;
     const int col = c0 + (int)(colok ? lane : 0u);
+    const bool use_prev = LZ && lz_k > 0;
     double inv_n = 1.0, cprev = 0.0;
-    if (LZ && colok) {
+    if (LZ) {
         inv_n = 1.0 / nrm[(size_t)lz_k * ld_sc + col];
         if (lz_k > 0) cprev = nrm[(size_t)lz_k * ld_sc + col] / nrm[(size_t)(lz_k - 1) * ld_sc + col];
     }
+    const double* Xc = X + col;
+    const double* Pc = use_prev ? P + col : Xc;
+    double* Yc = Y + col;
+    const size_t lds = (size_t)ld;
     double part = 0.0;
 
     uint32_t t0, tend, step;
@@ -416,13 +452,47 @@ __global__ __launch_bounds__(QK_TPB) void k_apply_block(
         const uint32_t lo = lolist[tl.lo_base + lidx];
         const ST s = (static_cast<ST>(hi) << NL) | static_cast<ST>(lo);
 
-        // ---- phase A, pass 1: diagonal and entry counts
+        // ---- phase A.1: the uniform half of every branch, one per lane,
+        // and the compacted list of the hi-only branches the tile takes
+        uint32_t ntaken = 0;
+        for (int b0 = 0; b0 < nbk; b0 += 64) {
+            const int b = b0 + (int)lane;
+            bool takeA = false;
+            uint32_t base = 0, vid = 0;
+            if (b < nbk) {
+                const QkBranchK B = brk[b];
+                const uint32_t hm = hi & B.hcm;
+                const uint32_t u0 = (hm == B.hcv) ? 1u : 0u;
+                const uint32_t u1 = (((B.halt | B.lalt) != 0) && (hm == (B.hcv ^ B.halt))) ? 2u : 0u;
+                const uint32_t hpar = (uint32_t)(__builtin_popcount(hi & B.hM) & 1);
+                base = hoff[hi ^ B.hF];
+                tb[b] = make_uint2(base, u0 | u1 | (hpar << 2));
+                takeA = (b < nA) && (u0 | u1);
+                vid = 2u * (uint32_t)b + hpar;
+            }
+            const uint64_t m = __ballot(takeA);
+            if (takeA) {
+                const uint32_t pos = ntaken + (uint32_t)__popcll(m & lt_mask);
+                a_base[pos] = base;
+                a_vid[pos] = vid;
+            }
+            ntaken += (uint32_t)__popcll(m);
+        }
         double dre, dim;
         eval_diag<ST, false>(dg, sc.ndg, sc.dcr, sc.dci, s, dre, dim);
-        uint32_t n_j = 0;
-        for (int b = 0; b < nbk; ++b) {
-            uint32_t tg, vd;
-            n_j += branch_hit<U16>(brk[b], hi, lo, my_lr, hoff, smem, b, tg, vd) ? 1u : 0u;
+        l_diag[lane] = dre;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
+        // ---- phase A.2: entries per row
+        uint32_t n_j = 1u + (use_prev ? 1u : 0u) + ntaken;
+        for (int g = 0; g < nB; g += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint4 L = lb4[g + u];
+                const uint32_t fl = tb[nA + g + u].y;
+                const uint32_t lf = lo & L.x;
+                n_j += ((fl & 1u) & (uint32_t)(lf == L.y)) | ((fl >> 1) & 1u & (uint32_t)(lf == (L.y ^ L.z)));
+            }
         }
         if (!active) n_j = 0;
         uint32_t incl = n_j;
@@ -431,68 +501,106 @@ __global__ __launch_bounds__(QK_TPB) void k_apply_block(
             if ((int)lane >= off) incl += o;
         }
         const uint32_t excl = incl - n_j;
-        l_diag[lane] = dre;
-        l_off[lane] = excl;
-        l_cnt[lane] = n_j;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
         uint32_t j0 = 0;
         while (j0 < count) {
-            const uint32_t base = __shfl(excl, (int)j0, 64);
-            const bool inr = (lane >= j0) && active && (excl + n_j - base <= (uint32_t)cap);
+            const uint32_t ebase = __builtin_amdgcn_readfirstlane(__shfl(excl, (int)j0, 64));
+            const bool inr = (lane >= j0) && active && (incl - ebase <= (uint32_t)cap);
             const uint64_t bal = __ballot(inr);
             const uint32_t j1 = j0 + (uint32_t)__popcll(bal);
-            // ---- phase A, pass 2: write the entries of rows [j0, j1)
+            const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, (int)j1 - 1, 64)) - ebase;
+            // ---- phase A.3: write the entries of rows [j0, j1)
             if (inr) {
-                uint32_t k = excl - base;
-                for (int b = 0; b < nbk; ++b) {
-                    uint32_t tg, vd;
-                    if (branch_hit<U16>(brk[b], hi, lo, my_lr, hoff, smem, b, tg, vd)) {
-                        l_tgt[k] = tg;
-                        l_vid[k] = (uint16_t)vd;
-                        ++k;
+                uint32_t k = excl - ebase;
+                const uint32_t row = tl.row_start + lane;
+                l_tgt[k] = row;
+                l_vid[k] = (uint16_t)(kDiag + lane);
+                ++k;
+                if (use_prev) {
+                    l_tgt[k] = row;
+                    l_vid[k] = (uint16_t)kPrev;
+                    ++k;
+                }
+                for (uint32_t i = 0; i < ntaken; ++i) {
+                    l_tgt[k] = a_base[i] + my_lr;
+                    l_vid[k] = (uint16_t)a_vid[i];
+                    ++k;
+                }
+                for (int g = 0; g < nB; g += 8) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const uint4 L = lb4[g + u];
+                        const uint2 tv = tb[nA + g + u];
+                        const uint32_t lf = lo & L.x;
+                        const bool ok = ((tv.y & 1u) && lf == L.y) || (((tv.y >> 1) & 1u) && lf == (L.y ^ L.z));
+                        if (ok) {
+                            const uint32_t lr = lds_rank<U16>(smem, lo ^ L.w);
+                            const uint32_t neg = ((uint32_t)__builtin_popcount(lo & lbm[g + u]) ^ (tv.y >> 2)) & 1u;
+                            l_tgt[k] = tv.x + lr;
+                            l_vid[k] = (uint16_t)(2u * (uint32_t)(nA + g + u) + neg);
+                            ++k;
+                        }
                     }
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            // ---- phase B: lane = column
-            if (colok) {
-                for (uint32_t j = j0; j < j1; ++j) {
-                    const size_t row = (size_t)tl.row_start + j;
-                    const double xs = X[row * ld + col];
-                    double acc = l_diag[j] * xs;
-                    const uint32_t e0 = l_off[j] - base, e1 = e0 + l_cnt[j];
-                    uint32_t e = e0;
-                    for (; e + 4 <= e1; e += 4) {
-                        const uint32_t ta = l_tgt[e], tb = l_tgt[e + 1], tc = l_tgt[e + 2], td = l_tgt[e + 3];
-                        const double xa = X[(size_t)ta * ld + col], xb = X[(size_t)tb * ld + col];
-                        const double xc = X[(size_t)tc * ld + col], xd = X[(size_t)td * ld + col];
-                        acc = fma(l_vals[l_vid[e]], xa, acc);
-                        acc = fma(l_vals[l_vid[e + 1]], xb, acc);
-                        acc = fma(l_vals[l_vid[e + 2]], xc, acc);
-                        acc = fma(l_vals[l_vid[e + 3]], xd, acc);
-                    }
-                    for (; e < e1; ++e) acc = fma(l_vals[l_vid[e]], X[(size_t)l_tgt[e] * ld + col], acc);
-                    if (LZ) {
-                        double w = acc * inv_n;
-                        if (lz_k > 0) w = fma(-cprev, P[row * ld + col], w);
-                        Y[row * ld + col] = w;
-                        part = fma(xs * inv_n, w, part);
-                    } else {
-                        Y[row * ld + col] = acc;
+
+            // ---- phase B: lane = column, flat walk over the list
+            uint32_t cur = kNone;
+            double acc = 0.0, xs = 0.0, pv = 0.0;
+            auto finish = [&]() {
+                const size_t r = (size_t)(tl.row_start + cur) * lds;
+                if (LZ) {
+                    double w = acc * inv_n;
+                    if (use_prev) w = fma(-cprev, pv, w);
+                    if (colok) __builtin_nontemporal_store(w, Yc + r);
+                    part = fma(xs * inv_n, w, part);
+                } else {
+                    if (colok) __builtin_nontemporal_store(acc, Yc + r);
+                }
+            };
+            auto consume = [&](const double* xg, const uint32_t* vg) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const uint32_t v = vg[u];
+                    if (v < kDiag) {
+                        acc = fma(l_vals[v], xg[u], acc);
+                    } else if (v < kPrev) {
+                        if (cur != kNone) finish();
+                        cur = v - kDiag;
+                        xs = xg[u];
+                        acc = l_diag[cur] * xs;
+                    } else if (v == kPrev) {
+                        pv = xg[u];
                     }
                 }
+            };
+            const uint32_t tfill = __builtin_amdgcn_readfirstlane(l_tgt[0]);
+            double xa[8], xb[8];
+            uint32_t va[8], vb[8];
+            block_gather8(l_tgt, l_vid, 0, total, tfill, Xc, Pc, lds, kPrev, kNop, xa, va);
+            for (uint32_t e0 = 0; e0 < total; e0 += 16) {
+                block_gather8(l_tgt, l_vid, e0 + 8, total, tfill, Xc, Pc, lds, kPrev, kNop, xb, vb);
+                consume(xa, va);
+                block_gather8(l_tgt, l_vid, e0 + 16, total, tfill, Xc, Pc, lds, kPrev, kNop, xa, va);
+                consume(xb, vb);
             }
+            if (cur != kNone) finish();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             j0 = j1;
         }
     }
     if (LZ) {
-        wsum[wv][lane] = part;
+        // per-column partial of <q_k, w>, waves added in index order
+        __syncthreads();
+        l_diag[lane] = part;
         __syncthreads();
         if (wv == 0 && colok) {
             double s = 0.0;
-            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += wsum[i][lane];
+            const int nw = (int)(blockDim.x >> 6);
+            for (int i = 0; i < nw; ++i)
+                s += reinterpret_cast<const double*>(smem + wave0 + (size_t)i * block_wave_bytes(nbk, nA, cap) +
+                                                     (size_t)cap * 6)[lane];
             partials[(size_t)lane * npart + blockIdx.x] = s;
         }
     }
@@ -899,24 +1007,41 @@ inline int apply_blocks(uint32_t ntiles)
 // LDS bytes of the lo-rank table, rounded to 16 so what follows it is aligned
 inline size_t table_bytes(const QkTables* T) { return ((size_t)T->nlo * (T->lrank_u16 ? 2 : 4) + 15) & ~(size_t)15; }
 
-inline size_t block_vals_bytes(const QkTables* T) { return ((size_t)(T->nA + T->nB) * 16 + 15) & ~(size_t)15; }
+// Block-apply launch shape: 16, 8 or 4 waves per workgroup, sized so one
+// CU holds 16 waves (one workgroup of 16, two of 8, four of 4), with as
+// large an entry list per wave as fits; the grid is one resident wave of
+// workgroups over the 256 CUs.
+struct BlockCfg {
+    int nw, cap, nb;
+    size_t lds;
+};
 
-// entry capacity per wave: as many as fit two workgroups per CU (160 KB LDS),
-// never fewer than one row's worth of branches
-inline int block_cap(const QkTables* T)
+inline bool block_config(const QkTables* T, BlockCfg& c)
 {
-    const long budget = 78 * 1024 - (long)table_bytes(T) - (long)block_vals_bytes(T) - QK_WPB * BlockLds::kHdr;
-    long cap = budget / (QK_WPB * BlockLds::kEntry);
-    cap = cap > 2048 ? 2048 : cap;
-    const long need = T->nA + T->nB;
-    if (cap < need) cap = need;
-    return (int)(cap & ~7L) + (int)((cap & 7L) ? 8 : 0);
-}
-
-inline size_t block_lds(const QkTables* T)
-{
-    return table_bytes(T) + block_vals_bytes(T) +
-           (size_t)QK_WPB * (BlockLds::kHdr + (((size_t)block_cap(T) * BlockLds::kEntry + 15) & ~(size_t)15));
+    const int nbk = T->nA + T->nB;
+    const size_t shared = table_bytes(T) + block_branch_bytes(T->nB) + (size_t)nbk * 16;
+    const int widths[3] = {16, 8, 4};
+    for (int w = 0; w < 3; ++w) {
+        const int nw = widths[w];
+        const size_t budget = (size_t)150 * 1024 * nw / 16;
+        if (shared + (size_t)nw * block_wave_bytes(nbk, T->nA, nbk + 8) > budget) continue;
+        const size_t per = (budget - shared) / nw;
+        long cap = ((long)per - (long)block_wave_bytes(nbk, T->nA, 0) - 16) / 6;
+        if (cap > 4096) cap = 4096;
+        cap &= ~7L;
+        if (cap < nbk + 2) continue;
+        c.nw = nw;
+        c.cap = (int)cap;
+        c.lds = shared + (size_t)nw * block_wave_bytes(nbk, T->nA, (int)cap);
+        uint32_t nb = (T->ntiles + nw - 1) / nw;
+        const uint32_t maxb = 256u * (16u / (uint32_t)nw);
+        if (nb > maxb) nb = maxb;
+        if (nb >= 8) nb = (nb + 7u) & ~7u;
+        if (nb == 0) nb = 1;
+        c.nb = (int)nb;
+        return true;
+    }
+    return false;
 }
 
 inline int grid_for(size_t n)
@@ -941,14 +1066,13 @@ int launch_apply(const QkTables* T, const double* x, double* y, int k, const dou
 }
 
 template <typename ST, bool U16, bool LZ>
-int launch_block(const QkTables* T, const double* X, double* Y, int ld, int c0, int nc, int k, const double* P,
-                 const double* nrm, int ld_sc, double* partials, int npart, hipStream_t s)
+int launch_block(const QkTables* T, const BlockCfg& c, const double* X, double* Y, int ld, int c0, int nc, int k,
+                 const double* P, const double* nrm, int ld_sc, double* partials, int npart, hipStream_t s)
 {
-    const int nb = apply_blocks(T->ntiles);
-    if (LZ && npart != nb) return -1;
-    hipLaunchKernelGGL((k_apply_block<ST, U16, LZ>), dim3(nb), dim3(QK_TPB), block_lds(T), s, T->tiles, T->hoff,
+    if (LZ && npart != c.nb) return -1;
+    hipLaunchKernelGGL((k_apply_block<ST, U16, LZ>), dim3(c.nb), dim3(64 * c.nw), c.lds, s, T->tiles, T->hoff,
                        T->lrank, T->lolist, T->brk, T->dg, make_scal(T), X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
-                       npart, block_cap(T), (int)table_bytes(T));
+                       npart, c.cap, (int)table_bytes(T));
     return rc(hipGetLastError());
 }
 
@@ -957,7 +1081,11 @@ int launch_block(const QkTables* T, const double* X, double* Y, int ld, int c0, 
 extern "C" {
 
 int qkl_apply_partials(const QkTables* T) { return apply_blocks(T->ntiles); }
-int qkl_block_partials(const QkTables* T) { return apply_blocks(T->ntiles); }
+int qkl_block_partials(const QkTables* T)
+{
+    BlockCfg c;
+    return block_config(T, c) ? c.nb : -2;
+}
 int qkl_blas_partials(void) { return QK_BLAS_BLOCKS; }
 
 int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y, int lz_k, const double* prev,
@@ -985,16 +1113,17 @@ int qkl_apply_block(const QkTables* T, int wide, const double* X, double* Y, int
                     const double* P, const double* nrm, int ld_sc, double* partials, int npart, qkl_stream sv)
 {
     hipStream_t s = (hipStream_t)sv;
-    if (nc <= 0 || nc > 64) return -1;
+    if (nc <= 0 || nc > 64 || ld < c0 + nc) return -1;
     if (T->ntiles == 0) return 0;
-    if (block_lds(T) > 160 * 1024) return -2;
-    if (2 * (T->nA + T->nB) > 65535) return -3;   // value ids are 16-bit
+    if (2 * (T->nA + T->nB) + 64 >= 0xFFFF) return -3;   // value ids are 16-bit
+    BlockCfg c;
+    if (!block_config(T, c)) return -2;
     const bool u16 = T->lrank_u16 != 0, lz = lz_k >= 0;
-#define QK_BDISPATCH(ST)                                                                                      \
-    if (u16) return lz ? launch_block<ST, true, true>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s) \
-                       : launch_block<ST, true, false>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s); \
-    else return lz ? launch_block<ST, false, true>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s)   \
-                   : launch_block<ST, false, false>(T, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s);
+#define QK_BDISPATCH(ST)                                                                                         \
+    if (u16) return lz ? launch_block<ST, true, true>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s) \
+                       : launch_block<ST, true, false>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s); \
+    else return lz ? launch_block<ST, false, true>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s)   \
+                   : launch_block<ST, false, false>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s);
     if (wide) { QK_BDISPATCH(uint64_t) } else { QK_BDISPATCH(uint32_t) }
 #undef QK_BDISPATCH
 }
