@@ -1,7 +1,8 @@
 """Benchmark of the matrix-free H.x apply and the Lanczos loop on one MI355X.
 
 Default workload: J1-J2 Heisenberg model on the 4x8 periodic square lattice
-(J1 = 1, J2 = 0.5), Sz = 0 sector, D = 601,080,390, real FP64. A "step" is
+(Lx = 4, Ly = 8, site index x + 4y; J1 = 1, J2 = 0.5), Sz = 0 sector,
+D = 601,080,390, real FP64. A "step" is
 one full apply y = H x with x and y resident in HBM. The JSON line reports
 the apply throughput in algorithmic bytes (16 B per row: x read once, y
 written once) over the kernel's mean time from HIP events on the engine's
@@ -25,8 +26,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    "j1j2_4x8": dict(desc="J1-J2 Heisenberg 4x8 periodic square lattice, J1=1 J2=0.5, Sz=0",
+    "j1j2_4x8": dict(desc="J1-J2 Heisenberg 4x8 periodic square lattice (Lx=4, Ly=8, site x + 4y), J1=1 J2=0.5, Sz=0",
                      kind="spin", n=32, sz2=0, lanczos=200),
+    "j1j2_8x4": dict(desc="J1-J2 Heisenberg 4x8 periodic square lattice labelled along the long side (Lx=8, Ly=4, "
+                          "site x + 8y), J1=1 J2=0.5, Sz=0", kind="spin", n=32, sz2=0, lanczos=200),
+    "heis_ring32": dict(desc="Heisenberg ring L=32 periodic, J=1, Sz=0", kind="spin", n=32, sz2=0, lanczos=200),
     "heis_ring20": dict(desc="Heisenberg ring L=20 periodic, J=1, Sz=0", kind="spin", n=20, sz2=0, lanczos=300),
     "hubbard16": dict(desc="Fermi-Hubbard 16-site periodic chain, t=1 U=4, Nup=Ndn=8", kind="hubbard", n=16,
                       lanczos=250),
@@ -41,7 +45,11 @@ def log(*a):
 def build_terms(name):
     from qkrylov_amd import models
     if name == "j1j2_4x8":
-        return models.j1j2_square(4, 8, 1.0, 0.5)
+        return models.j1j2_square(8, 4, 1.0, 0.5)     # 8 rows of 4 sites
+    if name == "j1j2_8x4":
+        return models.j1j2_square(4, 8, 1.0, 0.5)     # 4 rows of 8 sites
+    if name == "heis_ring32":
+        return models.heisenberg_chain(32)
     if name == "heis_ring20":
         return models.heisenberg_chain(20)
     if name == "hubbard16":
@@ -221,9 +229,15 @@ def measure_traffic(args):
     try:
         for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
             out = os.path.join(tmp, ctr)
+            sched = [] if args.sched is None else ["--sched", str(args.sched)]
+            if args.block_waves is not None:
+                sched += ["--block-waves", str(args.block_waves)]
+            if args.apply_waves is not None:
+                sched += ["--apply-waves", str(args.apply_waves)]
             cmd = ["timeout", "-s", "KILL", "300", prof, "--pmc", ctr, "-d", out, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--workload", args.workload,
-                   "--split-bits", str(args.split_bits), "--steps", "2", "--warmup", "0", "--no-lanczos",
+                   "--split-bits", str(args.split_bits), "--nvec", str(args.nvec)] + sched + [
+                   "--steps", "2", "--warmup", "0", "--no-lanczos",
                    "--no-cpu", "--no-traffic", "--copy-calib-gib", str(calib_gib)]
             log(f"[bench] traffic pass {ctr}: {' '.join(cmd[4:8])} ...")
             r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
@@ -232,7 +246,8 @@ def measure_traffic(args):
             if r.returncode != 0 or not os.path.exists(db):
                 return None, f"rocprofv3 --pmc {ctr} exited {r.returncode}: {r.stderr.strip()[-300:]}"
             dbs[ctr] = db
-        res = profiling.calibrated_traffic(dbs["FETCH_SIZE"], dbs["WRITE_SIZE"], calib_gib * (1 << 30), "k_apply<")
+        res = profiling.calibrated_traffic(dbs["FETCH_SIZE"], dbs["WRITE_SIZE"], calib_gib * (1 << 30),
+                                             "k_apply_block<" if args.nvec > 1 else "k_apply<")
         return res["bytes_per_dispatch"], res
     except Exception as e:          # a counter run that cannot be read leaves traffic unmeasured
         return None, f"{type(e).__name__}: {e}"
@@ -256,6 +271,12 @@ def main():
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--nvec", type=int, default=1,
+                    help="columns of the applied block (1: single-vector kernel; >1: row-major block kernel)")
+    ap.add_argument("--sched", type=int, default=None,
+                    help="tile schedule (0: one segment per XCD, 1: one front; default: the engine's choice)")
+    ap.add_argument("--block-waves", type=int, default=None, help="waves per workgroup of the block apply (16, 8, 4)")
+    ap.add_argument("--apply-waves", type=int, default=None, help="waves per workgroup of the apply (4, 8, 16)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per apply")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
@@ -297,10 +318,18 @@ def main():
     # ---- timed applies (x, y resident in HBM; HIP events on the engine stream)
     L.qkrylov_amd_synchronize()
     t0 = time.perf_counter()
-    ms = H.time_apply(1, args.warmup, args.steps)
+    tune = {}
+    if args.sched is not None:
+        tune.update(schedule=args.sched, block_schedule=args.sched)
+    if args.block_waves is not None:
+        tune["block_waves"] = args.block_waves
+    if args.apply_waves is not None:
+        tune["apply_waves"] = args.apply_waves
+    H.set_tuning(**tune)
+    ms = H.time_apply(args.nvec, args.warmup, args.steps)
     L.qkrylov_amd_synchronize()
     wall = time.perf_counter() - t0
-    alg_bytes = 16.0 * D
+    alg_bytes = 16.0 * D * args.nvec
     gbs = alg_bytes / (ms * 1e-3) / 1e9
     log(f"[bench] apply: {ms:.3f} ms/apply -> {gbs:.1f} GB/s algorithmic ({wall:.2f}s wall incl. warmup)")
 
@@ -317,7 +346,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (x uniform[-1,1) from the counter RNG)",
-        "config": {"workload": w["desc"], "D": D, "terms": len(terms),
+        "config": {"workload": w["desc"], "D": D, "terms": len(terms), "columns": args.nvec, "NL": H.info["NL"],
+                   "tuning": H.tuning(),
                    "offdiag_branches": H.info["n_offdiag_branches"], "setup_s": round(t_setup, 3)},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 5),
@@ -336,7 +366,7 @@ def main():
     else:
         out["roofline"]["traffic_detail"] = {"source": traffic_info}
 
-    if not args.no_lanczos:
+    if not args.no_lanczos and args.nvec == 1:
         iters = args.lanczos_iters or w["lanczos"]
         r = qa.lanczos_ground_state(H, maxiter=iters, reorthogonalize=False, fixed_steps=True, seed=1234)
         ips = r.report["steps_run"] / r.report["loop_seconds"]

@@ -48,6 +48,28 @@ typedef struct {
     uint64_t table_bytes;
 } qkrylov_hamiltonian_info_t;
 int qkrylov_hamiltonian_info(qkrylov_hamiltonian_h h, qkrylov_hamiltonian_info_t* info);
+/* Launch tuning of the apply kernels. Results are bitwise the same under
+ * every setting; only speed differs. Not to be changed while applies or a
+ * solver on this handle are running.
+ *   QKRYLOV_TUNE_SCHEDULE        tile schedule of the single-vector apply:
+ *                                0 one contiguous tile segment per XCD
+ *                                (default), 1 one front swept by all XCDs
+ *   QKRYLOV_TUNE_BLOCK_SCHEDULE  the same for the multi-vector apply
+ *                                (default 1)
+ *   QKRYLOV_TUNE_APPLY_WAVES     waves per workgroup of the single-vector
+ *                                apply, sharing one LDS copy of the tables:
+ *                                4, 8 or 16 (default)
+ *   QKRYLOV_TUNE_BLOCK_WAVES     waves per workgroup of the multi-vector
+ *                                apply: 16, 8 (default) or 4, with 8, 16 or
+ *                                32 gathers in flight per wave */
+enum {
+    QKRYLOV_TUNE_SCHEDULE = 0,
+    QKRYLOV_TUNE_BLOCK_SCHEDULE = 1,
+    QKRYLOV_TUNE_APPLY_WAVES = 2,
+    QKRYLOV_TUNE_BLOCK_WAVES = 3
+};
+int qkrylov_hamiltonian_set_tuning(qkrylov_hamiltonian_h h, int knob, int value);
+int qkrylov_hamiltonian_get_tuning(qkrylov_hamiltonian_h h, int knob, int* value);
 
 /* ---- device-resident applies (pointers from qkrylov_amd_malloc or any
  * device allocation; stream may be NULL for the library stream) ---- */

@@ -146,6 +146,8 @@ SIGNATURES = {
     "qkrylov_hamiltonian_create_ex": (c_vp, [c_vp, c_vp, c_vp, c_int]),
     "qkrylov_operator_create": (c_vp, [c_vp, c_vp, c_vp]),
     "qkrylov_hamiltonian_info": (c_int, [c_vp, P(HamInfo)]),
+    "qkrylov_hamiltonian_set_tuning": (c_int, [c_vp, c_int, c_int]),
+    "qkrylov_hamiltonian_get_tuning": (c_int, [c_vp, c_int, P(c_int)]),
     "qkrylov_hamiltonian_apply_device_f64": (c_int, [c_vp, c_vp, c_vp, c_int, c_vp]),
     "qkrylov_hamiltonian_apply_device_complex_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "qkrylov_hamiltonian_diagonal_device_f64": (c_int, [c_vp, c_vp, c_int, c_vp]),

@@ -323,6 +323,23 @@ class MatrixFreeHamiltonian:
         check(_lib.load().qkrylov_hamiltonian_states(self._h, row0, n, out.ctypes.data_as(_lib.p_u64)), "states")
         return out
 
+    TUNING_KNOBS = {"schedule": 0, "block_schedule": 1, "apply_waves": 2, "block_waves": 3}
+
+    def set_tuning(self, **knobs):
+        """Launch tuning of the apply kernels (same results, different
+        speed): schedule / block_schedule (0 one tile segment per XCD, 1 one
+        front over all XCDs), apply_waves (4, 8, 16), block_waves (16, 8, 4)."""
+        for k, v in knobs.items():
+            check(_lib.load().qkrylov_hamiltonian_set_tuning(self._h, self.TUNING_KNOBS[k], int(v)), f"set_tuning {k}")
+
+    def tuning(self):
+        out = {}
+        for k, i in self.TUNING_KNOBS.items():
+            v = ctypes.c_int()
+            check(_lib.load().qkrylov_hamiltonian_get_tuning(self._h, i, ctypes.byref(v)), "get_tuning")
+            out[k] = v.value
+        return out
+
     def time_apply(self, nvec=1, warmup=2, reps=10):
         ms = ctypes.c_double()
         check(_lib.load().qkrylov_hamiltonian_time_apply(self._h, nvec, warmup, reps, ctypes.byref(ms)), "time_apply")

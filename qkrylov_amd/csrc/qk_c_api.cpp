@@ -634,6 +634,32 @@ int qkrylov_hamiltonian_info(qkrylov_hamiltonian_h h, qkrylov_hamiltonian_info_t
     return QKRYLOV_SUCCESS;
 }
 
+int qkrylov_hamiltonian_set_tuning(qkrylov_hamiltonian_h h, int knob, int value)
+{
+    const auto* H = H_of(h);
+    if (!H) return QKRYLOV_ERROR_INVALID_ARG;
+    try {
+        H->engine()->set_tuning(knob, value);
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return QKRYLOV_ERROR_INVALID_ARG;
+    }
+    return QKRYLOV_SUCCESS;
+}
+
+int qkrylov_hamiltonian_get_tuning(qkrylov_hamiltonian_h h, int knob, int* value)
+{
+    const auto* H = H_of(h);
+    if (!H || !value) return QKRYLOV_ERROR_INVALID_ARG;
+    try {
+        *value = H->engine()->tuning(knob);
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return QKRYLOV_ERROR_INVALID_ARG;
+    }
+    return QKRYLOV_SUCCESS;
+}
+
 int qkrylov_hamiltonian_apply_device_f64(qkrylov_hamiltonian_h h, const double* x, double* y, int nvec, void* stream)
 {
     const auto* H = H_of(h);

@@ -157,7 +157,9 @@ Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpS
         const int nl = tables_->host.NL;
         const uint64_t lm = (nl >= 64) ? ~0ULL : ((1ULL << nl) - 1ULL);
         std::vector<QkBranchK> A, B;
+        T_.signs = 0;
         for (const auto& b : comp_.offdiag) {
+            if (b.M != 0) T_.signs = 1;
             QkBranchK k;
             k.hcm = (uint32_t)(b.cm >> nl);
             k.hcv = (uint32_t)(b.cv >> nl);
@@ -199,8 +201,39 @@ Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpS
     T_.NL = tables_->host.NL;
     T_.lrank_u16 = tables_->host.lrank_u16 ? 1 : 0;
     T_.nlo = 1 << tables_->host.NL;
+    T_.blk_waves = 8;
+    T_.apply_waves = 16;
+    // one XCD segment each for the single-vector apply (its x mostly fits the
+    // caches); one front for the block apply, whose 512 B rows do not
+    T_.sched = 0;
+    T_.sched_block = 1;
     table_bytes_ = tables_->tiles.bytes() + tables_->hoff.bytes() + tables_->lrank.bytes() + tables_->lolist.bytes() +
                    br_.bytes() + brk_.bytes() + dg_.bytes();
+}
+
+int Engine::tuning(int knob) const
+{
+    switch (knob) {
+    case 0: return T_.sched;
+    case 1: return T_.sched_block;
+    case 2: return T_.apply_waves;
+    case 3: return T_.blk_waves;
+    default: throw std::invalid_argument("unknown tuning knob");
+    }
+}
+
+void Engine::set_tuning(int knob, int value) const
+{
+    const bool sched_ok = value == 0 || value == 1;
+    const bool waves_ok = value == 4 || value == 8 || value == 16;
+    switch (knob) {
+    case 0: if (!sched_ok) break; T_.sched = value; return;
+    case 1: if (!sched_ok) break; T_.sched_block = value; return;
+    case 2: if (!waves_ok) break; T_.apply_waves = value; return;
+    case 3: if (!waves_ok) break; T_.blk_waves = value; return;
+    default: throw std::invalid_argument("unknown tuning knob");
+    }
+    throw std::invalid_argument("tuning value out of range");
 }
 
 void Engine::apply(const double* x, double* y, bool cplx, hipStream_t s) const
@@ -375,6 +408,7 @@ RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool
     alpha = mk((kcap + 2) * sizeof(double));
     nrm = mk((kcap + 3) * sizeof(double));
     npA = qkl_apply_partials(E.tables());
+    if (npA <= 0) throw std::runtime_error("apply: the operator tables do not fit in LDS");
     NB = qkl_blas_partials();
     partA = mk(npA * sizeof(double));
     partB = mk(NB * sizeof(double));
@@ -394,6 +428,9 @@ void RealCore::init_norm()
 // one step k: w = H v_k - beta v_{k-1}; alpha_k; q_{k+1}; nrm[k+1]
 void RealCore::step(int k, const double* basis)
 {
+    // the partial buffer was sized for the launch shape of the tuning at
+    // construction; a retuned engine would write past it
+    if (qkl_apply_partials(E.tables()) != npA) throw std::logic_error("apply launch shape changed during a run");
     lcheck(qkl_apply(E.tables(), 0, E.wide() ? 1 : 0, cur(), wrk(), k, prv(), nrm.d(), partA.d(), s), "apply");
     lcheck(qkl_reduce(partA.d(), npA, 1, 1, 0, 0, alpha.d(), k, 1, s), "reduce alpha");
     lcheck(qkl_lanczos_update(0, D, 1, 1, wrk(), cur(), alpha.d(), nrm.d(), k, 1, partB.d(), s), "update");

@@ -74,6 +74,11 @@ public:
     const Compiled& compiled() const { return comp_; }
     int NL() const { return tables_->host.NL; }
     size_t table_bytes() const { return table_bytes_; }
+    // launch tuning of the apply kernels (knobs in qkrylov_amd.h); results
+    // do not depend on it. Not to be changed while applies are in flight or
+    // a solver on this engine is running.
+    int tuning(int knob) const;
+    void set_tuning(int knob, int value) const;
 
     // y = H^dagger-convention apply of the reference (y[a] = sum conj(amp) x[b])
     void apply(const double* x, double* y, bool cplx, hipStream_t s) const;
@@ -87,7 +92,7 @@ private:
     Compiled comp_;
     std::shared_ptr<const BasisTables> tables_;
     DevBuf br_, brk_, dg_;
-    QkTables T_{};
+    mutable QkTables T_{};
     size_t table_bytes_ = 0;
 };
 

@@ -21,14 +21,13 @@ using namespace qk;
 #define QK_TPB 256
 #define QK_WPB 4
 #define QK_BLAS_BLOCKS 1024
-#define QK_MAX_APPLY_BLOCKS 2048
 
 namespace {
 
 struct QkScal {
     uint32_t lomask;
     uint32_t ntiles;
-    int32_t nbr, ndg, NL, nlo, nA, nB;
+    int32_t nbr, ndg, NL, nlo, nA, nB, sched;
     double dcr, dci;
 };
 
@@ -43,6 +42,7 @@ QkScal make_scal(const QkTables* T)
     s.ndg = T->ndg;
     s.NL = T->NL;
     s.nlo = T->nlo;
+    s.sched = T->sched;
     s.dcr = T->dconst_re;
     s.dci = T->dconst_im;
     return s;
@@ -57,16 +57,21 @@ __device__ __forceinline__ int popc<uint32_t>(uint32_t x) { return __builtin_pop
 template <>
 __device__ __forceinline__ int popc<uint64_t>(uint64_t x) { return __builtin_popcountll(x); }
 
-// XCD-aware tile schedule: workgroups are dispatched round-robin over the 8
-// XCDs, so workgroup b runs on XCD b % 8. Each XCD gets one contiguous
-// segment of tiles, and its waves sweep that segment together, keeping the
-// rows (and most gather targets) in flight on one XCD inside one window of
-// its L2.
-__device__ __forceinline__ void tile_schedule(uint32_t ntiles, uint32_t& t0, uint32_t& tend, uint32_t& step)
+// Tile schedules. Workgroups are dispatched round-robin over the 8 XCDs, so
+// workgroup b runs on XCD b % 8.
+//  sched 0: each XCD gets one contiguous segment of tiles and its waves
+//           sweep that segment together, keeping the rows (and the in-block
+//           gather targets) in flight on one XCD inside one window of its L2;
+//  sched 1: all waves of the grid sweep the tiles together in one front
+//           (wave g takes tiles g, g + G, ...), so the rows in flight on all
+//           XCDs share one window of the memory-side cache, which then also
+//           holds the targets of branches that leave the block.
+__device__ __forceinline__ void tile_schedule(uint32_t ntiles, int sched, uint32_t& t0, uint32_t& tend,
+                                              uint32_t& step)
 {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t w = rfl(threadIdx.x >> 6), wpb = blockDim.x >> 6;
-    if (nb >= 8 && (nb & 7u) == 0) {
+    if (sched == 0 && nb >= 8 && (nb & 7u) == 0) {
         const uint32_t xcd = b & 7u, loc = b >> 3, nloc = nb >> 3;
         const uint32_t seg = (ntiles + 7u) / 8u;
         const uint32_t s0 = min(xcd * seg, ntiles), s1 = min(s0 + seg, ntiles);
@@ -152,11 +157,13 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // ------------------------------------------------------------------ apply
 // Per-wave LDS scratch of the apply kernel: for every branch the tile's
-// uniform data (target block offset, hi-condition flags, hi parity), and the
-// compacted list of the hi-only branches the tile's hi takes.
+// uniform data (target block offset; the two values lo & lcm may take for
+// the branch to apply, given what the tile's hi allows, or an impossible
+// value; hi parity), and the compacted list of the hi-only branches the
+// tile's hi takes.
 __host__ __device__ __forceinline__ size_t apply_wave_bytes(int nbk, int nA)
 {
-    return (((size_t)nbk * 8 + (size_t)nA * 20) + 15) & ~(size_t)15;
+    return (((size_t)nbk * 16 + (size_t)nA * 20) + 15) & ~(size_t)15;
 }
 
 // Per-workgroup LDS copy of the lo side of the per-lane branches: 16 B of
@@ -166,8 +173,8 @@ __host__ __device__ __forceinline__ size_t apply_branch_bytes(int nB)
     return (((size_t)nB * (16 + 4 + 8 + 8)) + 15) & ~(size_t)15;
 }
 
-template <typename ST, bool CPLX, bool U16, bool LZ>
-__global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
+template <typename ST, bool CPLX, bool U16, bool LZ, bool SIGNS>
+__global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
                                                    const uint32_t* __restrict__ lrank,
                                                    const uint32_t* __restrict__ lolist,
                                                    const QkBranchK* __restrict__ brk,
@@ -177,7 +184,7 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
                                                    double* __restrict__ partials, int table_bytes)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double wsum[QK_WPB];
+    __shared__ double wsum[16];
     const int nA = sc.nA, nB = sc.nB, nbk = sc.nA + sc.nB;
     uint4* lb4 = reinterpret_cast<uint4*>(smem + table_bytes);
     uint32_t* lbm = reinterpret_cast<uint32_t*>(lb4 + nB);
@@ -195,8 +202,8 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
     const uint32_t lane = threadIdx.x & 63u;
     const int NL = sc.NL;
     unsigned char* wb = smem + table_bytes + apply_branch_bytes(nB) + (threadIdx.x >> 6) * apply_wave_bytes(nbk, nA);
-    uint2* tb = reinterpret_cast<uint2*>(wb);                          // [nbk] {block offset, flags}
-    double* a_cr = reinterpret_cast<double*>(wb + (size_t)nbk * 8);    // [nA] taken hi-only values
+    uint4* tb = reinterpret_cast<uint4*>(wb);                          // [nbk] {block offset, va, vb, hpar}
+    double* a_cr = reinterpret_cast<double*>(wb + (size_t)nbk * 16);   // [nA] taken hi-only values
     double* a_ci = a_cr + nA;
     uint32_t* a_base = reinterpret_cast<uint32_t*>(a_ci + nA);         // [nA] their block offsets
     const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
@@ -209,7 +216,7 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
     double part = 0.0;
 
     uint32_t t0, tend, step;
-    tile_schedule(sc.ntiles, t0, tend, step);
+    tile_schedule(sc.ntiles, sc.sched, t0, tend, step);
     for (uint32_t t = t0; t < tend; t += step) {
         const QkTile tl = tiles[t];
         const uint32_t hi = tl.hi;
@@ -233,9 +240,11 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
                 const uint32_t hm = hi & B.hcm;
                 const uint32_t u0 = (hm == B.hcv) ? 1u : 0u;
                 const uint32_t u1 = (((B.halt | B.lalt) != 0) && (hm == (B.hcv ^ B.halt))) ? 2u : 0u;
-                const uint32_t hpar = (uint32_t)(__builtin_popcount(hi & B.hM) & 1);
+                const uint32_t hpar = SIGNS ? (uint32_t)(__builtin_popcount(hi & B.hM) & 1) : 0u;
+                // lo & lcm < 2^NL <= 2^20, so ~lcm is a value it never takes
+                const uint32_t never = ~B.lcm;
                 base = hoff[hi ^ B.hF];
-                tb[b] = make_uint2(base, u0 | u1 | (hpar << 2));
+                tb[b] = make_uint4(base, u0 ? B.lcv : never, u1 ? (B.lcv ^ B.lalt) : never, hpar);
                 takeA = (b < nA) && (u0 | u1);
                 vr = hpar ? -B.re : B.re;
                 vi = hpar ? -B.im : B.im;
@@ -281,24 +290,28 @@ __global__ __launch_bounds__(QK_TPB) void k_apply(const QkTile* __restrict__ til
             accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
         }
 
-        // (4) remaining branches: per-lane condition on lo, lo rank from LDS
-        // (lF == 0 reads lrank[lo] = the row's own rank), eight in flight
+        // (4) remaining branches: per-lane test of lo & lcm against the two
+        // tile-uniform values, lo rank from LDS (lF == 0 reads lrank[lo] =
+        // the row's own rank), eight in flight
         for (int g = 0; g < nB; g += 8) {
             uint32_t rt[8];
             double cr[8], ci[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const uint4 L = lb4[g + u];            // lcm, lcv, lalt, lF
-                const uint2 tv = tb[nA + g + u];       // block offset, flags
+                const uint4 tv = tb[nA + g + u];       // block offset, va, vb, hpar
                 const uint32_t lf = lo & L.x;
-                const uint32_t ok = ((tv.y & 1u) & (uint32_t)(lf == L.y)) |
-                                    ((tv.y >> 1) & 1u & (uint32_t)(lf == (L.y ^ L.z)));
+                const bool ok = (lf == tv.y) || (lf == tv.z);
                 const uint32_t lr = lds_rank<U16>(smem, lo ^ L.w);
-                const uint32_t neg = ((uint32_t)__builtin_popcount(lo & lbm[g + u]) ^ (tv.y >> 2)) & 1u;
-                const double vr = lbre[g + u], vi = CPLX ? lbim[g + u] : 0.0;
+                double vr = lbre[g + u], vi = CPLX ? lbim[g + u] : 0.0;
+                if (SIGNS) {
+                    const uint32_t neg = ((uint32_t)__builtin_popcount(lo & lbm[g + u]) ^ tv.w) & 1u;
+                    vr = neg ? -vr : vr;
+                    vi = neg ? -vi : vi;
+                }
                 rt[u] = ok ? tv.x + lr : row;
-                cr[u] = ok ? (neg ? -vr : vr) : 0.0;
-                ci[u] = ok ? (neg ? -vi : vi) : 0.0;
+                cr[u] = ok ? vr : 0.0;
+                ci[u] = ok ? vi : 0.0;
             }
             accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
         }
@@ -359,22 +372,38 @@ __host__ __device__ __forceinline__ size_t block_wave_bytes(int nbk, int nA, int
     return ((size_t)cap * 6 + 64 * 8 + (size_t)nbk * 8 + (size_t)nA * 8 + 15) & ~(size_t)15;
 }
 
-// issue the gathers of entries [e0, e0 + 8) of the list (entries at or past
-// `total` gather row `tfill` of X and carry id kNop)
-__device__ __forceinline__ void block_gather8(const uint32_t* l_tgt, const uint16_t* l_vid, uint32_t e0,
-                                              uint32_t total, uint32_t tfill, const double* __restrict__ Xc,
-                                              const double* __restrict__ Pc, size_t ld, uint32_t kPrev,
-                                              uint32_t kNop, double* xg, uint32_t* vg)
+// issue the gathers of entries [e0, e0 + G) of the list (entries at or past
+// `total` gather row `tfill` of X and carry id kNop, so the walk can always
+// keep the next group in flight); e0 is a multiple of G and the list is
+// 16-byte aligned, so targets and ids are read as 16-byte LDS vectors
+template <int G>
+__device__ __forceinline__ void block_gather(const uint32_t* l_tgt, const uint16_t* l_vid, uint32_t e0,
+                                             uint32_t total, uint32_t tfill, const double* __restrict__ Xc,
+                                             const double* __restrict__ Pc, size_t ld, uint32_t kPrev,
+                                             uint32_t kNop, double* xg, uint32_t* vg)
 {
     const uint32_t eb = e0 < total ? e0 : 0u;
-    const uint4 ta = *reinterpret_cast<const uint4*>(l_tgt + eb);
-    const uint4 tc = *reinterpret_cast<const uint4*>(l_tgt + eb + 4);
-    const uint4 vv = *reinterpret_cast<const uint4*>(l_vid + eb);
-    const uint32_t tg[8] = {ta.x, ta.y, ta.z, ta.w, tc.x, tc.y, tc.z, tc.w};
-    const uint32_t vd[8] = {vv.x & 0xFFFFu, vv.x >> 16, vv.y & 0xFFFFu, vv.y >> 16,
-                            vv.z & 0xFFFFu, vv.z >> 16, vv.w & 0xFFFFu, vv.w >> 16};
+    uint32_t tg[G], vd[G];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int q = 0; q < G / 4; ++q) {
+        const uint4 t4 = *reinterpret_cast<const uint4*>(l_tgt + eb + 4 * q);
+        tg[4 * q] = t4.x;
+        tg[4 * q + 1] = t4.y;
+        tg[4 * q + 2] = t4.z;
+        tg[4 * q + 3] = t4.w;
+    }
+#pragma unroll
+    for (int q = 0; q < G / 8; ++q) {
+        const uint4 v4 = *reinterpret_cast<const uint4*>(l_vid + eb + 8 * q);
+        const uint32_t w4[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            vd[8 * q + 2 * k] = w4[k] & 0xFFFFu;
+            vd[8 * q + 2 * k + 1] = w4[k] >> 16;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
         const bool in = e0 + (uint32_t)u < total;
         uint32_t t = __builtin_amdgcn_readfirstlane(tg[u]);
         uint32_t v = __builtin_amdgcn_readfirstlane(vd[u]);
@@ -386,8 +415,8 @@ __device__ __forceinline__ void block_gather8(const uint32_t* l_tgt, const uint1
     }
 }
 
-template <typename ST, bool U16, bool LZ>
-__global__ __launch_bounds__(QK_TPB * 4) void k_apply_block(
+template <typename ST, bool U16, bool LZ, int NW>
+__global__ __launch_bounds__(64 * NW) void k_apply_block(
     const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff, const uint32_t* __restrict__ lrank,
     const uint32_t* __restrict__ lolist, const QkBranchK* __restrict__ brk, const QkDiagTerm* __restrict__ dg,
     QkScal sc, const double* __restrict__ X, double* __restrict__ Y, int ld, int c0, int nc, int lz_k,
@@ -440,7 +469,7 @@ __global__ __launch_bounds__(QK_TPB * 4) void k_apply_block(
     double part = 0.0;
 
     uint32_t t0, tend, step;
-    tile_schedule(sc.ntiles, t0, tend, step);
+    tile_schedule(sc.ntiles, sc.sched, t0, tend, step);
     for (uint32_t t = t0; t < tend; t += step) {
         const QkTile tl = tiles[t];
         const uint32_t hi = tl.hi;
@@ -559,9 +588,10 @@ __global__ __launch_bounds__(QK_TPB * 4) void k_apply_block(
                     if (colok) __builtin_nontemporal_store(acc, Yc + r);
                 }
             };
+            constexpr int G = 128 / NW;   // gathers per group: 8, 16 or 32 with 16, 8 or 4 waves
             auto consume = [&](const double* xg, const uint32_t* vg) {
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < G; ++u) {
                     const uint32_t v = vg[u];
                     if (v < kDiag) {
                         acc = fma(l_vals[v], xg[u], acc);
@@ -576,13 +606,13 @@ __global__ __launch_bounds__(QK_TPB * 4) void k_apply_block(
                 }
             };
             const uint32_t tfill = __builtin_amdgcn_readfirstlane(l_tgt[0]);
-            double xa[8], xb[8];
-            uint32_t va[8], vb[8];
-            block_gather8(l_tgt, l_vid, 0, total, tfill, Xc, Pc, lds, kPrev, kNop, xa, va);
-            for (uint32_t e0 = 0; e0 < total; e0 += 16) {
-                block_gather8(l_tgt, l_vid, e0 + 8, total, tfill, Xc, Pc, lds, kPrev, kNop, xb, vb);
+            double xa[G], xb[G];
+            uint32_t va[G], vb[G];
+            block_gather<G>(l_tgt, l_vid, 0, total, tfill, Xc, Pc, lds, kPrev, kNop, xa, va);
+            for (uint32_t e0 = 0; e0 < total; e0 += 2 * G) {
+                block_gather<G>(l_tgt, l_vid, e0 + G, total, tfill, Xc, Pc, lds, kPrev, kNop, xb, vb);
                 consume(xa, va);
-                block_gather8(l_tgt, l_vid, e0 + 16, total, tfill, Xc, Pc, lds, kPrev, kNop, xa, va);
+                block_gather<G>(l_tgt, l_vid, e0 + 2 * G, total, tfill, Xc, Pc, lds, kPrev, kNop, xa, va);
                 consume(xb, vb);
             }
             if (cur != kNone) finish();
@@ -995,22 +1025,44 @@ __global__ __launch_bounds__(QK_TPB) void k_precond(size_t n, const double* __re
 }
 
 // ------------------------------------------------------------------ launch helpers
-inline int apply_blocks(uint32_t ntiles)
-{
-    uint32_t nb = (ntiles + QK_WPB - 1) / QK_WPB;
-    if (nb > QK_MAX_APPLY_BLOCKS) nb = QK_MAX_APPLY_BLOCKS;
-    if (nb >= 8) nb = (nb + 7u) & ~7u;
-    if (nb == 0) nb = 1;
-    return (int)nb;
-}
-
 // LDS bytes of the lo-rank table, rounded to 16 so what follows it is aligned
 inline size_t table_bytes(const QkTables* T) { return ((size_t)T->nlo * (T->lrank_u16 ? 2 : 4) + 15) & ~(size_t)15; }
 
-// Block-apply launch shape: 16, 8 or 4 waves per workgroup, sized so one
-// CU holds 16 waves (one workgroup of 16, two of 8, four of 4), with as
-// large an entry list per wave as fits; the grid is one resident wave of
-// workgroups over the 256 CUs.
+// Single-vector apply launch shape: the preferred number of waves per
+// workgroup (4, 8 or 16; they share one LDS copy of the tables), or fewer
+// when the LDS of that many waves exceeds 160 KB; the grid holds 8192
+// waves at most, a multiple of 8 workgroups (XCD segments).
+struct ApplyCfg {
+    int wpb, nb;
+    size_t lds;
+};
+
+inline bool apply_config(const QkTables* T, ApplyCfg& c)
+{
+    int wpb = (T->apply_waves == 8 || T->apply_waves == 16) ? T->apply_waves : 4;
+    for (; wpb >= 1; wpb /= 2) {
+        const size_t lds = table_bytes(T) + apply_branch_bytes(T->nB) +
+                           (size_t)wpb * apply_wave_bytes(T->nA + T->nB, T->nA);
+        if (lds > 160 * 1024) continue;
+        uint32_t nb = (T->ntiles + wpb - 1) / wpb;
+        const uint32_t maxb = 8192u / (uint32_t)wpb;
+        if (nb > maxb) nb = maxb;
+        if (nb >= 8) nb = (nb + 7u) & ~7u;
+        if (nb == 0) nb = 1;
+        c.wpb = wpb;
+        c.nb = (int)nb;
+        c.lds = lds;
+        return true;
+    }
+    return false;
+}
+
+// Block-apply launch shape: one workgroup per CU (up to 150 KB of LDS) of
+// 16, 8 or 4 waves, walking 8, 16 or 32 gathers per group, so every shape
+// keeps about 128 gathers per CU in flight while fewer waves keep fewer
+// tiles (and their rows) live in each XCD's L2. The preferred width is
+// tried first, then the others, until the entry list of a wave holds at
+// least one row's entries.
 struct BlockCfg {
     int nw, cap, nb;
     size_t lds;
@@ -1020,22 +1072,22 @@ inline bool block_config(const QkTables* T, BlockCfg& c)
 {
     const int nbk = T->nA + T->nB;
     const size_t shared = table_bytes(T) + block_branch_bytes(T->nB) + (size_t)nbk * 16;
-    const int widths[3] = {16, 8, 4};
+    const size_t budget = (size_t)150 * 1024;
+    const int pref = (T->blk_waves == 4 || T->blk_waves == 8) ? T->blk_waves : 16;
+    const int widths[3] = {pref, pref == 16 ? 8 : 16, pref == 4 ? 8 : 4};
     for (int w = 0; w < 3; ++w) {
         const int nw = widths[w];
-        const size_t budget = (size_t)150 * 1024 * nw / 16;
-        if (shared + (size_t)nw * block_wave_bytes(nbk, T->nA, nbk + 8) > budget) continue;
+        if (shared >= budget) break;
         const size_t per = (budget - shared) / nw;
-        long cap = ((long)per - (long)block_wave_bytes(nbk, T->nA, 0) - 16) / 6;
-        if (cap > 4096) cap = 4096;
-        cap &= ~7L;
+        long cap = ((long)per - (long)block_wave_bytes(nbk, T->nA, 0) - 32) / 6;
+        if (cap > 8192) cap = 8192;
+        cap &= ~31L;
         if (cap < nbk + 2) continue;
         c.nw = nw;
         c.cap = (int)cap;
         c.lds = shared + (size_t)nw * block_wave_bytes(nbk, T->nA, (int)cap);
         uint32_t nb = (T->ntiles + nw - 1) / nw;
-        const uint32_t maxb = 256u * (16u / (uint32_t)nw);
-        if (nb > maxb) nb = maxb;
+        if (nb > 256u) nb = 256u;
         if (nb >= 8) nb = (nb + 7u) & ~7u;
         if (nb == 0) nb = 1;
         c.nb = (int)nb;
@@ -1053,15 +1105,27 @@ inline int grid_for(size_t n)
 
 inline int rc(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
-template <typename ST, bool CPLX, bool U16, bool LZ>
+template <typename ST, bool CPLX, bool U16, bool LZ, bool SIGNS>
 int launch_apply(const QkTables* T, const double* x, double* y, int k, const double* prev, const double* nrm,
                  double* partials, hipStream_t s)
 {
-    const int nb = apply_blocks(T->ntiles);
-    const size_t lds = table_bytes(T) + apply_branch_bytes(T->nB) + (size_t)QK_WPB * apply_wave_bytes(T->nA + T->nB, T->nA);
-    if (lds > 160 * 1024) return -2;
-    hipLaunchKernelGGL((k_apply<ST, CPLX, U16, LZ>), dim3(nb), dim3(QK_TPB), lds, s, T->tiles, T->hoff, T->lrank,
+    ApplyCfg c;
+    if (!apply_config(T, c)) return -2;
+    hipLaunchKernelGGL((k_apply<ST, CPLX, U16, LZ, SIGNS>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff, T->lrank,
                        T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T));
+    return rc(hipGetLastError());
+}
+
+template <typename ST, bool U16, bool LZ, int NW>
+int launch_block_nw(const QkTables* T, const BlockCfg& c, const double* X, double* Y, int ld, int c0, int nc, int k,
+                 const double* P, const double* nrm, int ld_sc, double* partials, int npart, hipStream_t s)
+{
+    if (LZ && npart != c.nb) return -1;
+    QkScal sc = make_scal(T);
+    sc.sched = T->sched_block;
+    hipLaunchKernelGGL((k_apply_block<ST, U16, LZ, NW>), dim3(c.nb), dim3(64 * NW), c.lds, s, T->tiles, T->hoff,
+                       T->lrank, T->lolist, T->brk, T->dg, sc, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
+                       npart, c.cap, (int)table_bytes(T));
     return rc(hipGetLastError());
 }
 
@@ -1069,18 +1133,20 @@ template <typename ST, bool U16, bool LZ>
 int launch_block(const QkTables* T, const BlockCfg& c, const double* X, double* Y, int ld, int c0, int nc, int k,
                  const double* P, const double* nrm, int ld_sc, double* partials, int npart, hipStream_t s)
 {
-    if (LZ && npart != c.nb) return -1;
-    hipLaunchKernelGGL((k_apply_block<ST, U16, LZ>), dim3(c.nb), dim3(64 * c.nw), c.lds, s, T->tiles, T->hoff,
-                       T->lrank, T->lolist, T->brk, T->dg, make_scal(T), X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
-                       npart, c.cap, (int)table_bytes(T));
-    return rc(hipGetLastError());
+    if (c.nw == 4) return launch_block_nw<ST, U16, LZ, 4>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+    if (c.nw == 8) return launch_block_nw<ST, U16, LZ, 8>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+    return launch_block_nw<ST, U16, LZ, 16>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
 }
 
 } // namespace
 
 extern "C" {
 
-int qkl_apply_partials(const QkTables* T) { return apply_blocks(T->ntiles); }
+int qkl_apply_partials(const QkTables* T)
+{
+    ApplyCfg c;
+    return apply_config(T, c) ? c.nb : -2;
+}
 int qkl_block_partials(const QkTables* T)
 {
     BlockCfg c;
@@ -1096,17 +1162,20 @@ int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y,
     const bool u16 = T->lrank_u16 != 0;
     const bool lz = lz_k >= 0;
     if (lz && cplx) return -1;
+#define QK_DISPATCH_S(ST, C, SG)                                                                              \
+    if (u16) return lz ? launch_apply<ST, C, true, true, SG>(T, x, y, lz_k, prev, nrm, partials, s)         \
+                       : launch_apply<ST, C, true, false, SG>(T, x, y, lz_k, prev, nrm, partials, s);       \
+    else return lz ? launch_apply<ST, C, false, true, SG>(T, x, y, lz_k, prev, nrm, partials, s)            \
+                   : launch_apply<ST, C, false, false, SG>(T, x, y, lz_k, prev, nrm, partials, s);
 #define QK_DISPATCH(ST, C)                                                                                   \
-    if (u16) return lz ? launch_apply<ST, C, true, true>(T, x, y, lz_k, prev, nrm, partials, s)             \
-                       : launch_apply<ST, C, true, false>(T, x, y, lz_k, prev, nrm, partials, s);           \
-    else return lz ? launch_apply<ST, C, false, true>(T, x, y, lz_k, prev, nrm, partials, s)                \
-                   : launch_apply<ST, C, false, false>(T, x, y, lz_k, prev, nrm, partials, s);
+    if (T->signs) { QK_DISPATCH_S(ST, C, true) } else { QK_DISPATCH_S(ST, C, false) }
     if (wide) {
         if (cplx) { QK_DISPATCH(uint64_t, true) } else { QK_DISPATCH(uint64_t, false) }
     } else {
         if (cplx) { QK_DISPATCH(uint32_t, true) } else { QK_DISPATCH(uint32_t, false) }
     }
 #undef QK_DISPATCH
+#undef QK_DISPATCH_S
 }
 
 int qkl_apply_block(const QkTables* T, int wide, const double* X, double* Y, int ld, int c0, int nc, int lz_k,

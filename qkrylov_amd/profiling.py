@@ -4,11 +4,14 @@
 
     python -m qkrylov_amd.profiling kernels DIR/run_results.db
     python -m qkrylov_amd.profiling counters FETCH.db WRITE.db --copy-bytes N [--match k_apply<]
+    python -m qkrylov_amd.profiling pmc DB [DB ...] [--match k_apply<]
 
 `kernels` prints a per-kernel table (calls, mean/min/max duration) from a
 `--kernel-trace` run; `counters` prints FETCH_SIZE / WRITE_SIZE per dispatch
 (KiB) for the kernels whose name contains `--match`, scaled by what the same
-counter reads for the plain copy kernel `k_copy` of known size.
+counter reads for the plain copy kernel `k_copy` of known size; `pmc`
+prints the mean per dispatch of every counter collected in the databases
+for the matching kernels.
 """
 import argparse
 import sqlite3
@@ -41,6 +44,20 @@ def counter_values(db_path, counter):
         k = (int(did), name)
         acc[k] = acc.get(k, 0.0) + float(val)
     return [(d, n, v) for (d, n), v in sorted(acc.items())]
+
+
+def counter_means(db_path, match):
+    """{counter: (mean per dispatch, dispatches)} over the kernels whose name
+    contains `match`."""
+    con = sqlite3.connect(db_path)
+    rows = con.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection").fetchall()
+    con.close()
+    acc = {}
+    for did, name, ctr, val in rows:
+        if match in name:
+            acc.setdefault(ctr, {}).setdefault(int(did), 0.0)
+            acc[ctr][int(did)] += float(val)
+    return {c: (statistics.fmean(d.values()), len(d)) for c, d in acc.items()}
 
 
 def short_name(name):
@@ -88,7 +105,15 @@ def _main(argv=None):
     c.add_argument("write_db")
     c.add_argument("--copy-bytes", type=float, required=True)
     c.add_argument("--match", default="k_apply<")
+    m = sub.add_parser("pmc")
+    m.add_argument("dbs", nargs="+")
+    m.add_argument("--match", default="k_apply<")
     a = ap.parse_args(argv)
+    if a.cmd == "pmc":
+        for db in a.dbs:
+            for ctr, (v, n) in sorted(counter_means(db, a.match).items()):
+                print(f"{ctr:40s} {v:20.1f}   ({n} dispatches)")
+        return 0
     if a.cmd == "kernels":
         tab = kernel_table(a.db)
         tot = sum(v["total_ns"] for v in tab.values())

@@ -300,16 +300,17 @@ def test_block_apply_equals_single_columns(name, split):
     H = qa.MatrixFreeHamiltonian(basis, site, qa.OpSum(terms), split_bits=split)
     D = H.dimension
     rng = np.random.default_rng(11)
-    for nvec in (2, 5, 40, 64, 70):
+    for waves, nvec in ((16, 2), (16, 5), (8, 40), (4, 40), (16, 64), (8, 64), (4, 64), (4, 70)):
+        H.set_tuning(block_waves=waves)
         Xh = rng.uniform(-1, 1, (D, nvec))
         X, Y = qa.DeviceArray.from_host(Xh), qa.DeviceArray(D * nvec)
-        print(f"block apply {name} split={split} D={D} nvec={nvec}", flush=True)
+        print(f"block apply {name} split={split} D={D} nvec={nvec} waves={waves}", flush=True)
         H.apply_device(X.ptr, Y.ptr, nvec)
         Yh = Y.to_host().reshape(D, nvec)
         for c in (0, nvec // 2, nvec - 1):
             x1, y1 = qa.DeviceArray.from_host(Xh[:, c]), qa.DeviceArray(D)
             H.apply_device(x1.ptr, y1.ptr, 1)
-            assert np.array_equal(Yh[:, c], y1.to_host()), (name, split, nvec, c)
+            assert np.array_equal(Yh[:, c], y1.to_host()), (name, split, nvec, waves, c)
         X.free()
         Y.free()
 
@@ -326,3 +327,29 @@ def test_block_apply_long_range_vs_oracle():
     for c in range(3):
         ref = om.apply(Xh[:, c] + 0j).real
         assert np.max(np.abs(Yh[:, c] - ref)) < 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", ["heis12", "tj2x4", "longrange16"])
+def test_schedules_give_identical_results(name):
+    """The launch tuning only changes which wave handles which tile: single
+    and block applies are bitwise the same under every schedule and
+    workgroup width."""
+    basis, site, terms = BLOCK_MODELS[name]()
+    H = qa.MatrixFreeHamiltonian(basis, site, qa.OpSum(terms))
+    D = H.dimension
+    Xh = np.random.default_rng(13).uniform(-1, 1, (D, 7))
+    X = qa.DeviceArray.from_host(Xh)
+    out = []
+    for sched, waves in ((0, 4), (1, 4), (0, 8), (1, 16)):
+        H.set_tuning(schedule=sched, block_schedule=sched, apply_waves=waves)
+        Y, y = qa.DeviceArray(D * 7), qa.DeviceArray(D)
+        x1 = qa.DeviceArray.from_host(Xh[:, 0])
+        H.apply_device(X.ptr, Y.ptr, 7)
+        H.apply_device(x1.ptr, y.ptr, 1)
+        out.append((Y.to_host(), y.to_host()))
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0]) and np.array_equal(out[0][1], o[1])
+    with pytest.raises(RuntimeError):
+        H.set_tuning(schedule=2)
+    with pytest.raises(RuntimeError):
+        H.set_tuning(apply_waves=5)

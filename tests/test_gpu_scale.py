@@ -46,7 +46,7 @@ def test_heisenberg_ring20_lanczos_vs_bethe_and_oracle():
 
 
 CONFIGS = {
-    "j1j2_4x8": (O.SPIN, 32, dict(use_sz=1, sz2=0), lambda: models.j1j2_square(4, 8, 1.0, 0.5),
+    "j1j2_4x8": (O.SPIN, 32, dict(use_sz=1, sz2=0), lambda: models.j1j2_square(8, 4, 1.0, 0.5),
                  lambda: qa.SpinHalfBasis(32, sz2=0), 601080390),
     "hubbard16": (O.HUBBARD, 16, dict(use_nup=1, nup=8, use_ndn=1, ndn=8), lambda: models.hubbard_chain(16, 1.0, 4.0),
                   lambda: qa.HubbardBasis(16, nup=8, ndn=8), 165636900),
