@@ -142,6 +142,12 @@ def bench_ftlm(args):
     betas = 1.0 / T
     H, terms = make_hamiltonian("tj4x5")
     D = H.dimension
+    tune = None
+    if args.autotune:
+        # the split for the single-vector runs, then the block launch shape
+        # at that split, so both modes run their best configuration
+        tune = [H.autotune(nvec=1, allow_split=True), H.autotune(nvec=R, allow_split=False)]
+        log(f"[bench-ftlm] autotune: {tune}")
     log(f"[bench-ftlm] D={D} R={R} M={M}")
     b = qa.ftlm_batched(H, R, M, betas, seed=42)
     log(f"[bench-ftlm] batched: {b['seconds']:.3f}s")
@@ -170,7 +176,8 @@ def bench_ftlm(args):
         "value": round(R / b["seconds"], 3), "unit": "random-vector solves/s", "n_gpus": 1, "steps": M,
         "warmup": 0, "ms_per_step": round(1e3 * b["seconds"] / M, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (Rademacher vectors from the counter RNG)",
-        "config": {"workload": WORKLOADS["tj4x5"]["desc"], "D": D, "R": R, "M": M, "temperatures": 64},
+        "config": {"workload": WORKLOADS["tj4x5"]["desc"], "D": D, "R": R, "M": M, "temperatures": 64,
+                   "NL": H.info["NL"], "tuning": H.tuning(), "autotune": tune},
         "batched_seconds": round(b["seconds"], 4),
         "sequential_seconds": round(seq["seconds"], 4) if seq else None,
         "speedup_vs_sequential": round(seq["seconds"] / b["seconds"], 3) if seq else None,
@@ -209,12 +216,24 @@ def bench_serve(args):
     print(json.dumps(res), flush=True)
 
 
-def measure_traffic(args):
+def config_flags(args):
+    """Command-line flags that reproduce this run's split and launch tuning."""
+    f = ["--split-bits", str(args.split_bits), "--nvec", str(args.nvec)]
+    if args.sched is not None:
+        f += ["--sched", str(args.sched)]
+    if args.block_waves is not None:
+        f += ["--block-waves", str(args.block_waves)]
+    if args.apply_waves is not None:
+        f += ["--apply-waves", str(args.apply_waves)]
+    return f
+
+
+def measure_traffic(args, flags):
     """HBM-side bytes per apply launch from two child runs of this script
-    under `rocprofv3 --pmc` (FETCH_SIZE, then WRITE_SIZE), each also running
-    one plain copy kernel of known size whose counter reading gives the
-    scale factor.  Runs before this process touches the GPU.  Returns
-    (bytes per launch, detail dict) or (None, reason)."""
+    under `rocprofv3 --pmc` (FETCH_SIZE, then WRITE_SIZE) with the given
+    configuration flags, each also running one plain copy kernel of known
+    size whose counter reading gives the scale factor.  Returns (bytes per
+    launch, detail dict) or (None, reason)."""
     import shutil
     import subprocess
     import tempfile
@@ -229,14 +248,8 @@ def measure_traffic(args):
     try:
         for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
             out = os.path.join(tmp, ctr)
-            sched = [] if args.sched is None else ["--sched", str(args.sched)]
-            if args.block_waves is not None:
-                sched += ["--block-waves", str(args.block_waves)]
-            if args.apply_waves is not None:
-                sched += ["--apply-waves", str(args.apply_waves)]
             cmd = ["timeout", "-s", "KILL", "300", prof, "--pmc", ctr, "-d", out, "-o", "run", "--",
-                   sys.executable, os.path.abspath(__file__), "--workload", args.workload,
-                   "--split-bits", str(args.split_bits), "--nvec", str(args.nvec)] + sched + [
+                   sys.executable, os.path.abspath(__file__), "--workload", args.workload] + flags + [
                    "--steps", "2", "--warmup", "0", "--no-lanczos",
                    "--no-cpu", "--no-traffic", "--copy-calib-gib", str(calib_gib)]
             log(f"[bench] traffic pass {ctr}: {' '.join(cmd[4:8])} ...")
@@ -277,6 +290,9 @@ def main():
                     help="tile schedule (0: one segment per XCD, 1: one front; default: the engine's choice)")
     ap.add_argument("--block-waves", type=int, default=None, help="waves per workgroup of the block apply (16, 8, 4)")
     ap.add_argument("--apply-waves", type=int, default=None, help="waves per workgroup of the apply (4, 8, 16)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the autotuner (which times the candidate launch tunings and lo-bit splits before the "
+                         "timed region and keeps the fastest)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per apply")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
@@ -288,8 +304,10 @@ def main():
         sys.exit(2)
 
     traffic, traffic_info = None, "not measured (--no-traffic)"
-    if not args.no_traffic and args.workload in WORKLOADS:
-        traffic, traffic_info = measure_traffic(args)
+    want_traffic = not args.no_traffic and args.workload in WORKLOADS
+    if want_traffic and not args.autotune:
+        # the counter runs go first, before this process touches the GPU
+        traffic, traffic_info = measure_traffic(args, config_flags(args))
         log(f"[bench] traffic: {traffic_info if traffic is None else f'{traffic / 1e9:.3f} GB per apply'}")
 
     import qkrylov_amd as qa
@@ -326,6 +344,21 @@ def main():
     if args.apply_waves is not None:
         tune["apply_waves"] = args.apply_waves
     H.set_tuning(**tune)
+    tune_report = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tune_report = H.autotune(nvec=args.nvec, allow_split=args.split_bits < 0)
+        tune_report["seconds"] = round(time.perf_counter() - t0, 2)
+        log(f"[bench] autotune: {tune_report}")
+        if want_traffic:
+            # counter runs as child processes with the configuration chosen here
+            t = H.tuning()
+            flags = ["--split-bits", str(H.info["NL"]), "--nvec", str(args.nvec),
+                     "--apply-waves", str(t["apply_waves"]), "--block-waves", str(t["block_waves"])]
+            sched = t["block_schedule"] if args.nvec > 1 else t["schedule"]
+            flags += ["--sched", str(sched)]
+            traffic, traffic_info = measure_traffic(args, flags)
+            log(f"[bench] traffic: {traffic_info if traffic is None else f'{traffic / 1e9:.3f} GB per apply'}")
     ms = H.time_apply(args.nvec, args.warmup, args.steps)
     L.qkrylov_amd_synchronize()
     wall = time.perf_counter() - t0
@@ -347,7 +380,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (x uniform[-1,1) from the counter RNG)",
         "config": {"workload": w["desc"], "D": D, "terms": len(terms), "columns": args.nvec, "NL": H.info["NL"],
-                   "tuning": H.tuning(),
+                   "tuning": H.tuning(), "autotune": tune_report,
                    "offdiag_branches": H.info["n_offdiag_branches"], "setup_s": round(t_setup, 3)},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 5),

@@ -266,6 +266,16 @@ public:
     std::shared_ptr<const qk::Engine> engine() const { return engine_; }
     std::shared_ptr<Basis> basis() const { return basis_; }
 
+    // Times the apply (nvec columns) under candidate launch tunings and, when
+    // allow_split, lo-bit splits around the current one, and keeps the
+    // fastest. Launch tunings never change results; another split changes
+    // the order in which a row's entries are added (the last bits).
+    struct TuneReport {
+        int split_bits, apply_waves, schedule, block_waves, block_schedule, candidates;
+        double ms_before, ms_after;
+    };
+    TuneReport autotune(int nvec = 1, bool allow_split = true);
+
 private:
     Device device_;
     std::shared_ptr<Basis> basis_;

@@ -69,6 +69,17 @@ enum {
     QKRYLOV_TUNE_BLOCK_WAVES = 3
 };
 int qkrylov_hamiltonian_set_tuning(qkrylov_hamiltonian_h h, int knob, int value);
+
+/* Autotuner: times y = H x (nvec columns, random vectors) under every
+ * launch tuning and, when allow_split, the lo-bit splits around the current
+ * one, and keeps the fastest for this handle. Launch tunings never change
+ * results; another split changes the order in which a row's entries are
+ * added, i.e. the last bits. Allocates two vectors of nvec columns. */
+typedef struct {
+    int split_bits, apply_waves, schedule, block_waves, block_schedule, candidates;
+    double ms_before, ms_after;
+} qkrylov_tune_report_t;
+int qkrylov_hamiltonian_autotune(qkrylov_hamiltonian_h h, int nvec, int allow_split, qkrylov_tune_report_t* report);
 int qkrylov_hamiltonian_get_tuning(qkrylov_hamiltonian_h h, int knob, int* value);
 
 /* ---- device-resident applies (pointers from qkrylov_amd_malloc or any

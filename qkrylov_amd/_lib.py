@@ -70,6 +70,11 @@ class ServeOptions(ctypes.Structure):
 
 
 # name -> (restype, argtypes)
+class TuneReport(ctypes.Structure):
+    _fields_ = [("split_bits", c_int), ("apply_waves", c_int), ("schedule", c_int), ("block_waves", c_int),
+                ("block_schedule", c_int), ("candidates", c_int), ("ms_before", c_double), ("ms_after", c_double)]
+
+
 SIGNATURES = {
     # sector
     "qkrylov_sector_create": (c_vp, []),
@@ -148,6 +153,7 @@ SIGNATURES = {
     "qkrylov_hamiltonian_info": (c_int, [c_vp, P(HamInfo)]),
     "qkrylov_hamiltonian_set_tuning": (c_int, [c_vp, c_int, c_int]),
     "qkrylov_hamiltonian_get_tuning": (c_int, [c_vp, c_int, P(c_int)]),
+    "qkrylov_hamiltonian_autotune": (c_int, [c_vp, c_int, c_int, P(TuneReport)]),
     "qkrylov_hamiltonian_apply_device_f64": (c_int, [c_vp, c_vp, c_vp, c_int, c_vp]),
     "qkrylov_hamiltonian_apply_device_complex_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "qkrylov_hamiltonian_diagonal_device_f64": (c_int, [c_vp, c_vp, c_int, c_vp]),

@@ -269,8 +269,11 @@ class MatrixFreeHamiltonian:
             L.qkrylov_opsum_destroy(oh)
         if not self._h:
             raise ValueError(f"Hamiltonian creation failed: {_lib.last_error()}")
+        self._refresh_info()
+
+    def _refresh_info(self):
         info = _lib.HamInfo()
-        check(L.qkrylov_hamiltonian_info(self._h, ctypes.byref(info)), "info")
+        check(_lib.load().qkrylov_hamiltonian_info(self._h, ctypes.byref(info)), "info")
         self.info = {f: getattr(info, f) for f, _ in info._fields_}
 
     def __del__(self):
@@ -331,6 +334,15 @@ class MatrixFreeHamiltonian:
         front over all XCDs), apply_waves (4, 8, 16), block_waves (16, 8, 4)."""
         for k, v in knobs.items():
             check(_lib.load().qkrylov_hamiltonian_set_tuning(self._h, self.TUNING_KNOBS[k], int(v)), f"set_tuning {k}")
+
+    def autotune(self, nvec=1, allow_split=True):
+        """Times the apply under candidate launch tunings (and lo-bit splits
+        when allow_split) and keeps the fastest; returns the report."""
+        r = _lib.TuneReport()
+        check(_lib.load().qkrylov_hamiltonian_autotune(self._h, int(nvec), 1 if allow_split else 0, ctypes.byref(r)),
+              "autotune")
+        self._refresh_info()
+        return {f: getattr(r, f) for f, _ in r._fields_}
 
     def tuning(self):
         out = {}

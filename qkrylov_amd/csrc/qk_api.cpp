@@ -150,6 +150,14 @@ MatrixFreeHamiltonian::MatrixFreeHamiltonian(std::shared_ptr<Basis> basis, std::
 
 MatrixFreeHamiltonian::~MatrixFreeHamiltonian() = default;
 
+MatrixFreeHamiltonian::TuneReport MatrixFreeHamiltonian::autotune(int nvec, bool allow_split)
+{
+    qk::TuneResult r;
+    engine_ = qk::autotune(basis_->spec(), static_cast<qk::Kind>(site_->kind()), to_qk(ops_), engine_, nvec,
+                           allow_split, qk::default_stream(), r);
+    return TuneReport{r.NL, r.apply_waves, r.sched, r.blk_waves, r.blk_sched, r.tried, r.ms_before, r.ms};
+}
+
 Index MatrixFreeHamiltonian::dimension() const { return static_cast<Index>(engine_->dim()); }
 bool MatrixFreeHamiltonian::is_real() const { return engine_->is_real(); }
 

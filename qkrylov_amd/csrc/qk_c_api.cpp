@@ -647,6 +647,25 @@ int qkrylov_hamiltonian_set_tuning(qkrylov_hamiltonian_h h, int knob, int value)
     return QKRYLOV_SUCCESS;
 }
 
+int qkrylov_hamiltonian_autotune(qkrylov_hamiltonian_h h, int nvec, int allow_split, qkrylov_tune_report_t* report)
+{
+    if (!h || !h->ptr || nvec <= 0 || nvec > 4096) return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    const auto r = h->ptr->autotune(nvec, allow_split != 0);
+    if (report) {
+        report->split_bits = r.split_bits;
+        report->apply_waves = r.apply_waves;
+        report->schedule = r.schedule;
+        report->block_waves = r.block_waves;
+        report->block_schedule = r.block_schedule;
+        report->candidates = r.candidates;
+        report->ms_before = r.ms_before;
+        report->ms_after = r.ms_after;
+    }
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
 int qkrylov_hamiltonian_get_tuning(qkrylov_hamiltonian_h h, int knob, int* value)
 {
     const auto* H = H_of(h);

@@ -211,6 +211,126 @@ Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpS
                    br_.bytes() + brk_.bytes() + dg_.bytes();
 }
 
+namespace {
+
+double time_engine(const Engine& E, const double* x, double* y, int nvec, hipStream_t s, int reps)
+{
+    auto run = [&]() {
+        if (nvec == 1) E.apply(x, y, false, s);
+        else E.apply_block(x, y, nvec, nvec, s);
+    };
+    run();   // warm-up
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, s), "record");
+    for (int i = 0; i < reps; ++i) run();
+    hip_check(hipEventRecord(e1, s), "record");
+    hip_check(hipEventSynchronize(e1), "sync");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / reps;
+}
+
+} // namespace
+
+std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& basis, Kind kind, const OpSum& ops,
+                                       const std::shared_ptr<const Engine>& current, int nvec, bool allow_split,
+                                       hipStream_t s, TuneResult& out)
+{
+    if (nvec < 1) throw std::invalid_argument("autotune: nvec must be positive");
+    if (nvec > 1 && !current->is_real()) throw std::invalid_argument("autotune: block apply needs a real Hamiltonian");
+    const size_t D = (size_t)current->dim();
+    const bool cplx1 = !current->is_real();
+    out = TuneResult{};
+    if (D == 0) return current;
+    const size_t n = D * (size_t)nvec * (cplx1 ? 2 : 1);
+    DevBuf x(n * sizeof(double)), y(n * sizeof(double));
+    lcheck(qkl_fill_random(0, n, 11, 0, 1, 1, 0, x.d(), s), "fill");
+    const int reps = D > ((size_t)1 << 24) ? 2 : 5;
+    auto timed = [&](const Engine& E) {
+        if (cplx1) {
+            E.apply(x.d(), y.d(), true, s);
+            hipEvent_t e0, e1;
+            hip_check(hipEventCreate(&e0), "event");
+            hip_check(hipEventCreate(&e1), "event");
+            hip_check(hipEventRecord(e0, s), "record");
+            for (int i = 0; i < reps; ++i) E.apply(x.d(), y.d(), true, s);
+            hip_check(hipEventRecord(e1, s), "record");
+            hip_check(hipEventSynchronize(e1), "sync");
+            float ms = 0.0f;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return (double)ms / reps;
+        }
+        return time_engine(E, x.d(), y.d(), nvec, s, reps);
+    };
+    out.ms_before = timed(*current);
+    double best_ms = out.ms_before;
+    int best_NL = current->NL(), best_tune[4];
+    for (int k = 0; k < 4; ++k) best_tune[k] = current->tuning(k);
+    bool improved = false;
+    std::vector<int> splits{current->NL()};
+    if (allow_split) {
+        const int nb = basis->nbits(), m = current->NL();
+        const int stepNL = basis->spinless() ? 1 : 2;
+        for (int d = -2; d <= 2; ++d) {
+            const int NL = m + d * stepNL;
+            if (d == 0 || NL < 4 || NL > std::min(nb, 16) || nb - NL > 28) continue;
+            splits.push_back(NL);
+        }
+    }
+    const int knob_w = nvec == 1 ? 2 : 3, knob_s = nvec == 1 ? 0 : 1;
+    for (int NL : splits) {
+        EngineOptions eo;
+        eo.NL = NL;
+        std::shared_ptr<Engine> E;
+        try {
+            E = std::make_shared<Engine>(basis, kind, ops, eo);
+        } catch (const std::exception&) {
+            continue;   // a split whose tables cannot be built is not a candidate
+        }
+        for (int k = 0; k < 4; ++k) E->set_tuning(k, current->tuning(k));
+        for (int w : {8, 16}) {
+            for (int sc : {0, 1}) {
+                E->set_tuning(knob_w, w);
+                E->set_tuning(knob_s, sc);
+                double ms;
+                try {
+                    ms = timed(*E);
+                } catch (const std::exception&) {
+                    continue;   // a launch shape that does not fit this split
+                }
+                ++out.tried;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best_NL = NL;
+                    for (int k = 0; k < 4; ++k) best_tune[k] = E->tuning(k);
+                    improved = true;
+                }
+            }
+        }
+    }
+    std::shared_ptr<const Engine> best = current;
+    if (improved) {
+        EngineOptions eo;
+        eo.NL = best_NL;
+        auto E = std::make_shared<Engine>(basis, kind, ops, eo);
+        for (int k = 0; k < 4; ++k) E->set_tuning(k, best_tune[k]);
+        best = E;
+    }
+    out.ms = best_ms;
+    out.NL = best->NL();
+    out.sched = best->tuning(0);
+    out.blk_sched = best->tuning(1);
+    out.apply_waves = best->tuning(2);
+    out.blk_waves = best->tuning(3);
+    return best;
+}
+
 int Engine::tuning(int knob) const
 {
     switch (knob) {

@@ -98,6 +98,20 @@ private:
 
 double now_s();
 
+// Autotuner: times y = H x (nvec columns) on random vectors under candidate
+// launch tunings and, when allow_split, lo-bit splits around the current
+// one; returns the fastest engine (the current one when nothing beats it).
+// Launch tunings never change results; a different split changes the order
+// in which a row's entries are added, i.e. the last bits.
+struct TuneResult {
+    int NL = -1, apply_waves = 0, sched = 0, blk_waves = 0, blk_sched = 0;
+    double ms = 0.0, ms_before = 0.0;
+    int tried = 0;
+};
+std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& basis, Kind kind, const OpSum& ops,
+                                       const std::shared_ptr<const Engine>& current, int nvec, bool allow_split,
+                                       hipStream_t s, TuneResult& out);
+
 // Replays the reference's convergence logic of lanczos_ground_state
 // (src/solvers/lanczos.cpp:135-183) on alpha_k and beta_k = |q_{k+1}|.
 struct LanczosReplay {

@@ -353,3 +353,30 @@ def test_schedules_give_identical_results(name):
         H.set_tuning(schedule=2)
     with pytest.raises(RuntimeError):
         H.set_tuning(apply_waves=5)
+
+
+def test_autotune_keeps_results():
+    """Autotuning launch shapes leaves the apply bitwise unchanged; letting it
+    move the lo-bit split changes only the summation order of a row."""
+    basis, site, terms = BLOCK_MODELS["longrange16"]()
+    H = qa.MatrixFreeHamiltonian(basis, site, qa.OpSum(terms))
+    D = H.dimension
+    xh = np.random.default_rng(14).uniform(-1, 1, D)
+    x, y = qa.DeviceArray.from_host(xh), qa.DeviceArray(D)
+    H.apply_device(x.ptr, y.ptr, 1)
+    y0 = y.to_host()
+    r = H.autotune(nvec=1, allow_split=False)
+    assert r["split_bits"] == H.info["NL"] and r["candidates"] >= 4 and r["ms_after"] <= r["ms_before"]
+    H.apply_device(x.ptr, y.ptr, 1)
+    assert np.array_equal(y.to_host(), y0)
+    r = H.autotune(nvec=1, allow_split=True)
+    assert r["split_bits"] == H.info["NL"]
+    H.apply_device(x.ptr, y.ptr, 1)
+    assert np.max(np.abs(y.to_host() - y0)) <= 1e-14 * np.max(np.abs(y0))
+    Xh = np.random.default_rng(15).uniform(-1, 1, (D, 16))
+    X, Y = qa.DeviceArray.from_host(Xh), qa.DeviceArray(D * 16)
+    H.apply_device(X.ptr, Y.ptr, 16)
+    Y0 = Y.to_host()
+    H.autotune(nvec=16, allow_split=False)
+    H.apply_device(X.ptr, Y.ptr, 16)
+    assert np.array_equal(Y.to_host(), Y0)
