@@ -47,7 +47,7 @@ class HamInfo(ctypes.Structure):
 
 class LanczosOptions(ctypes.Structure):
     _fields_ = [("maxiter", c_int), ("tol", c_double), ("reorthogonalize", c_int), ("fixed_steps", c_int),
-                ("want_vector", c_int), ("seed", c_u64), ("check_every", c_int)]
+                ("want_vector", c_int), ("seed", c_u64), ("check_every", c_int), ("graph_mode", c_int)]
 
 
 class LanczosReport(ctypes.Structure):

@@ -376,10 +376,13 @@ class LanczosResult:
 
 
 def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return_state=False, seed=1234,
-                         fixed_steps=False, start=None, check_every=0):
+                         fixed_steps=False, start=None, check_every=0, graph=None):
+    """graph: None automatic (replay a captured step graph below 2^22 rows
+    when no basis is stored), True always, False never."""
     L = _lib.load()
+    gm = 0 if graph is None else (1 if graph else 2)
     o = _lib.LanczosOptions(int(maxiter), float(tol), 1 if reorthogonalize else 0, 1 if fixed_steps else 0,
-                            1 if return_state else 0, int(seed), int(check_every))
+                            1 if return_state else 0, int(seed), int(check_every), gm)
     al = np.zeros(max(maxiter, 1))
     be = np.zeros(max(maxiter, 1))
     n = H.dimension if H.is_real else 2 * H.dimension

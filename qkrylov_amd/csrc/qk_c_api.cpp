@@ -788,6 +788,8 @@ int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_optio
     o.want_vector = (opt->want_vector != 0) && vector != nullptr;
     o.seed = opt->seed;
     o.check_every = opt->check_every;
+    if (opt->graph_mode < 0 || opt->graph_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    o.graph = opt->graph_mode == 0 ? -1 : (opt->graph_mode == 1 ? 1 : 0);
     o.start = start;
     auto r = qk::lanczos_run(*H->engine(), o, qk::default_stream());
     rep->energy = r.energy;

@@ -538,6 +538,55 @@ RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool
     }
 }
 
+RealCore::~RealCore()
+{
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+}
+
+void RealCore::steps_graph(int k, int n)
+{
+    if (n % 3 != 0 || k + n > kcap) throw std::logic_error("graph steps: n must be a multiple of 3 within the cap");
+    if (qkl_apply_partials(E.tables()) != npA) throw std::logic_error("apply launch shape changed during a run");
+    if (gexec && ic != cap_ic) {
+        // eager steps moved the rotation: capture again for the current roles
+        (void)hipGraphExecDestroy(gexec);
+        (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
+    if (!gexec) {
+        if (kdev.bytes() == 0) kdev = DevBuf(sizeof(int));
+        cap_ic = ic;
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        int rc = 0;
+        for (int i = 0; i < 3 && rc == 0; ++i) {
+            int* kd = static_cast<int*>(kdev.get());
+            rc |= qkl_apply_dev(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), kd, prv(), nrm.d(), partA.d(), s);
+            rc |= qkl_reduce_dev(partA.d(), npA, 0, alpha.d(), kd, 0, 0, s);
+            rc |= qkl_update_dev(D, wrk(), cur(), alpha.d(), nrm.d(), kd, partB.d(), s);
+            rc |= qkl_reduce_dev(partB.d(), NB, 1, nrm.d(), kd, 1, 1, s);
+            const int t = ip;
+            ip = ic;
+            ic = iw;
+            iw = t;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != 0 || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            throw std::runtime_error("capturing the Lanczos step graph failed");
+        }
+        graph = g;
+        hip_check(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "graph instantiate");
+    }
+    // the counter is set by a copy outside the graph; three rotations leave
+    // the vector roles as they were
+    k_host = k;
+    hip_check(hipMemcpyAsync(kdev.get(), &k_host, sizeof(int), hipMemcpyHostToDevice, s), "set step");
+    for (int r = 0; r < n / 3; ++r) hip_check(hipGraphLaunch(gexec, s), "graph launch");
+}
+
 void RealCore::init_norm()
 {
     lcheck(qkl_fill_zero(D, prv(), s), "zero");
@@ -601,13 +650,20 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
     if (store) lcheck(qkl_copy_scaled((size_t)D, C.cur(), basis.d(), C.nrm.d(), 0, s), "store v0");
 
     LanczosReplay rp{kmax, o.tol, o.fixed_steps, {}, {}};
-    int check = o.check_every > 0 ? o.check_every : (D < (1 << 22) ? 8 : 1);
+    // launch-bound sizes replay a captured three-step graph between host checks
+    const bool graph = !store && !reorth && kmax >= 3 && (o.graph > 0 || (o.graph < 0 && D < (1 << 22)));
+    int check = o.check_every > 0 ? o.check_every : (D < (1 << 22) ? (graph ? 12 : 8) : 1);
     if (o.fixed_steps) check = kmax;
     std::vector<double> ha(kmax + 2), hn(kmax + 3);
     const double t0 = now_s();
     int k = 0;
     while (!rp.done) {
         const int kend = std::min(kmax, k + check);
+        if (graph && kend - k >= 3) {
+            const int n3 = (kend - k) / 3 * 3;
+            C.steps_graph(k, n3);
+            k += n3;
+        }
         for (; k < kend; ++k) {
             C.step(k, reorth ? basis.d() : nullptr);
             if (store)

@@ -138,13 +138,27 @@ struct RealCore {
     DevBuf alpha, nrm, hcoef, partA, partB, partM;
     int npA, NB;
     int ip = 0, ic = 1, iw = 2;
+    // graph of three plain steps (one full turn of the vector rotation) whose
+    // kernels read the step from the device counter kdev
+    DevBuf kdev;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int k_host = 0;
+    int cap_ic = -1;   // rotation state the graph was captured in
 
     RealCore(const Engine& e, int kcap, hipStream_t st, bool reorth, bool stream_ordered = false);
+    ~RealCore();
+    RealCore(const RealCore&) = delete;
+    RealCore& operator=(const RealCore&) = delete;
     double* cur() { return Q[ic].d(); }
     double* prv() { return Q[ip].d(); }
     double* wrk() { return Q[iw].d(); }
     void init_norm();                        // nrm[0] = |cur|, prv = 0
     void step(int k, const double* basis);   // basis: normalised v_0..v_k for reorthogonalisation
+    // steps k .. k + n - 1 (n a multiple of 3, k + n <= kcap) as replays of
+    // the captured three-step graph; the same kernels in the same order as
+    // step(), so the same numbers
+    void steps_graph(int k, int n);
 };
 
 // ---------------------------------------------------------------- tridiagonal
@@ -164,6 +178,7 @@ struct LanczosOptions {
     bool fixed_steps = false;    // run min(maxiter, dim) steps, no convergence test
     uint64_t seed = 1234;
     int check_every = 0;         // 0: automatic
+    int graph = -1;              // replay captured step graphs: -1 below 2^22 rows, 0 never, 1 always
     const double* start = nullptr;   // host start vector (real, or interleaved complex for complex H)
     const double* start_dev = nullptr;   // device start vector (same layout)
 };

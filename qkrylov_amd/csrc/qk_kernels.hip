@@ -181,7 +181,8 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
                                                    const QkDiagTerm* __restrict__ dg, QkScal sc,
                                                    const double* __restrict__ x, double* __restrict__ y, int lz_k,
                                                    const double* __restrict__ prev, const double* __restrict__ nrm,
-                                                   double* __restrict__ partials, int table_bytes)
+                                                   double* __restrict__ partials, int table_bytes,
+                                                   const int* __restrict__ kdev)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double wsum[16];
@@ -208,10 +209,13 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
     uint32_t* a_base = reinterpret_cast<uint32_t*>(a_ci + nA);         // [nA] their block offsets
     const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
 
+    // Lanczos step: from the argument, or from the device step counter
+    // when the launch is replayed from a captured graph
+    const int lzk = (LZ && kdev) ? *kdev : lz_k;
     double inv_n = 1.0, cprev = 0.0;
     if (LZ) {
-        inv_n = 1.0 / nrm[lz_k];
-        if (lz_k > 0) cprev = nrm[lz_k] / nrm[lz_k - 1];
+        inv_n = 1.0 / nrm[lzk];
+        if (lzk > 0) cprev = nrm[lzk] / nrm[lzk - 1];
     }
     double part = 0.0;
 
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
         if (active) {
             if (LZ) {
                 double w = ar * inv_n;
-                if (lz_k > 0) w = fma(-cprev, __builtin_nontemporal_load(prev + row), w);
+                if (lzk > 0) w = fma(-cprev, __builtin_nontemporal_load(prev + row), w);
                 __builtin_nontemporal_store(w, y + row);
                 part = fma(xr * inv_n, w, part);
             } else if (CPLX) {
@@ -643,10 +647,11 @@ template <bool CPLX>
 __global__ __launch_bounds__(QK_TPB) void k_update1(size_t n, double* __restrict__ w, const double* __restrict__ q,
                                                      const double* __restrict__ alpha,
                                                      const double* __restrict__ nrm, int k,
-                                                     double* __restrict__ partials)
+                                                     double* __restrict__ partials, const int* __restrict__ kdev)
 {
     __shared__ double wsum[QK_WPB];
-    const double c = alpha[k] / nrm[k];
+    const int kk = kdev ? *kdev : k;
+    const double c = alpha[kk] / nrm[kk];
     double part = 0.0;
     const size_t nel = CPLX ? 2 * n : n;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nel; i += (size_t)gridDim.x * blockDim.x) {
@@ -697,8 +702,14 @@ __global__ __launch_bounds__(QK_TPB) void k_update_block(size_t nrows, int ncol,
 // One workgroup per result group. Fixed order: thread-strided sums, then a
 // fixed tree, then columns of the group added in index order.
 __global__ __launch_bounds__(QK_TPB) void k_reduce(const double* __restrict__ partials, int npart, int group,
-                                                    int mode, int cplx, double* __restrict__ out, int idx, int ld_sc)
+                                                    int mode, int cplx, double* __restrict__ out, int idx, int ld_sc,
+                                                    int* __restrict__ kdev, int advance)
 {
+    // with a device step counter the output index is counter + idx; the
+    // launch that advances the counter has one workgroup, and its thread 0
+    // writes the counter after every thread has read it
+    const int k0 = kdev ? *kdev : 0;
+    idx += k0;
     __shared__ double red[QK_TPB];
     __shared__ double red_i[QK_TPB];
     const int g = blockIdx.x;
@@ -738,6 +749,7 @@ __global__ __launch_bounds__(QK_TPB) void k_reduce(const double* __restrict__ pa
                 o[1] = tot_i;
             }
         }
+        if (advance && kdev) kdev[0] = k0 + 1;
     }
 }
 
@@ -1107,12 +1119,13 @@ inline int rc(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
 template <typename ST, bool CPLX, bool U16, bool LZ, bool SIGNS>
 int launch_apply(const QkTables* T, const double* x, double* y, int k, const double* prev, const double* nrm,
-                 double* partials, hipStream_t s)
+                 double* partials, hipStream_t s, const int* kdev = nullptr)
 {
     ApplyCfg c;
     if (!apply_config(T, c)) return -2;
     hipLaunchKernelGGL((k_apply<ST, CPLX, U16, LZ, SIGNS>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff, T->lrank,
-                       T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T));
+                       T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T),
+                       kdev);
     return rc(hipGetLastError());
 }
 
@@ -1178,6 +1191,43 @@ int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y,
 #undef QK_DISPATCH_S
 }
 
+// Real single-vector Lanczos step kernels reading the step k from a device
+// counter (for launches captured in a graph and replayed): the fused apply,
+// the update, and the reduction that writes out[*kdev + idx] (and, when
+// advance, moves the counter to the next step).
+int qkl_apply_dev(const QkTables* T, int wide, const double* x, double* y, const int* kdev, const double* prev,
+                  const double* nrm, double* partials, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (T->ntiles == 0 || !kdev) return -1;
+    const bool u16 = T->lrank_u16 != 0, sg = T->signs != 0;
+#define QK_DEV(ST)                                                                                              \
+    if (u16) return sg ? launch_apply<ST, false, true, true, true>(T, x, y, 0, prev, nrm, partials, s, kdev)   \
+                       : launch_apply<ST, false, true, true, false>(T, x, y, 0, prev, nrm, partials, s, kdev); \
+    else return sg ? launch_apply<ST, false, false, true, true>(T, x, y, 0, prev, nrm, partials, s, kdev)      \
+                   : launch_apply<ST, false, false, true, false>(T, x, y, 0, prev, nrm, partials, s, kdev);
+    if (wide) { QK_DEV(uint64_t) } else { QK_DEV(uint32_t) }
+#undef QK_DEV
+}
+
+int qkl_update_dev(size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, const int* kdev,
+                   double* partials, qkl_stream sv)
+{
+    if (!kdev) return -1;
+    hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, (hipStream_t)sv, nrows, w, q, alpha,
+                       nrm, 0, partials, kdev);
+    return rc(hipGetLastError());
+}
+
+int qkl_reduce_dev(const double* partials, int npart, int mode, double* out, int* kdev, int idx, int advance,
+                   qkl_stream sv)
+{
+    if (!kdev) return -1;
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(QK_TPB), 0, (hipStream_t)sv, partials, npart, 1, mode, 0, out, idx, 1,
+                       kdev, advance);
+    return rc(hipGetLastError());
+}
+
 int qkl_apply_block(const QkTables* T, int wide, const double* X, double* Y, int ld, int c0, int nc, int lz_k,
                     const double* P, const double* nrm, int ld_sc, double* partials, int npart, qkl_stream sv)
 {
@@ -1202,8 +1252,12 @@ int qkl_lanczos_update(int cplx, size_t nrows, int ncol, int ld, double* w, cons
 {
     hipStream_t s = (hipStream_t)sv;
     if (ncol == 1 && ld == 1) {
-        if (cplx) hipLaunchKernelGGL(k_update1<true>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, alpha, nrm, k, partials);
-        else hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, alpha, nrm, k, partials);
+        if (cplx)
+            hipLaunchKernelGGL(k_update1<true>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, alpha, nrm, k,
+                               partials, (const int*)nullptr);
+        else
+            hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, alpha, nrm, k,
+                               partials, (const int*)nullptr);
     } else {
         if (cplx || ncol > 64) return -1;
         hipLaunchKernelGGL(k_update_block, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, ncol, ld, w, q, alpha, nrm,
@@ -1218,7 +1272,7 @@ int qkl_reduce(const double* partials, int npart, int ncol, int group, int mode,
     hipStream_t s = (hipStream_t)sv;
     if (group <= 0 || ncol % group) return -1;
     hipLaunchKernelGGL(k_reduce, dim3(ncol / group), dim3(QK_TPB), 0, s, partials, npart, group, mode, cplx, out, idx,
-                       ld_sc);
+                       ld_sc, (int*)nullptr, 0);
     return rc(hipGetLastError());
 }
 

@@ -34,6 +34,18 @@ int qkl_apply_block(const qk::QkTables* T, int wide, const double* X, double* Y,
                     qkl_stream s);
 int qkl_block_partials(const qk::QkTables* T);
 
+/* The real single-vector Lanczos step with the step k read from a device
+ * counter *kdev (launches that are captured once in a graph and replayed):
+ * fused apply, update, and a one-workgroup reduction into out[*kdev + idx]
+ * that, when advance, moves the counter to the next step. */
+int qkl_apply_dev(const qk::QkTables* T, int wide, const double* x, double* y, const int* kdev, const double* prev,
+                  const double* nrm, double* partials, qkl_stream s);
+int qkl_update_dev(size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, const int* kdev,
+                   double* partials, qkl_stream s);
+int qkl_reduce_dev(const double* partials, int npart, int mode, double* out, int* kdev, int idx, int advance,
+                   qkl_stream s);
+
+
 /* q_{k+1} = w - alpha[k] * q_k / nrm[k]; partials of |q_{k+1}|^2 (ncol columns
  * interleaved with stride ld; ncol = 1, ld = 1 for a single vector; cplx for
  * interleaved complex single vector). */

@@ -380,3 +380,21 @@ def test_autotune_keeps_results():
     H.autotune(nvec=16, allow_split=False)
     H.apply_device(X.ptr, Y.ptr, 16)
     assert np.array_equal(Y.to_host(), Y0)
+
+
+@pytest.mark.parametrize("check_every", [0, 8])
+def test_lanczos_graph_replay_equals_eager(check_every):
+    """The captured three-step graph replays the eager launches exactly: the
+    same alpha/beta bit for bit, with host reads and eager steps between the
+    replays (check_every = 8 leaves the vector rotation off the graph's
+    capture state, which forces a second capture)."""
+    terms = models.heisenberg_chain(16)
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(16, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms))
+    for fixed, tol in ((True, 1e-12), (False, 1e-13)):
+        kw = dict(maxiter=100, tol=tol, reorthogonalize=False, fixed_steps=fixed, seed=5, check_every=check_every)
+        a = qa.lanczos_ground_state(H, graph=False, **kw)
+        b = qa.lanczos_ground_state(H, graph=True, **kw)
+        c = qa.lanczos_ground_state(H, graph=True, **kw)
+        for r in (b, c):
+            assert np.array_equal(a.alphas, r.alphas) and np.array_equal(a.betas, r.betas)
+            assert a.energy == r.energy and a.iterations == r.iterations
