@@ -300,7 +300,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part of this "
-                                   "build (see DESIGN.md for how the apply would shard)"}))
+                                   "build (the apply would shard by ranges of output rows cut at block boundaries, "
+                                   "with a halo of x exchanged directly between peers)"}))
         sys.exit(2)
 
     traffic, traffic_info = None, "not measured (--no-traffic)"

@@ -40,7 +40,8 @@ def flags(debug=False):
         "-fPIC",
         "-O1" if debug else "-O3",
         "-g" if debug else "-DNDEBUG",
-        # packed FP32 instructions stay off
+        # no packed FP32 instructions: some of their forms have given wrong
+        # results on these GPUs without any error
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-I" + os.path.join(ROOT, "include"),
         "-I" + CSRC,
