@@ -143,23 +143,25 @@ def bench_ftlm(args):
     H, terms = make_hamiltonian("tj4x5")
     D = H.dimension
     tune = None
+    Hs = H
     if args.autotune:
-        # the split for the single-vector runs, then the block launch shape
-        # at that split, so both modes run their best configuration
-        tune = [H.autotune(nvec=1, allow_split=True), H.autotune(nvec=R, allow_split=False)]
+        # each mode runs its own best configuration: one handle tuned for
+        # the R-column block, one for the single-vector runs
+        Hs, _ = make_hamiltonian("tj4x5")
+        tune = {"block": H.autotune(nvec=R, allow_split=True), "single": Hs.autotune(nvec=1, allow_split=True)}
         log(f"[bench-ftlm] autotune: {tune}")
     log(f"[bench-ftlm] D={D} R={R} M={M}")
     b = qa.ftlm_batched(H, R, M, betas, seed=42)
     log(f"[bench-ftlm] batched: {b['seconds']:.3f}s")
     seq = None
     if not args.no_sequential:
-        seq = qa.ftlm_batched(H, R, M, betas, seed=42, sequential=True)
+        seq = qa.ftlm_batched(Hs, R, M, betas, seed=42, sequential=True)
         log(f"[bench-ftlm] sequential: {seq['seconds']:.3f}s")
     rel = None
     if seq is not None:
         rel = max(float(np.max(np.abs(b[k] - seq[k]) / np.maximum(np.abs(seq[k]), 1e-300))) for k in ("Z", "E", "Cv"))
     # ground state and S^zz(q, omega) seeds
-    g = qa.lanczos_ground_state(H, maxiter=300, tol=1e-10, reorthogonalize=False, return_state=True)
+    g = qa.lanczos_ground_state(Hs, maxiter=300, tol=1e-10, reorthogonalize=False, return_state=True)
     psi = g.eigenvector
     basis = qa.TJBasis(20, nup=9, ndn=9)
     seeds = []

@@ -54,6 +54,8 @@ struct QkTables {
     const QkBranchK* brk;     // the same branches split at NL: nA hi-only, then nB others (padded)
     const QkDiagTerm* dg;     // diagonal terms
     int32_t nA, nB;           // padded counts (multiples of the kernel's unroll)
+    int32_t nB1, nB2;         // the first nB1 per-lane branches flip only low-half bits of lo and keep its
+                              // class, the next nB2 flip only high-half bits (two-level rank only)
     uint64_t lomask;
     uint64_t dim;             // number of basis rows
     double dconst_re, dconst_im;
@@ -67,6 +69,15 @@ struct QkTables {
     int32_t blk_waves;        // waves per workgroup of the multi-vector apply (16, 8 or 4; 0: 16)
     int32_t signs;            // 1: some branch carries a parity mask (fermionic signs)
     int32_t apply_waves;      // waves per workgroup of the single-vector apply (4, 8 or 16; 0: 4)
+    // lo rank in LDS: 0 one-level uint32 table, 1 one-level uint16 table,
+    // 2 two-level (tl[2^NL2] uint16, th[2^(NL-NL2) * nc] uint32):
+    // rank(lo) = th[(lo >> NL2) * nc + c] + tl[lo & (2^NL2 - 1)],
+    // c = popcount(l & upm2) * KB + popcount(l & dnm2)
+    int32_t rank_mode;
+    const uint16_t* tl;
+    const uint32_t* th;
+    int32_t NL2, nc, KB, ntl, nth;
+    uint32_t upm2, dnm2;
 };
 
 } // namespace qk

@@ -17,6 +17,11 @@
 
 namespace qk {
 
+namespace {
+inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+} // namespace
+
+
 void hip_check(hipError_t e, const char* what)
 {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -131,12 +136,18 @@ std::shared_ptr<const BasisTables> basis_tables(const BasisSpec& b, int NL)
     t->hoff = upload(t->host.hoff, nullptr);
     t->lrank = upload(t->host.lrank, nullptr);
     t->lolist = upload(t->host.lolist, nullptr);
+    if (t->host.split) {
+        t->tl = upload(t->host.tl, nullptr);
+        t->th = upload(t->host.th, nullptr);
+    }
     // the host copies of the large arrays are not needed after upload
     t->ntiles = t->host.tiles.size();
     std::vector<QkTile>().swap(t->host.tiles);
     std::vector<uint32_t>().swap(t->host.hoff);
     std::vector<uint32_t>().swap(t->host.lolist);
     std::vector<uint32_t>().swap(t->host.lrank);
+    std::vector<uint16_t>().swap(t->host.tl);
+    std::vector<uint32_t>().swap(t->host.th);
     cache[key] = t;
     return t;
 }
@@ -178,10 +189,37 @@ Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpS
         }
         const QkBranchK pad{0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0};
         while (A.size() % 4) A.push_back(pad);
-        while (B.size() % 8) B.push_back(pad);
+        // with the two-level lo rank the per-lane branches are grouped by how
+        // they move lo: flips inside the low half that keep its class (the
+        // high-half term of the rank is the row's own), flips inside the high
+        // half (the low-half term is the row's own), the rest; each group is
+        // padded to 8
+        std::vector<QkBranchK> B1, B2, B0;
+        const HostTables& ht = tables_->host;
+        for (const auto& k : B) {
+            int kind = 0;
+            if (ht.split) {
+                const uint32_t m2 = (1u << ht.NL2) - 1u;
+                if ((k.lF & ~m2) == 0 && (k.lcm & k.lF) == k.lF) {
+                    auto keeps = [&](uint32_t pre) {
+                        const uint32_t post = pre ^ k.lF;
+                        return popc((uint64_t)(pre & ht.upm2)) == popc((uint64_t)(post & ht.upm2)) &&
+                               popc((uint64_t)(pre & ht.dnm2)) == popc((uint64_t)(post & ht.dnm2));
+                    };
+                    if (keeps(k.lcv & k.lF) && keeps((k.lcv ^ k.lalt) & k.lF)) kind = 1;
+                } else if ((k.lF & m2) == 0) {
+                    kind = 2;
+                }
+            }
+            (kind == 1 ? B1 : kind == 2 ? B2 : B0).push_back(k);
+        }
+        for (auto* v : {&B1, &B2, &B0})
+            while (v->size() % 8) v->push_back(pad);
         T_.nA = (int32_t)A.size();
-        T_.nB = (int32_t)B.size();
-        A.insert(A.end(), B.begin(), B.end());
+        T_.nB1 = (int32_t)B1.size();
+        T_.nB2 = (int32_t)B2.size();
+        T_.nB = (int32_t)(B1.size() + B2.size() + B0.size());
+        for (auto* v : {&B1, &B2, &B0}) A.insert(A.end(), v->begin(), v->end());
         brk_ = upload(A, opt.stream);
     }
     T_.tiles = static_cast<const QkTile*>(tables_->tiles.get());
@@ -201,6 +239,19 @@ Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpS
     T_.NL = tables_->host.NL;
     T_.lrank_u16 = tables_->host.lrank_u16 ? 1 : 0;
     T_.nlo = 1 << tables_->host.NL;
+    {
+        const HostTables& h = tables_->host;
+        T_.rank_mode = h.split ? 2 : (h.lrank_u16 ? 1 : 0);
+        T_.tl = h.split ? static_cast<const uint16_t*>(tables_->tl.get()) : nullptr;
+        T_.th = h.split ? static_cast<const uint32_t*>(tables_->th.get()) : nullptr;
+        T_.NL2 = h.NL2;
+        T_.nc = h.nc;
+        T_.KB = h.KB;
+        T_.ntl = h.split ? (1 << h.NL2) : 0;
+        T_.nth = h.split ? (int32_t)((1 << (h.NL - h.NL2)) * h.nc) : 0;
+        T_.upm2 = h.upm2;
+        T_.dnm2 = h.dnm2;
+    }
     T_.blk_waves = 8;
     T_.apply_waves = 16;
     // one XCD segment each for the single-vector apply (its x mostly fits the
@@ -275,12 +326,17 @@ std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& b
     bool improved = false;
     std::vector<int> splits{current->NL()};
     if (allow_split) {
+        // around the current split, and the wide splits ranked by two-level
+        // tables (more branches inside the low bits, fewer far gathers)
         const int nb = basis->nbits(), m = current->NL();
         const int stepNL = basis->spinless() ? 1 : 2;
-        for (int d = -2; d <= 2; ++d) {
-            const int NL = m + d * stepNL;
-            if (d == 0 || NL < 4 || NL > std::min(nb, 16) || nb - NL > 28) continue;
-            splits.push_back(NL);
+        std::vector<int> cand;
+        for (int d = -2; d <= 2; ++d) cand.push_back(m + d * stepNL);
+        for (int w : {16, 18, 20}) cand.push_back(w);
+        for (int NL : cand) {
+            if (NL < 4 || NL > std::min(nb, 20) || nb - NL > 28) continue;
+            if (!basis->spinless() && (NL & 1)) continue;
+            if (std::find(splits.begin(), splits.end(), NL) == splits.end()) splits.push_back(NL);
         }
     }
     const int knob_w = nvec == 1 ? 2 : 3, knob_s = nvec == 1 ? 0 : 1;

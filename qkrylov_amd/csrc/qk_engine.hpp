@@ -52,7 +52,7 @@ void ensure_device(int id);
 // Hamiltonian over the same basis shares one copy.
 struct BasisTables {
     HostTables host;   // NL, NH, lrank_u16 (the large host arrays are released)
-    DevBuf tiles, hoff, lrank, lolist;
+    DevBuf tiles, hoff, lrank, lolist, tl, th;
     size_t ntiles = 0;
 };
 std::shared_ptr<const BasisTables> basis_tables(const BasisSpec& b, int NL);

@@ -12,6 +12,8 @@
 // fixed-order: per-workgroup partials, then one workgroup sums them.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <math.h>
 
 #include "qk_launch.h"
@@ -28,6 +30,8 @@ struct QkScal {
     uint32_t lomask;
     uint32_t ntiles;
     int32_t nbr, ndg, NL, nlo, nA, nB, sched;
+    int32_t NL2, nc, KB, ntl, nB1, nB2;
+    uint32_t upm2, dnm2;
     double dcr, dci;
 };
 
@@ -43,6 +47,14 @@ QkScal make_scal(const QkTables* T)
     s.NL = T->NL;
     s.nlo = T->nlo;
     s.sched = T->sched;
+    s.NL2 = T->NL2;
+    s.nc = T->nc;
+    s.KB = T->KB;
+    s.ntl = T->ntl;
+    s.upm2 = T->upm2;
+    s.dnm2 = T->dnm2;
+    s.nB1 = T->nB1;
+    s.nB2 = T->nB2;
     s.dcr = T->dconst_re;
     s.dci = T->dconst_im;
     return s;
@@ -103,21 +115,72 @@ __device__ __forceinline__ void eval_diag(const QkDiagTerm* __restrict__ dg, int
     }
 }
 
-template <bool U16>
-__device__ __forceinline__ uint32_t lds_rank(const unsigned char* smem, uint32_t lo)
+// Lo rank from the LDS tables. RM 0 / 1: one-level uint32 / uint16 table of
+// 2^NL entries. RM 2: two-level, lo = (h << NL2) | l,
+// rank = th[h * nc + c(l)] + tl[l] with the class c(l) of the low half.
+__host__ __device__ __forceinline__ size_t tl_bytes(int ntl) { return ((size_t)ntl * 2 + 15) & ~(size_t)15; }
+
+template <int RM>
+__device__ __forceinline__ uint32_t lds_rank(const unsigned char* smem, uint32_t lo, const QkScal& sc)
 {
-    if (U16) return reinterpret_cast<const uint16_t*>(smem)[lo];
-    return reinterpret_cast<const uint32_t*>(smem)[lo];
+    if (RM == 1) return reinterpret_cast<const uint16_t*>(smem)[lo];
+    if (RM == 0) return reinterpret_cast<const uint32_t*>(smem)[lo];
+    const uint32_t l = lo & ((1u << sc.NL2) - 1u), h = lo >> sc.NL2;
+    const uint32_t c = (uint32_t)__builtin_popcount(l & sc.upm2) * (uint32_t)sc.KB + (uint32_t)__builtin_popcount(l & sc.dnm2);
+    const uint16_t* tl = reinterpret_cast<const uint16_t*>(smem);
+    const uint32_t* th = reinterpret_cast<const uint32_t*>(smem + tl_bytes(sc.ntl));
+    return (uint32_t)tl[l] + th[h * (uint32_t)sc.nc + c];
 }
 
-template <bool U16>
-__device__ __forceinline__ void load_lrank(unsigned char* smem, const uint32_t* __restrict__ lrank, int nlo)
+template <int RM>
+__device__ __forceinline__ void load_lrank(unsigned char* smem, const uint32_t* __restrict__ lrank,
+                                           const uint16_t* __restrict__ tl, const uint32_t* __restrict__ th,
+                                           int nlo, int ntl, int nth)
 {
-    for (int i = threadIdx.x; i < nlo; i += blockDim.x) {
-        if (U16) reinterpret_cast<uint16_t*>(smem)[i] = static_cast<uint16_t>(lrank[i]);
-        else reinterpret_cast<uint32_t*>(smem)[i] = lrank[i];
+    if (RM == 2) {
+        for (int i = threadIdx.x; i < ntl; i += blockDim.x) reinterpret_cast<uint16_t*>(smem)[i] = tl[i];
+        uint32_t* d = reinterpret_cast<uint32_t*>(smem + tl_bytes(ntl));
+        for (int i = threadIdx.x; i < nth; i += blockDim.x) d[i] = th[i];
+    } else {
+        for (int i = threadIdx.x; i < nlo; i += blockDim.x) {
+            if (RM == 1) reinterpret_cast<uint16_t*>(smem)[i] = static_cast<uint16_t>(lrank[i]);
+            else reinterpret_cast<uint32_t*>(smem)[i] = lrank[i];
+        }
     }
     __syncthreads();
+}
+
+// Rank of a target lo = own lo ^ lF. KIND 1: lF inside the low half and the
+// low half keeps its class, so the high-half term is the row's own; KIND 2:
+// lF inside the high half, so the low-half term is the row's own; KIND 0:
+// the full lookup. With one-level tables every kind is one lookup.
+struct OwnRank {
+    uint32_t h, c, th_own, tl_own;
+};
+
+template <int RM>
+__device__ __forceinline__ OwnRank own_rank(const unsigned char* smem, uint32_t lo, const QkScal& sc)
+{
+    OwnRank o{0, 0, 0, 0};
+    if (RM == 2) {
+        const uint32_t l = lo & ((1u << sc.NL2) - 1u);
+        o.h = lo >> sc.NL2;
+        o.c = (uint32_t)__builtin_popcount(l & sc.upm2) * (uint32_t)sc.KB + (uint32_t)__builtin_popcount(l & sc.dnm2);
+        const uint32_t* th = reinterpret_cast<const uint32_t*>(smem + tl_bytes(sc.ntl));
+        o.th_own = th[o.h * (uint32_t)sc.nc + o.c];
+        o.tl_own = reinterpret_cast<const uint16_t*>(smem)[l];
+    }
+    return o;
+}
+
+template <int RM, int KIND>
+__device__ __forceinline__ uint32_t target_rank(const unsigned char* smem, uint32_t lo, uint32_t lF, const OwnRank& o,
+                                                const QkScal& sc)
+{
+    if (RM != 2 || KIND == 0) return lds_rank<RM>(smem, lo ^ lF, sc);
+    if (KIND == 1) return o.th_own + reinterpret_cast<const uint16_t*>(smem)[(lo ^ lF) & ((1u << sc.NL2) - 1u)];
+    const uint32_t* th = reinterpret_cast<const uint32_t*>(smem + tl_bytes(sc.ntl));
+    return th[(o.h ^ (lF >> sc.NL2)) * (uint32_t)sc.nc + o.c] + o.tl_own;
 }
 
 // acc += sum_u c_u * x[rt_u], all U loads issued before the first FMA;
@@ -173,9 +236,46 @@ __host__ __device__ __forceinline__ size_t apply_branch_bytes(int nB)
     return (((size_t)nB * (16 + 4 + 8 + 8)) + 15) & ~(size_t)15;
 }
 
-template <typename ST, bool CPLX, bool U16, bool LZ, bool SIGNS>
+// Per-lane branches [gbeg, gend) of one tile, eight gathers in flight.
+// Branch-free: a branch that does not apply gathers the row's own x with
+// coefficient 0 (masked, not selected by control flow, so the rank and
+// value loads of the eight branches overlap).
+template <bool CPLX, int RM, bool SIGNS, int KIND>
+__device__ __forceinline__ void lane_branches(int gbeg, int gend, const unsigned char* smem, const uint4* lb4,
+                                              const uint32_t* lbm, const double* lbre, const double* lbim,
+                                              const uint4* tb, uint32_t lo, uint32_t row, const OwnRank& orank,
+                                              const QkScal& sc, const double* __restrict__ x, double& ar, double& ai)
+{
+    for (int g = gbeg; g < gend; g += 8) {
+        uint32_t rt[8];
+        double cr[8], ci[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint4 L = lb4[g + u];            // lcm, lcv, lalt, lF
+            const uint4 tv = tb[g + u];            // block offset, va, vb, hpar
+            const uint32_t lf = lo & L.x;
+            const uint32_t m = 0u - (uint32_t)((lf == tv.y) | (lf == tv.z));
+            const uint32_t lr = target_rank<RM, KIND>(smem, lo, L.w, orank, sc);
+            uint64_t br = __builtin_bit_cast(uint64_t, lbre[g + u]);
+            uint64_t bi = CPLX ? __builtin_bit_cast(uint64_t, lbim[g + u]) : 0ull;
+            if (SIGNS) {
+                const uint64_t neg = (uint64_t)(((uint32_t)__builtin_popcount(lo & lbm[g + u]) ^ tv.w) & 1u) << 63;
+                br ^= neg;
+                bi ^= neg;
+            }
+            const uint64_t m64 = ((uint64_t)m << 32) | m;
+            rt[u] = row ^ ((row ^ (tv.x + lr)) & m);
+            cr[u] = __builtin_bit_cast(double, br & m64);
+            ci[u] = __builtin_bit_cast(double, bi & m64);
+        }
+        accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
+    }
+}
+
+template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
 __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
-                                                   const uint32_t* __restrict__ lrank,
+                                                   const uint32_t* __restrict__ lrank, const uint16_t* __restrict__ tl,
+                                                   const uint32_t* __restrict__ th, int nth,
                                                    const uint32_t* __restrict__ lolist,
                                                    const QkBranchK* __restrict__ brk,
                                                    const QkDiagTerm* __restrict__ dg, QkScal sc,
@@ -198,7 +298,7 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
         lbre[i] = B.re;
         lbim[i] = B.im;
     }
-    load_lrank<U16>(smem, lrank, sc.nlo);
+    load_lrank<RM>(smem, lrank, tl, th, sc.nlo, sc.ntl, nth);
 
     const uint32_t lane = threadIdx.x & 63u;
     const int NL = sc.NL;
@@ -295,29 +395,17 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
         }
 
         // (4) remaining branches: per-lane test of lo & lcm against the two
-        // tile-uniform values, lo rank from LDS (lF == 0 reads lrank[lo] =
-        // the row's own rank), eight in flight
-        for (int g = 0; g < nB; g += 8) {
-            uint32_t rt[8];
-            double cr[8], ci[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const uint4 L = lb4[g + u];            // lcm, lcv, lalt, lF
-                const uint4 tv = tb[nA + g + u];       // block offset, va, vb, hpar
-                const uint32_t lf = lo & L.x;
-                const bool ok = (lf == tv.y) || (lf == tv.z);
-                const uint32_t lr = lds_rank<U16>(smem, lo ^ L.w);
-                double vr = lbre[g + u], vi = CPLX ? lbim[g + u] : 0.0;
-                if (SIGNS) {
-                    const uint32_t neg = ((uint32_t)__builtin_popcount(lo & lbm[g + u]) ^ tv.w) & 1u;
-                    vr = neg ? -vr : vr;
-                    vi = neg ? -vi : vi;
-                }
-                rt[u] = ok ? tv.x + lr : row;
-                cr[u] = ok ? vr : 0.0;
-                ci[u] = ok ? vi : 0.0;
-            }
-            accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
+        // tile-uniform values, lo rank from LDS, eight in flight; with the
+        // two-level rank, three runs by rank kind
+        const OwnRank orank = own_rank<RM>(smem, lo, sc);
+        if (RM == 2) {
+            lane_branches<CPLX, RM, SIGNS, 1>(0, sc.nB1, smem, lb4, lbm, lbre, lbim, tb + nA, lo, row, orank, sc, x, ar, ai);
+            lane_branches<CPLX, RM, SIGNS, 2>(sc.nB1, sc.nB1 + sc.nB2, smem, lb4, lbm, lbre, lbim, tb + nA, lo, row, orank,
+                                              sc, x, ar, ai);
+            lane_branches<CPLX, RM, SIGNS, 0>(sc.nB1 + sc.nB2, nB, smem, lb4, lbm, lbre, lbim, tb + nA, lo, row, orank, sc,
+                                              x, ar, ai);
+        } else {
+            lane_branches<CPLX, RM, SIGNS, 0>(0, nB, smem, lb4, lbm, lbre, lbim, tb + nA, lo, row, orank, sc, x, ar, ai);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
@@ -419,9 +507,10 @@ __device__ __forceinline__ void block_gather(const uint32_t* l_tgt, const uint16
     }
 }
 
-template <typename ST, bool U16, bool LZ, int NW>
+template <typename ST, int RM, bool LZ, int NW>
 __global__ __launch_bounds__(64 * NW) void k_apply_block(
     const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff, const uint32_t* __restrict__ lrank,
+    const uint16_t* __restrict__ tl, const uint32_t* __restrict__ th, int nth,
     const uint32_t* __restrict__ lolist, const QkBranchK* __restrict__ brk, const QkDiagTerm* __restrict__ dg,
     QkScal sc, const double* __restrict__ X, double* __restrict__ Y, int ld, int c0, int nc, int lz_k,
     const double* __restrict__ P, const double* __restrict__ nrm, int ld_sc, double* __restrict__ partials,
@@ -442,7 +531,7 @@ __global__ __launch_bounds__(64 * NW) void k_apply_block(
         l_vals[2 * i] = v;
         l_vals[2 * i + 1] = -v;
     }
-    load_lrank<U16>(smem, lrank, sc.nlo);
+    load_lrank<RM>(smem, lrank, tl, th, sc.nlo, sc.ntl, nth);
 
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wv = threadIdx.x >> 6;
@@ -559,21 +648,33 @@ __global__ __launch_bounds__(64 * NW) void k_apply_block(
                     l_vid[k] = (uint16_t)a_vid[i];
                     ++k;
                 }
-                for (int g = 0; g < nB; g += 8) {
+                const OwnRank orank = own_rank<RM>(smem, lo, sc);
+                auto write_run = [&](auto kind_tag, int gbeg, int gend) {
+                    constexpr int KIND = decltype(kind_tag)::value;
+                    for (int g = gbeg; g < gend; g += 8) {
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const uint4 L = lb4[g + u];
-                        const uint2 tv = tb[nA + g + u];
-                        const uint32_t lf = lo & L.x;
-                        const bool ok = ((tv.y & 1u) && lf == L.y) || (((tv.y >> 1) & 1u) && lf == (L.y ^ L.z));
-                        if (ok) {
-                            const uint32_t lr = lds_rank<U16>(smem, lo ^ L.w);
-                            const uint32_t neg = ((uint32_t)__builtin_popcount(lo & lbm[g + u]) ^ (tv.y >> 2)) & 1u;
-                            l_tgt[k] = tv.x + lr;
-                            l_vid[k] = (uint16_t)(2u * (uint32_t)(nA + g + u) + neg);
-                            ++k;
+                        for (int u = 0; u < 8; ++u) {
+                            const uint4 L = lb4[g + u];
+                            const uint2 tv = tb[nA + g + u];
+                            const uint32_t lf = lo & L.x;
+                            const bool ok = ((tv.y & 1u) && lf == L.y) || (((tv.y >> 1) & 1u) && lf == (L.y ^ L.z));
+                            if (ok) {
+                                const uint32_t lr = target_rank<RM, KIND>(smem, lo, L.w, orank, sc);
+                                const uint32_t neg =
+                                    ((uint32_t)__builtin_popcount(lo & lbm[g + u]) ^ (tv.y >> 2)) & 1u;
+                                l_tgt[k] = tv.x + lr;
+                                l_vid[k] = (uint16_t)(2u * (uint32_t)(nA + g + u) + neg);
+                                ++k;
+                            }
                         }
                     }
+                };
+                if (RM == 2) {
+                    write_run(std::integral_constant<int, 1>{}, 0, sc.nB1);
+                    write_run(std::integral_constant<int, 2>{}, sc.nB1, sc.nB1 + sc.nB2);
+                    write_run(std::integral_constant<int, 0>{}, sc.nB1 + sc.nB2, nB);
+                } else {
+                    write_run(std::integral_constant<int, 0>{}, 0, nB);
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1038,7 +1139,11 @@ __global__ __launch_bounds__(QK_TPB) void k_precond(size_t n, const double* __re
 
 // ------------------------------------------------------------------ launch helpers
 // LDS bytes of the lo-rank table, rounded to 16 so what follows it is aligned
-inline size_t table_bytes(const QkTables* T) { return ((size_t)T->nlo * (T->lrank_u16 ? 2 : 4) + 15) & ~(size_t)15; }
+inline size_t table_bytes(const QkTables* T)
+{
+    if (T->rank_mode == 2) return tl_bytes(T->ntl) + (((size_t)T->nth * 4 + 15) & ~(size_t)15);
+    return ((size_t)T->nlo * (T->rank_mode == 1 ? 2 : 4) + 15) & ~(size_t)15;
+}
 
 // Single-vector apply launch shape: the preferred number of waves per
 // workgroup (4, 8 or 16; they share one LDS copy of the tables), or fewer
@@ -1117,38 +1222,72 @@ inline int grid_for(size_t n)
 
 inline int rc(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
-template <typename ST, bool CPLX, bool U16, bool LZ, bool SIGNS>
+template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
 int launch_apply(const QkTables* T, const double* x, double* y, int k, const double* prev, const double* nrm,
                  double* partials, hipStream_t s, const int* kdev = nullptr)
 {
     ApplyCfg c;
     if (!apply_config(T, c)) return -2;
-    hipLaunchKernelGGL((k_apply<ST, CPLX, U16, LZ, SIGNS>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff, T->lrank,
+    hipLaunchKernelGGL((k_apply<ST, CPLX, RM, LZ, SIGNS>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff,
+                       T->lrank, T->tl, T->th, T->nth,
                        T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T),
                        kdev);
     return rc(hipGetLastError());
 }
 
-template <typename ST, bool U16, bool LZ, int NW>
+template <typename ST, int RM, bool LZ, int NW>
 int launch_block_nw(const QkTables* T, const BlockCfg& c, const double* X, double* Y, int ld, int c0, int nc, int k,
                  const double* P, const double* nrm, int ld_sc, double* partials, int npart, hipStream_t s)
 {
     if (LZ && npart != c.nb) return -1;
     QkScal sc = make_scal(T);
     sc.sched = T->sched_block;
-    hipLaunchKernelGGL((k_apply_block<ST, U16, LZ, NW>), dim3(c.nb), dim3(64 * NW), c.lds, s, T->tiles, T->hoff,
-                       T->lrank, T->lolist, T->brk, T->dg, sc, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
+    hipLaunchKernelGGL((k_apply_block<ST, RM, LZ, NW>), dim3(c.nb), dim3(64 * NW), c.lds, s, T->tiles, T->hoff,
+                       T->lrank, T->tl, T->th, T->nth, T->lolist, T->brk, T->dg, sc, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
                        npart, c.cap, (int)table_bytes(T));
     return rc(hipGetLastError());
 }
 
-template <typename ST, bool U16, bool LZ>
+template <typename ST, int RM, bool LZ>
 int launch_block(const QkTables* T, const BlockCfg& c, const double* X, double* Y, int ld, int c0, int nc, int k,
                  const double* P, const double* nrm, int ld_sc, double* partials, int npart, hipStream_t s)
 {
-    if (c.nw == 4) return launch_block_nw<ST, U16, LZ, 4>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
-    if (c.nw == 8) return launch_block_nw<ST, U16, LZ, 8>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
-    return launch_block_nw<ST, U16, LZ, 16>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+    if (c.nw == 4) return launch_block_nw<ST, RM, LZ, 4>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+    if (c.nw == 8) return launch_block_nw<ST, RM, LZ, 8>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+    return launch_block_nw<ST, RM, LZ, 16>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+}
+
+template <typename ST, bool C, bool LZ, bool SG>
+int apply_rm(const QkTables* T, const double* x, double* y, int k, const double* prev, const double* nrm,
+             double* partials, hipStream_t s, const int* kdev)
+{
+    switch (T->rank_mode) {
+    case 2: return launch_apply<ST, C, 2, LZ, SG>(T, x, y, k, prev, nrm, partials, s, kdev);
+    case 1: return launch_apply<ST, C, 1, LZ, SG>(T, x, y, k, prev, nrm, partials, s, kdev);
+    default: return launch_apply<ST, C, 0, LZ, SG>(T, x, y, k, prev, nrm, partials, s, kdev);
+    }
+}
+
+template <typename ST, bool C>
+int apply_any(const QkTables* T, bool lz, const double* x, double* y, int k, const double* prev, const double* nrm,
+              double* partials, hipStream_t s, const int* kdev)
+{
+    const bool sg = T->signs != 0;
+    if (lz) return sg ? apply_rm<ST, C, true, true>(T, x, y, k, prev, nrm, partials, s, kdev)
+                      : apply_rm<ST, C, true, false>(T, x, y, k, prev, nrm, partials, s, kdev);
+    return sg ? apply_rm<ST, C, false, true>(T, x, y, k, prev, nrm, partials, s, kdev)
+              : apply_rm<ST, C, false, false>(T, x, y, k, prev, nrm, partials, s, kdev);
+}
+
+template <typename ST, bool LZ>
+int block_rm(const QkTables* T, const BlockCfg& c, const double* X, double* Y, int ld, int c0, int nc, int k,
+             const double* P, const double* nrm, int ld_sc, double* partials, int npart, hipStream_t s)
+{
+    switch (T->rank_mode) {
+    case 2: return launch_block<ST, 2, LZ>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+    case 1: return launch_block<ST, 1, LZ>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+    default: return launch_block<ST, 0, LZ>(T, c, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials, npart, s);
+    }
 }
 
 } // namespace
@@ -1172,23 +1311,14 @@ int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y,
 {
     hipStream_t s = (hipStream_t)sv;
     if (T->ntiles == 0) return 0;
-    const bool u16 = T->lrank_u16 != 0;
     const bool lz = lz_k >= 0;
     if (lz && cplx) return -1;
-#define QK_DISPATCH_S(ST, C, SG)                                                                              \
-    if (u16) return lz ? launch_apply<ST, C, true, true, SG>(T, x, y, lz_k, prev, nrm, partials, s)         \
-                       : launch_apply<ST, C, true, false, SG>(T, x, y, lz_k, prev, nrm, partials, s);       \
-    else return lz ? launch_apply<ST, C, false, true, SG>(T, x, y, lz_k, prev, nrm, partials, s)            \
-                   : launch_apply<ST, C, false, false, SG>(T, x, y, lz_k, prev, nrm, partials, s);
-#define QK_DISPATCH(ST, C)                                                                                   \
-    if (T->signs) { QK_DISPATCH_S(ST, C, true) } else { QK_DISPATCH_S(ST, C, false) }
     if (wide) {
-        if (cplx) { QK_DISPATCH(uint64_t, true) } else { QK_DISPATCH(uint64_t, false) }
-    } else {
-        if (cplx) { QK_DISPATCH(uint32_t, true) } else { QK_DISPATCH(uint32_t, false) }
+        return cplx ? apply_any<uint64_t, true>(T, lz, x, y, lz_k, prev, nrm, partials, s, nullptr)
+                    : apply_any<uint64_t, false>(T, lz, x, y, lz_k, prev, nrm, partials, s, nullptr);
     }
-#undef QK_DISPATCH
-#undef QK_DISPATCH_S
+    return cplx ? apply_any<uint32_t, true>(T, lz, x, y, lz_k, prev, nrm, partials, s, nullptr)
+                : apply_any<uint32_t, false>(T, lz, x, y, lz_k, prev, nrm, partials, s, nullptr);
 }
 
 // Real single-vector Lanczos step kernels reading the step k from a device
@@ -1200,14 +1330,8 @@ int qkl_apply_dev(const QkTables* T, int wide, const double* x, double* y, const
 {
     hipStream_t s = (hipStream_t)sv;
     if (T->ntiles == 0 || !kdev) return -1;
-    const bool u16 = T->lrank_u16 != 0, sg = T->signs != 0;
-#define QK_DEV(ST)                                                                                              \
-    if (u16) return sg ? launch_apply<ST, false, true, true, true>(T, x, y, 0, prev, nrm, partials, s, kdev)   \
-                       : launch_apply<ST, false, true, true, false>(T, x, y, 0, prev, nrm, partials, s, kdev); \
-    else return sg ? launch_apply<ST, false, false, true, true>(T, x, y, 0, prev, nrm, partials, s, kdev)      \
-                   : launch_apply<ST, false, false, true, false>(T, x, y, 0, prev, nrm, partials, s, kdev);
-    if (wide) { QK_DEV(uint64_t) } else { QK_DEV(uint32_t) }
-#undef QK_DEV
+    return wide ? apply_any<uint64_t, false>(T, true, x, y, 0, prev, nrm, partials, s, kdev)
+                : apply_any<uint32_t, false>(T, true, x, y, 0, prev, nrm, partials, s, kdev);
 }
 
 int qkl_update_dev(size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, const int* kdev,
@@ -1237,14 +1361,11 @@ int qkl_apply_block(const QkTables* T, int wide, const double* X, double* Y, int
     if (2 * (T->nA + T->nB) + 64 >= 0xFFFF) return -3;   // value ids are 16-bit
     BlockCfg c;
     if (!block_config(T, c)) return -2;
-    const bool u16 = T->lrank_u16 != 0, lz = lz_k >= 0;
-#define QK_BDISPATCH(ST)                                                                                         \
-    if (u16) return lz ? launch_block<ST, true, true>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s) \
-                       : launch_block<ST, true, false>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s); \
-    else return lz ? launch_block<ST, false, true>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s)   \
-                   : launch_block<ST, false, false>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s);
-    if (wide) { QK_BDISPATCH(uint64_t) } else { QK_BDISPATCH(uint32_t) }
-#undef QK_BDISPATCH
+    const bool lz = lz_k >= 0;
+    if (wide) return lz ? block_rm<uint64_t, true>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s)
+                        : block_rm<uint64_t, false>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s);
+    return lz ? block_rm<uint32_t, true>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s)
+              : block_rm<uint32_t, false>(T, c, X, Y, ld, c0, nc, lz_k, P, nrm, ld_sc, partials, npart, s);
 }
 
 int qkl_lanczos_update(int cplx, size_t nrows, int ncol, int ld, double* w, const double* q, const double* alpha,

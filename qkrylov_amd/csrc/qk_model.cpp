@@ -496,6 +496,85 @@ int choose_NL(const BasisSpec& basis)
     return nl;
 }
 
+// Two-level rank of the lo field (see HostTables): the low half of NL2 bits
+// (even for two-species bases, so no site straddles the halves) is ranked
+// inside its own class, the high half contributes the count of valid lo of
+// the same total class with a smaller high half. Checked against the
+// one-level table for every valid lo.
+static void build_split_rank(const BasisSpec& basis, HostTables& T)
+{
+    const int NL = T.NL;
+    const bool two = !basis.spinless();
+    const bool tj = basis.no_double();
+    const bool ca = basis.target_a() != -1, cb = basis.target_b() != -1;
+    // the low-half width with the smallest pair of tables
+    auto n_classes = [&](int n2) {
+        return (ca ? (two ? n2 / 2 : n2) + 1 : 1) * (cb ? n2 / 2 + 1 : 1);
+    };
+    int NL2 = 0;
+    size_t best = ~(size_t)0;
+    for (int n2 = (NL + 1) / 2; n2 < NL && n2 <= 16; ++n2) {
+        if (two && (n2 & 1)) continue;
+        const size_t bytes = ((size_t)1 << n2) * 2 + ((size_t)1 << (NL - n2)) * (size_t)n_classes(n2) * 4;
+        if (bytes < best) {
+            best = bytes;
+            NL2 = n2;
+        }
+    }
+    const int NLH = NL - NL2;
+    const uint32_t m2 = (1u << NL2) - 1u;
+    uint32_t up = 0, dn = 0;
+    for (int p = 0; p < NL2; ++p) (basis.is_up_bit(p) ? up : dn) |= 1u << p;
+    const int maxa = two ? NL2 / 2 : NL2, maxb = two ? NL2 / 2 : 0;
+    const int KA = ca ? maxa + 1 : 1, KB = cb ? maxb + 1 : 1;
+    const uint32_t upm2 = ca ? up : 0u, dnm2 = cb ? dn : 0u;
+    auto valid = [&](uint32_t v) { return !tj || (v & (v >> 1) & 0x55555555u) == 0; };
+    auto cls = [&](uint32_t l) { return popc((uint64_t)(l & upm2)) * KB + popc((uint64_t)(l & dnm2)); };
+    const int nc = KA * KB;
+    // low halves: rank inside their class, and class sizes
+    std::vector<uint32_t> nlow(nc, 0);
+    T.tl.assign(1u << NL2, 0);
+    for (uint32_t l = 0; l <= m2; ++l) {
+        if (!valid(l)) continue;
+        T.tl[l] = static_cast<uint16_t>(nlow[cls(l)]++);
+    }
+    // high halves, ascending: for every total class (a, b) the number of
+    // valid lo seen so far (with a smaller high half)
+    const int TA = ca ? 2 * maxa + 2 + (two ? NLH / 2 : NLH) : 1;
+    const int TB = cb ? 2 * maxb + 2 + NLH / 2 : 1;
+    std::vector<uint64_t> run((size_t)TA * TB, 0);
+    const uint32_t nh = 1u << NLH;
+    T.th.assign((size_t)nh * nc, 0);
+    for (uint32_t h = 0; h < nh; ++h) {
+        const uint32_t hs = h << NL2;
+        if (!valid(hs)) continue;
+        const int ah = ca ? popc((uint64_t)(hs & (basis.up_mask() & ((1ULL << NL) - 1)))) : 0;
+        const int bh = cb ? popc((uint64_t)(hs & (basis.dn_mask() & ((1ULL << NL) - 1)))) : 0;
+        for (int c = 0; c < nc; ++c) {
+            const int a = ah + c / KB, b = bh + c % KB;
+            const uint64_t v = run[(size_t)a * TB + b];
+            if (v > 0xFFFFFFFFull) throw std::logic_error("split rank overflow");
+            T.th[(size_t)h * nc + c] = static_cast<uint32_t>(v);
+        }
+        // lo with this high half add to every total class they reach
+        for (int c = 0; c < nc; ++c) {
+            const int a = ah + c / KB, b = bh + c % KB;
+            run[(size_t)a * TB + b] += nlow[c];
+        }
+    }
+    for (uint32_t lo = 0; lo < (1u << NL); ++lo) {
+        if (T.lrank[lo] == 0xFFFFFFFFu) continue;
+        const uint32_t l = lo & m2, h = lo >> NL2;
+        if (T.th[(size_t)h * nc + cls(l)] + T.tl[l] != T.lrank[lo]) throw std::logic_error("split rank mismatch");
+    }
+    T.split = true;
+    T.NL2 = NL2;
+    T.nc = nc;
+    T.KB = KB;
+    T.upm2 = upm2;
+    T.dnm2 = dnm2;
+}
+
 HostTables build_tables(const BasisSpec& basis, int NL)
 {
     HostTables T;
@@ -542,6 +621,7 @@ HostTables build_tables(const BasisSpec& basis, int NL)
     uint32_t maxc = 0;
     for (auto c : class_count) maxc = std::max(maxc, c);
     T.lrank_u16 = (maxc <= 65536);
+    if (NL > 14) build_split_rank(basis, T);
     const uint64_t nhi = 1ULL << NH;
     T.hoff.assign(nhi, 0);
     uint64_t rows = 0;

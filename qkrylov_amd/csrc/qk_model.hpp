@@ -111,6 +111,18 @@ struct HostTables {
     std::vector<uint32_t> hoff, lrank, lolist;
     std::vector<QkTile> tiles;
     bool lrank_u16 = false;
+    // Two-level lo rank for wide lo fields (NL > 14), small enough for LDS:
+    // lo = (h << NL2) | l, rank(lo) = th[h * nc + c(l)] + tl[l], where
+    // c(l) = popcount(l & upm2) * KB + popcount(l & dnm2) is the class of
+    // the low half (a zero mask for a count the sector leaves free), tl[l]
+    // the rank of l among the valid low halves of its class, and th[h][c]
+    // the number of valid lo of the same total class whose high half is
+    // below h.
+    bool split = false;
+    int NL2 = 0, nc = 0, KB = 1;
+    uint32_t upm2 = 0, dnm2 = 0;
+    std::vector<uint16_t> tl;
+    std::vector<uint32_t> th;
 };
 
 HostTables build_tables(const BasisSpec& basis, int NL);

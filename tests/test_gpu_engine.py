@@ -398,3 +398,42 @@ def test_lanczos_graph_replay_equals_eager(check_every):
         for r in (b, c):
             assert np.array_equal(a.alphas, r.alphas) and np.array_equal(a.betas, r.betas)
             assert a.energy == r.energy and a.iterations == r.iterations
+
+
+@pytest.mark.parametrize("kind,n,sec,terms,splits", [
+    (O.SPIN, 20, dict(use_sz=1, sz2=0), models.j1j2_square(5, 4, 1.0, 0.5), (15, 16, 17, 18, 20)),
+    (O.SPIN, 18, dict(use_sz=1, sz2=4), models.xxz_chain(18, 0.7), (16, 18)),
+    (O.HUBBARD, 10, dict(use_nup=1, nup=5, use_ndn=1, ndn=4), models.hubbard_chain(10, 1.0, 4.0), (16, 18, 20)),
+    (O.TJ, 10, dict(use_nup=1, nup=4, use_ndn=1, ndn=4), models.tj_model(2, 5, 1.0, 0.4), (16, 18, 20)),
+])
+def test_two_level_lo_rank(kind, n, sec, terms, splits):
+    """Splits above 14 bits rank lo with two half-width LDS tables: rows
+    bit-exact against the oracle, applies within rounding, block applies
+    equal to the single-vector kernel."""
+    om = O.Model(kind, n, O.sector(**sec), kind, terms)
+    sample = np.unique(np.linspace(0, om.dim - 1, 200).astype(np.int64))
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(om.dim)
+    y_ref = om.apply(x + 0j).real
+    Xh = rng.uniform(-1, 1, (om.dim, 6))
+    for split in splits:
+        H = qa.MatrixFreeHamiltonian(engine_basis(kind, n, sec), SITE_CLS[kind](), qa.OpSum(terms), split_bits=split)
+        assert H.info["NL"] == split and H.dimension == om.dim
+        rows = H.row_entries(0, om.dim)
+        for r in sample:
+            c, v = om.row_entries(int(r))
+            assert set(merged(*rows[r])) == set(merged(c, v)), (split, r)
+            # off-diagonal values are single products (exact); the diagonal
+            # sums its terms in another order than the oracle
+            assert merged(*rows[r]) == pytest.approx(merged(c, v), rel=4e-15, abs=1e-15)
+        xd, yd = qa.DeviceArray.from_host(x), qa.DeviceArray(om.dim)
+        H.apply_device(xd.ptr, yd.ptr, 1)
+        y = yd.to_host()
+        assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref), split
+        X, Y = qa.DeviceArray.from_host(Xh), qa.DeviceArray(om.dim * 6)
+        H.apply_device(X.ptr, Y.ptr, 6)
+        Yh = Y.to_host().reshape(om.dim, 6)
+        for cidx in (0, 5):
+            x1, y1 = qa.DeviceArray.from_host(Xh[:, cidx]), qa.DeviceArray(om.dim)
+            H.apply_device(x1.ptr, y1.ptr, 1)
+            assert np.array_equal(Yh[:, cidx], y1.to_host()), (split, cidx)
