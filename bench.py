@@ -204,6 +204,9 @@ def bench_serve(args):
     reqs = serving.make_trace(args.serve_n, rate=50.0, seed=2024)
     out, tot = serving.run(reqs, realtime=True, max_active=args.serve_active)
     log(f"[bench-serve] {len(reqs)} requests: {tot}")
+    # capacity: the same requests all present at t = 0
+    _, sat = serving.run(reqs, realtime=False, max_active=args.serve_active)
+    log(f"[bench-serve] saturated: {sat}")
     res = {
         "metric": "serving solves/s with p50/p99 latency (FP64)",
         "value": round(tot["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": len(reqs), "warmup": 0,
@@ -214,6 +217,8 @@ def bench_serve(args):
         "p50_latency_s": round(tot["p50_s"], 5), "p99_latency_s": round(tot["p99_s"], 5),
         "wall_s": round(tot["wall_s"], 3),
         "converged": int(sum(o["converged"] for o in out)),
+        "saturated": {"solves_per_s": round(float(sat["solves_per_s"]), 3), "p50_latency_s": round(float(sat["p50_s"]), 5),
+                      "p99_latency_s": round(float(sat["p99_s"]), 5), "wall_s": round(float(sat["wall_s"]), 3)},
     }
     print(json.dumps(res), flush=True)
 

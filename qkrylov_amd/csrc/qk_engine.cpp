@@ -636,10 +636,10 @@ void RealCore::steps_graph(int k, int n)
         graph = g;
         hip_check(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "graph instantiate");
     }
-    // the counter is set by a copy outside the graph; three rotations leave
-    // the vector roles as they were
-    k_host = k;
-    hip_check(hipMemcpyAsync(kdev.get(), &k_host, sizeof(int), hipMemcpyHostToDevice, s), "set step");
+    // the counter is set by a one-thread kernel outside the graph (no host
+    // staging, so the caller never waits); three rotations leave the vector
+    // roles as they were
+    lcheck(qkl_set_int(static_cast<int*>(kdev.get()), k, s), "set step");
     for (int r = 0; r < n / 3; ++r) hip_check(hipGraphLaunch(gexec, s), "graph launch");
 }
 

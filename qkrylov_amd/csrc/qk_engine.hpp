@@ -143,7 +143,6 @@ struct RealCore {
     DevBuf kdev;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int k_host = 0;
     int cap_ic = -1;   // rotation state the graph was captured in
 
     RealCore(const Engine& e, int kcap, hipStream_t st, bool reorth, bool stream_ordered = false);

@@ -934,6 +934,11 @@ __global__ __launch_bounds__(QK_TPB) void k_copy(size_t n, const double* __restr
         dst[i] = src[i];
 }
 
+__global__ void k_set_int(int* __restrict__ p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
 __global__ __launch_bounds__(QK_TPB) void k_fill_zero(size_t n, double* __restrict__ x)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1438,6 +1443,12 @@ int qkl_fill_random(int kind, size_t nrows, uint64_t seed, uint64_t counter0, in
 int qkl_copy(size_t n, const double* src, double* dst, qkl_stream sv)
 {
     hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, src, dst);
+    return rc(hipGetLastError());
+}
+
+int qkl_set_int(int* p, int v, qkl_stream sv)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, (hipStream_t)sv, p, v);
     return rc(hipGetLastError());
 }
 

@@ -68,6 +68,8 @@ int qkl_copy_scaled(size_t n_doubles, const double* src, double* dst, const doub
 int qkl_fill_random(int kind, size_t n_doubles, uint64_t seed, uint64_t counter0, int ncol, int ld, int c0,
                     double* x, qkl_stream s);
 int qkl_fill_zero(size_t n_doubles, double* x, qkl_stream s);
+/* *p = v on the stream (a one-thread kernel: no host staging, graph-safe) */
+int qkl_set_int(int* p, int v, qkl_stream s);
 int qkl_copy(size_t n_doubles, const double* src, double* dst, qkl_stream s);
 
 /* h[j] partials: partials[j * npart + b] = sum_i V[j][i] u[i], j < m (real) */

@@ -110,6 +110,12 @@ void copy_scalars(Job& j, int nsteps)
 void enqueue_gs_chunk(Job& j)
 {
     const int kend = std::min(j.kcap, j.k + kChunk);
+    // whole turns of the vector rotation as graph replays, the rest eager
+    const int n3 = (kend - j.k) / 3 * 3;
+    if (n3 > 0) {
+        j.C->steps_graph(j.k, n3);
+        j.k += n3;
+    }
     for (; j.k < kend; ++j.k) j.C->step(j.k, nullptr);
     copy_scalars(j, j.k);
     hip_check(hipEventRecord(j.ev, j.s), "event");
@@ -191,7 +197,11 @@ void advance(Job& j, qkrylov_serve_result_t& res, double* cf_a, double* cf_b, in
         if (qkl_reduce(j.C->partB.d(), j.C->NB, 1, 1, 0, 0, j.psi2.d(), 0, 1, j.s) != 0)
             throw std::runtime_error("reduce");
         j.C->init_norm();
-        for (int kk = 0; kk < j.rq.cf_steps; ++kk) j.C->step(kk, nullptr);
+        {
+            const int n3 = j.rq.cf_steps / 3 * 3;
+            if (n3 > 0) j.C->steps_graph(0, n3);
+            for (int kk = n3; kk < j.rq.cf_steps; ++kk) j.C->step(kk, nullptr);
+        }
         copy_scalars(j, j.rq.cf_steps);
         hip_check(hipMemcpyAsync(j.ha->p + j.rq.cf_steps + 1, j.psi2.d(), sizeof(double), hipMemcpyDeviceToHost, j.s),
                   "d2h");
