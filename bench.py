@@ -409,7 +409,8 @@ def main():
 
     if not args.no_lanczos and args.nvec == 1:
         iters = args.lanczos_iters or w["lanczos"]
-        r = qa.lanczos_ground_state(H, maxiter=iters, reorthogonalize=False, fixed_steps=True, seed=1234)
+        r = qa.lanczos_ground_state(H, maxiter=iters, reorthogonalize=False, fixed_steps=True, seed=1234,
+                                   return_state=False)
         ips = r.report["steps_run"] / r.report["loop_seconds"]
         log(f"[bench] lanczos: {r.report['steps_run']} steps in {r.report['loop_seconds']:.3f}s "
             f"-> {ips:.2f} it/s, E({r.iterations})={r.energy:.12f}")

@@ -32,6 +32,14 @@ int qkrylov_amd_fill_random(double* x_dev, size_t n, uint64_t seed, int kind);
  * also the reference kernel for calibrating HBM byte counters */
 int qkrylov_amd_copy(double* dst_dev, const double* src_dev, size_t n);
 
+/* ---- sector ---- */
+/* Fixes one species of a Hubbard / t-J sector alone (species 0: up
+ * electrons, 1: down electrons), leaving the other free; the reference's
+ * Python Sector sets use_nup / use_ndn independently
+ * (bindings/python/qkrylov/basis.py:136-149), its C ABI only both at once
+ * (qkrylov_sector_set_hubbard_particles). */
+int qkrylov_amd_sector_set_species(qkrylov_sector_h sector, int species, int count);
+
 /* ---- Hamiltonian construction options and info ---- */
 /* NL: number of low state bits ranked per lane (-1: automatic) */
 qkrylov_hamiltonian_h qkrylov_hamiltonian_create_ex(qkrylov_basis_h basis, qkrylov_site_h site,

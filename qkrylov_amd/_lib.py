@@ -82,6 +82,7 @@ SIGNATURES = {
     "qkrylov_sector_set_sz": (c_int, [c_vp, c_int]),
     "qkrylov_sector_set_hubbard_particles": (c_int, [c_vp, c_int, c_int]),
     "qkrylov_sector_set_n": (c_int, [c_vp, c_int]),
+    "qkrylov_amd_sector_set_species": (c_int, [c_vp, c_int, c_int]),
     "qkrylov_sector_set_nb": (c_int, [c_vp, c_int]),
     "qkrylov_sector_get_sz": (c_int, [c_vp, p_int, p_int]),
     "qkrylov_sector_get_hubbard_particles": (c_int, [c_vp, p_int, p_int, p_int]),

@@ -1,18 +1,50 @@
 """Python mirror of the reference's Python package (bindings/python/qkrylov:
 basis.py, site.py, operators.py, hamiltonian.py, solvers.py) over the
-engine's C ABI. Default dtype is float64 (the reference defaults to
-float32). Device memory and streams are the engine's own; no torch."""
+engine's C ABI. The engine computes in float64 whatever `dtype` says;
+`dtype=np.float32` (the reference's default there) only makes the returned
+vectors complex64 as the reference's are. The default here is float64.
+Device memory and streams are the engine's own; no torch."""
 import ctypes
+import enum
 
 import numpy as np
 
 from . import _lib
 from ._lib import check
 
+
 # ---------------------------------------------------------------- operators
+class Op(str, enum.Enum):
+    """Local operator names (bindings/python/qkrylov/operators.py:6-22)."""
+    Sz = "Sz"
+    Sp = "Sp"
+    Sm = "Sm"
+    Sx = "Sx"
+    Sy = "Sy"
+    CdagUp = "CdagUp"
+    CUp = "CUp"
+    CdagDn = "CdagDn"
+    CDn = "CDn"
+    Nup = "Nup"
+    Ndn = "Ndn"
+    Nupdn = "Nupdn"
+    Bdag = "Bdag"
+    B = "B"
+    N = "N"
+
+
+def _op_name(o):
+    return o.value if isinstance(o, Op) else str(o)
+
+
+def _vec_dtype(dtype):
+    """Complex dtype of returned vectors: complex64 for float32 handles."""
+    return np.complex64 if np.dtype(dtype) == np.float32 else np.complex128
+
+
 class LocalOpExpr:
     def __init__(self, name, site):
-        self.name, self.site = name, int(site)
+        self.name, self.site = _op_name(name), int(site)
 
     def __mul__(self, other):
         if isinstance(other, LocalOpExpr):
@@ -102,16 +134,34 @@ class OpSum:
     """Ordered list of terms (coefficient, [(op, site), ...]); the first
     factor acts first, as in the reference."""
 
-    def __init__(self, terms=None):
+    def __init__(self, dtype=np.float64, terms=None):
+        if terms is None and not isinstance(dtype, (type, np.dtype, str)):
+            dtype, terms = np.float64, dtype      # OpSum([(coeff, factors), ...])
+        self.dtype = dtype
         self.terms = []
         for t in terms or []:
             self.add(*t)
 
     def add(self, coeff, factors):
-        self.terms.append((complex(coeff), [(str(o), int(s)) for o, s in factors]))
+        """Appends coeff * f0 * f1 * ... with factors [(name, site), ...]."""
+        self.terms.append((complex(coeff), [(_op_name(o), int(s)) for o, s in factors]))
         return self
 
+    def add_term(self, coeff, *ops):
+        """add_term(coeff, name0, site0, name1, site1, ...) as in
+        bindings/python/qkrylov/operators.py:151-161."""
+        if len(ops) % 2 != 0:
+            raise ValueError("Operators must be provided in (name, site) pairs.")
+        self.add(coeff, [(ops[i], ops[i + 1]) for i in range(0, len(ops), 2)])
+
     def __iadd__(self, other):
+        """Accepts (coeff, name0, site0, ...) tuples, algebraic terms and
+        sums of them, and other OpSums (operators.py:163-179)."""
+        if isinstance(other, tuple):
+            if len(other) < 3 or len(other) % 2 == 0:
+                raise ValueError("Term must be a tuple of (coeff, op1, site1, ...)")
+            self.add_term(other[0], *other[1:])
+            return self
         if isinstance(other, TermExpr):
             other = OpSumExpr([other])
         if isinstance(other, OpSumExpr):
@@ -121,10 +171,20 @@ class OpSum:
         if isinstance(other, OpSum):
             self.terms += other.terms
             return self
-        return NotImplemented
+        raise ValueError("Unsupported type for += on OpSum")
+
+    def clear(self):
+        self.terms = []
+
+    @property
+    def size(self):
+        return len(self.terms)
 
     def __len__(self):
         return len(self.terms)
+
+    def __repr__(self):
+        return f"OpSum(terms={self.size})"
 
     def _handle(self):
         L = _lib.load()
@@ -140,8 +200,12 @@ class OpSum:
 class Site:
     _create = None
 
-    def __init__(self):
+    def __init__(self, dtype=np.float64):
+        self.dtype = dtype
         self._h = getattr(_lib.load(), self._create)()
+
+    def __repr__(self):
+        return f"{type(self).__name__}()"
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -170,19 +234,28 @@ class Basis:
     _create = None
     site_class = None
 
-    def _build(self, N, sz2=None, n=None, nup=None, ndn=None):
+    def _build(self, N, dtype, sz2=None, n=None, nup=None, ndn=None):
+        self.dtype = dtype
         L = _lib.load()
         sec = L.qkrylov_sector_create()
         if sz2 is not None:
             L.qkrylov_sector_set_sz(sec, int(sz2))
         if n is not None:
             L.qkrylov_sector_set_n(sec, int(n))
-        if nup is not None or ndn is not None:
-            L.qkrylov_sector_set_hubbard_particles(sec, int(nup or 0), int(ndn or 0))
+        if nup is not None:
+            L.qkrylov_amd_sector_set_species(sec, 0, int(nup))
+        if ndn is not None:
+            L.qkrylov_amd_sector_set_species(sec, 1, int(ndn))
         self._h = getattr(L, self._create)(int(N), sec)
         L.qkrylov_sector_destroy(sec)
         if not self._h:
             raise ValueError(f"basis creation failed: {_lib.last_error()}")
+
+    def _sector_repr(self):
+        return ""
+
+    def __repr__(self):
+        return f"{type(self).__name__}(N={self.nsites}, dim={self.size}{self._sector_repr()})"
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -208,33 +281,55 @@ class Basis:
 
 
 class SpinHalfBasis(Basis):
+    """Spin-1/2 basis; `sz` (or conserve_sz, which means sz = 0) fixes total
+    Sz (bindings/python/qkrylov/basis.py:49-84). `sz2` = 2 Sz is accepted
+    too."""
     _create = "qkrylov_spinhalf_basis_create"
     site_class = SpinHalfSite
 
-    def __init__(self, N, conserve_sz=False, sz=None, sz2=None):
+    def __init__(self, N, conserve_sz=False, sz=None, dtype=np.float64, sz2=None):
         if sz2 is None and (sz is not None or conserve_sz):
             sz2 = int(2 * (sz or 0))
-        self._build(N, sz2=sz2)
+        self.sz2 = sz2
+        self._build(N, dtype, sz2=sz2)
+
+    def _sector_repr(self):
+        return "" if self.sz2 is None else f", sz={self.sz2 / 2:g}"
 
 
 class FermionBasis(Basis):
+    """Spinless fermions; n != 0 or conserve_n fixes the particle number
+    (basis.py:87-114)."""
     _create = "qkrylov_fermion_basis_create"
     site_class = FermionSite
 
-    def __init__(self, N, conserve_n=False, n=0):
-        self._build(N, n=n if (conserve_n or n) else None)
+    def __init__(self, N, conserve_n=False, n=0, dtype=np.float64):
+        self.n = n if (conserve_n or n) else None
+        self._build(N, dtype, n=self.n)
+
+    def _sector_repr(self):
+        return "" if self.n is None else f", n={self.n}"
 
 
 class HubbardBasis(Basis):
+    """Spin-1/2 electrons, up at bit 2i and down at bit 2i+1; each species
+    is fixed on its own by nup != 0 / conserve_nup and ndn != 0 /
+    conserve_ndn (basis.py:117-155)."""
     _create = "qkrylov_hubbard_basis_create"
     site_class = HubbardSite
 
-    def __init__(self, N, conserve_nup=False, nup=0, conserve_ndn=False, ndn=0):
-        fix = conserve_nup or conserve_ndn or nup or ndn
-        self._build(N, nup=nup if fix else None, ndn=ndn if fix else None)
+    def __init__(self, N, conserve_nup=False, nup=0, conserve_ndn=False, ndn=0, dtype=np.float64):
+        self.nup = nup if (conserve_nup or nup) else None
+        self.ndn = ndn if (conserve_ndn or ndn) else None
+        self._build(N, dtype, nup=self.nup, ndn=self.ndn)
+
+    def _sector_repr(self):
+        parts = [f"{k}={v}" for k, v in (("nup", self.nup), ("ndn", self.ndn)) if v is not None]
+        return ", " + ", ".join(parts) if parts else ""
 
 
 class TJBasis(HubbardBasis):
+    """Hubbard layout without double occupancy (basis.py:158-187)."""
     _create = "qkrylov_tj_basis_create"
     site_class = TJSite
 
@@ -246,9 +341,16 @@ def _dp(a):
 
 class MatrixFreeHamiltonian:
     """y = H x with the reference's conventions (row a sums
-    conj(<b|term|a>) x[b]); FP64 on the device."""
+    conj(<b|term|a>) x[b]); float64 on the device.
 
-    def __init__(self, basis, site=None, ops=None, split_bits=-1, operator=False):
+    `device` and `dtype` are accepted as in
+    bindings/python/qkrylov/hamiltonian.py:24-70. The engine has only the
+    MI355X backend: "cpu" (the reference's default), "gpu", "hip" and "cuda"
+    all run it on the current HIP device, and a ":N" suffix selects device
+    N first. dtype float32 returns complex64 vectors as the reference's FP32
+    build does; the arithmetic stays float64."""
+
+    def __init__(self, basis, site=None, ops=None, device="cpu", dtype=None, split_bits=-1, operator=False):
         if ops is None and isinstance(site, (OpSum, OpSumExpr, TermExpr)):
             site, ops = None, site
         if site is None:
@@ -257,8 +359,15 @@ class MatrixFreeHamiltonian:
             o = OpSum()
             o += ops
             ops = o
+        if dtype is None:
+            dtype = getattr(ops, "dtype", np.float64)
         L = _lib.load()
         self.basis, self.site, self.ops = basis, site, ops
+        self.device, self.dtype = str(device), dtype
+        self._split_bits, self._operator = int(split_bits), bool(operator)
+        dev = self.device.rpartition(":")[2] if ":" in self.device else None
+        if dev is not None:
+            check(L.qkrylov_amd_set_device(int(dev)), f"device {self.device}")
         oh = ops._handle()
         try:
             if operator:
@@ -270,6 +379,13 @@ class MatrixFreeHamiltonian:
         if not self._h:
             raise ValueError(f"Hamiltonian creation failed: {_lib.last_error()}")
         self._refresh_info()
+
+    def to(self, device=None, dtype=None):
+        """New handle over the same basis, site and terms
+        (hamiltonian.py:113-117)."""
+        return MatrixFreeHamiltonian(self.basis, self.site, self.ops, device=device or self.device,
+                                     dtype=dtype or self.dtype, split_bits=self._split_bits,
+                                     operator=self._operator)
 
     def _refresh_info(self):
         info = _lib.HamInfo()
@@ -290,18 +406,43 @@ class MatrixFreeHamiltonian:
         return bool(self.info["is_real"])
 
     def apply(self, x):
+        """y = H x for one vector of `dimension` entries (complex128, or
+        complex64 for a float32 handle); a wrong length raises ValueError."""
         x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.ndim != 1 or x.shape[0] != self.dimension:
+            raise ValueError(f"input vector has shape {x.shape}, expected ({self.dimension},)")
         y = np.zeros(self.dimension, dtype=np.complex128)
         check(_lib.load().qkrylov_hamiltonian_apply_complex_f64(self._h, _dp(x.view(np.float64)),
                                                                _dp(y.view(np.float64))), "apply")
-        return y
+        return y.astype(_vec_dtype(self.dtype), copy=False)
 
-    __matmul__ = apply
+    def __matmul__(self, x):
+        return self.apply(x)
 
     def diagonal(self):
         d = np.zeros(self.dimension)
         check(_lib.load().qkrylov_hamiltonian_diagonal_f64(self._h, _dp(d)), "diagonal")
-        return d
+        return d.astype(np.float32) if np.dtype(self.dtype) == np.float32 else d
+
+    def to_sparse(self):
+        """Explicit CSR matrix (hamiltonian.py:147-180: column i holds H e_i),
+        built from each row's entries instead of D applies; for small
+        systems."""
+        import scipy.sparse as sp
+        D = self.dimension
+        rows, cols, vals = [], [], []
+        step = 4096
+        for r0 in range(0, D, step):
+            for i, (c, v) in enumerate(self.row_entries(r0, min(step, D - r0))):
+                rows.append(np.full(len(c), r0 + i, dtype=np.int64))
+                cols.append(c)
+                vals.append(v)
+        rows = np.concatenate(rows) if rows else np.zeros(0, np.int64)
+        cols = np.concatenate(cols) if cols else np.zeros(0, np.int64)
+        vals = np.concatenate(vals) if vals else np.zeros(0, np.complex128)
+        keep = vals != 0
+        m = sp.csr_matrix((vals[keep], (rows[keep], cols[keep])), shape=(D, D), dtype=np.complex128)
+        return m.astype(_vec_dtype(self.dtype))
 
     def apply_device(self, x_ptr, y_ptr, nvec=1, stream=None):
         check(_lib.load().qkrylov_hamiltonian_apply_device_f64(self._h, x_ptr, y_ptr, nvec, stream), "apply_device")
@@ -359,7 +500,7 @@ class MatrixFreeHamiltonian:
 
     def aslinearoperator(self):
         import scipy.sparse.linalg as sla
-        return sla.LinearOperator((self.dimension, self.dimension), matvec=self.apply, dtype=np.complex128)
+        return sla.LinearOperator((self.dimension, self.dimension), matvec=self.apply, dtype=_vec_dtype(self.dtype))
 
     def __repr__(self):
         return f"MatrixFreeHamiltonian(dim={self.dimension})"
@@ -375,10 +516,12 @@ class LanczosResult:
         return f"LanczosResult(energy={self.energy:.12f}, iterations={self.iterations}, converged={self.converged})"
 
 
-def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return_state=False, seed=1234,
+def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return_state=True, seed=1234,
                          fixed_steps=False, start=None, check_every=0, graph=None):
-    """graph: None automatic (replay a captured step graph below 2^22 rows
-    when no basis is stored), True always, False never."""
+    """Ground state energy and, as in bindings/python/qkrylov/solvers.py:24-51,
+    the eigenvector (complex, `H.dimension` entries; return_state=False
+    skips it). graph: None automatic (replay a captured step graph below
+    2^22 rows when no Krylov basis is stored), True always, False never."""
     L = _lib.load()
     gm = 0 if graph is None else (1 if graph else 2)
     o = _lib.LanczosOptions(int(maxiter), float(tol), 1 if reorthogonalize else 0, 1 if fixed_steps else 0,
@@ -393,8 +536,9 @@ def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return
     rep = _lib.LanczosReport()
     check(L.qkrylov_lanczos_run_f64(H._h, ctypes.byref(o), _dp(st) if st is not None else None, _dp(al), _dp(be),
                                      _dp(vec) if vec is not None else None, ctypes.byref(rep)), "lanczos")
-    if vec is not None and not H.is_real:
-        vec = vec.view(np.complex128)
+    if vec is not None:
+        vec = vec.view(np.complex128) if not H.is_real else vec
+        vec = vec.astype(_vec_dtype(H.dtype), copy=False)
     return LanczosResult(rep.energy, rep.iterations, bool(rep.converged), vec, al[:rep.n_alpha].copy(),
                          be[:rep.n_beta].copy(), {f: getattr(rep, f) for f, _ in rep._fields_})
 
@@ -404,6 +548,9 @@ class DavidsonResult:
         self.eigenvalues, self.eigenvectors = eigenvalues, eigenvectors
         self.iterations, self.converged = iterations, converged
 
+    def __repr__(self):
+        return f"DavidsonResult(energies={self.eigenvalues})"
+
 
 def davidson_lowest(H, n_eig=1, max_subspace=20, tol=1e-8):
     ev = np.zeros(n_eig)
@@ -411,7 +558,8 @@ def davidson_lowest(H, n_eig=1, max_subspace=20, tol=1e-8):
     info = _lib.DavidsonResultF64()
     check(_lib.load().qkrylov_davidson_lowest_complex_f64(H._h, n_eig, max_subspace, tol, _dp(ev), _dp(vecs),
                                                          ctypes.byref(info)), "davidson")
-    return DavidsonResult(ev, [v.view(np.complex128) for v in vecs], info.iterations, bool(info.converged))
+    return DavidsonResult(ev, [v.view(np.complex128).astype(_vec_dtype(H.dtype), copy=False) for v in vecs],
+                          info.iterations, bool(info.converged))
 
 
 class DynamicsResult:
@@ -421,6 +569,8 @@ class DynamicsResult:
 
 def continued_fraction_coeffs(H, phi0, n_iter=100):
     phi = np.ascontiguousarray(phi0, dtype=np.complex128)
+    if phi.ndim != 1 or phi.shape[0] != H.dimension:
+        raise ValueError(f"phi0 has shape {phi.shape}, expected ({H.dimension},)")
     al, be = np.zeros(n_iter), np.zeros(n_iter)
     nrm = ctypes.c_double()
     na = ctypes.c_int()

@@ -121,6 +121,18 @@ int qkrylov_sector_set_hubbard_particles(qkrylov_sector_h s, int nup, int ndn)
     s->sector.ndn = ndn;
     return QKRYLOV_SUCCESS;
 }
+int qkrylov_amd_sector_set_species(qkrylov_sector_h s, int species, int count)
+{
+    if (!s || species < 0 || species > 1) return QKRYLOV_ERROR_INVALID_ARG;
+    if (species == 0) {
+        s->sector.use_nup = true;
+        s->sector.nup = count;
+    } else {
+        s->sector.use_ndn = true;
+        s->sector.ndn = count;
+    }
+    return QKRYLOV_SUCCESS;
+}
 int qkrylov_sector_set_n(qkrylov_sector_h s, int n)
 {
     if (!s) return QKRYLOV_ERROR_INVALID_ARG;
