@@ -232,6 +232,8 @@ def config_flags(args):
         f += ["--block-waves", str(args.block_waves)]
     if args.apply_waves is not None:
         f += ["--apply-waves", str(args.apply_waves)]
+    if args.block_rows is not None:
+        f += ["--block-rows", str(args.block_rows)]
     return f
 
 
@@ -294,7 +296,10 @@ def main():
     ap.add_argument("--nvec", type=int, default=1,
                     help="columns of the applied block (1: single-vector kernel; >1: row-major block kernel)")
     ap.add_argument("--sched", type=int, default=None,
-                    help="tile schedule (0: one segment per XCD, 1: one front; default: the engine's choice)")
+                    help="tile schedule (0: one segment per XCD, 1: one front, 2: one front of XCD-contiguous "
+                         "chunks; default: the engine's choice)")
+    ap.add_argument("--block-rows", type=int, default=None,
+                    help="rows per work item of the block apply (64, 32, 16)")
     ap.add_argument("--block-waves", type=int, default=None, help="waves per workgroup of the block apply (16, 8, 4)")
     ap.add_argument("--apply-waves", type=int, default=None, help="waves per workgroup of the apply (4, 8, 16)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
@@ -351,6 +356,8 @@ def main():
         tune["block_waves"] = args.block_waves
     if args.apply_waves is not None:
         tune["apply_waves"] = args.apply_waves
+    if args.block_rows is not None:
+        tune["block_rows"] = args.block_rows
     H.set_tuning(**tune)
     tune_report = None
     if args.autotune:
@@ -362,7 +369,8 @@ def main():
             # counter runs as child processes with the configuration chosen here
             t = H.tuning()
             flags = ["--split-bits", str(H.info["NL"]), "--nvec", str(args.nvec),
-                     "--apply-waves", str(t["apply_waves"]), "--block-waves", str(t["block_waves"])]
+                     "--apply-waves", str(t["apply_waves"]), "--block-waves", str(t["block_waves"]),
+                     "--block-rows", str(t["block_rows"])]
             sched = t["block_schedule"] if args.nvec > 1 else t["schedule"]
             flags += ["--sched", str(sched)]
             traffic, traffic_info = measure_traffic(args, flags)

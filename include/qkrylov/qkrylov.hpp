@@ -273,6 +273,7 @@ public:
     struct TuneReport {
         int split_bits, apply_waves, schedule, block_waves, block_schedule, candidates;
         double ms_before, ms_after;
+        int block_rows;
     };
     TuneReport autotune(int nvec = 1, bool allow_split = true);
 

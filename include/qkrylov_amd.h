@@ -56,12 +56,15 @@ typedef struct {
     uint64_t table_bytes;
 } qkrylov_hamiltonian_info_t;
 int qkrylov_hamiltonian_info(qkrylov_hamiltonian_h h, qkrylov_hamiltonian_info_t* info);
-/* Launch tuning of the apply kernels. Results are bitwise the same under
- * every setting; only speed differs. Not to be changed while applies or a
+/* Launch tuning of the apply kernels. The apply's output is bitwise the
+ * same under every setting; only speed differs (the Lanczos drivers' fused
+ * per-workgroup partial sums follow the tile-to-workgroup assignment, so
+ * alpha can change in the last bits). Not to be changed while applies or a
  * solver on this handle are running.
  *   QKRYLOV_TUNE_SCHEDULE        tile schedule of the single-vector apply:
  *                                0 one contiguous tile segment per XCD
- *                                (default), 1 one front swept by all XCDs
+ *                                (default), 1 one front swept by all XCDs,
+ *                                2 one front of XCD-contiguous chunks
  *   QKRYLOV_TUNE_BLOCK_SCHEDULE  the same for the multi-vector apply
  *                                (default 1)
  *   QKRYLOV_TUNE_APPLY_WAVES     waves per workgroup of the single-vector
@@ -69,23 +72,30 @@ int qkrylov_hamiltonian_info(qkrylov_hamiltonian_h h, qkrylov_hamiltonian_info_t
  *                                4, 8 or 16 (default)
  *   QKRYLOV_TUNE_BLOCK_WAVES     waves per workgroup of the multi-vector
  *                                apply: 16, 8 (default) or 4, with 8, 16 or
- *                                32 gathers in flight per wave */
+ *                                32 gathers in flight per wave
+ *   QKRYLOV_TUNE_BLOCK_ROWS      rows per work item of the multi-vector
+ *                                apply: 64 (default), 32 or 16; fewer rows
+ *                                in flight share more of their gathers in
+ *                                the caches */
 enum {
     QKRYLOV_TUNE_SCHEDULE = 0,
     QKRYLOV_TUNE_BLOCK_SCHEDULE = 1,
     QKRYLOV_TUNE_APPLY_WAVES = 2,
-    QKRYLOV_TUNE_BLOCK_WAVES = 3
+    QKRYLOV_TUNE_BLOCK_WAVES = 3,
+    QKRYLOV_TUNE_BLOCK_ROWS = 4
 };
 int qkrylov_hamiltonian_set_tuning(qkrylov_hamiltonian_h h, int knob, int value);
 
-/* Autotuner: times y = H x (nvec columns, random vectors) under every
- * launch tuning and, when allow_split, the lo-bit splits around the current
- * one, and keeps the fastest for this handle. Launch tunings never change
- * results; another split changes the order in which a row's entries are
- * added, i.e. the last bits. Allocates two vectors of nvec columns. */
+/* Autotuner: times y = H x (nvec columns, random vectors) under the launch
+ * tunings (for nvec > 1 also the work-item widths) and, when allow_split,
+ * the lo-bit splits around the current one, and keeps the fastest for this
+ * handle. Launch tunings never change the apply's output; another split
+ * changes the order in which a row's entries are added, i.e. the last bits.
+ * Allocates two vectors of nvec columns. */
 typedef struct {
     int split_bits, apply_waves, schedule, block_waves, block_schedule, candidates;
     double ms_before, ms_after;
+    int block_rows;
 } qkrylov_tune_report_t;
 int qkrylov_hamiltonian_autotune(qkrylov_hamiltonian_h h, int nvec, int allow_split, qkrylov_tune_report_t* report);
 int qkrylov_hamiltonian_get_tuning(qkrylov_hamiltonian_h h, int knob, int* value);

@@ -72,7 +72,8 @@ class ServeOptions(ctypes.Structure):
 # name -> (restype, argtypes)
 class TuneReport(ctypes.Structure):
     _fields_ = [("split_bits", c_int), ("apply_waves", c_int), ("schedule", c_int), ("block_waves", c_int),
-                ("block_schedule", c_int), ("candidates", c_int), ("ms_before", c_double), ("ms_after", c_double)]
+                ("block_schedule", c_int), ("candidates", c_int), ("ms_before", c_double), ("ms_after", c_double),
+                ("block_rows", c_int)]
 
 
 SIGNATURES = {

@@ -467,12 +467,14 @@ class MatrixFreeHamiltonian:
         check(_lib.load().qkrylov_hamiltonian_states(self._h, row0, n, out.ctypes.data_as(_lib.p_u64)), "states")
         return out
 
-    TUNING_KNOBS = {"schedule": 0, "block_schedule": 1, "apply_waves": 2, "block_waves": 3}
+    TUNING_KNOBS = {"schedule": 0, "block_schedule": 1, "apply_waves": 2, "block_waves": 3, "block_rows": 4}
 
     def set_tuning(self, **knobs):
-        """Launch tuning of the apply kernels (same results, different
+        """Launch tuning of the apply kernels (same apply output, different
         speed): schedule / block_schedule (0 one tile segment per XCD, 1 one
-        front over all XCDs), apply_waves (4, 8, 16), block_waves (16, 8, 4)."""
+        front over all XCDs, 2 one front of XCD-contiguous chunks),
+        apply_waves (4, 8, 16), block_waves (16, 8, 4), block_rows (64, 32,
+        16 rows per work item of the multi-vector apply)."""
         for k, v in knobs.items():
             check(_lib.load().qkrylov_hamiltonian_set_tuning(self._h, self.TUNING_KNOBS[k], int(v)), f"set_tuning {k}")
 

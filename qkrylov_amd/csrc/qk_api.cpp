@@ -155,7 +155,7 @@ MatrixFreeHamiltonian::TuneReport MatrixFreeHamiltonian::autotune(int nvec, bool
     qk::TuneResult r;
     engine_ = qk::autotune(basis_->spec(), static_cast<qk::Kind>(site_->kind()), to_qk(ops_), engine_, nvec,
                            allow_split, qk::default_stream(), r);
-    return TuneReport{r.NL, r.apply_waves, r.sched, r.blk_waves, r.blk_sched, r.tried, r.ms_before, r.ms};
+    return TuneReport{r.NL, r.apply_waves, r.sched, r.blk_waves, r.blk_sched, r.tried, r.ms_before, r.ms, r.blk_rows};
 }
 
 Index MatrixFreeHamiltonian::dimension() const { return static_cast<Index>(engine_->dim()); }

@@ -673,6 +673,7 @@ int qkrylov_hamiltonian_autotune(qkrylov_hamiltonian_h h, int nvec, int allow_sp
         report->candidates = r.candidates;
         report->ms_before = r.ms_before;
         report->ms_after = r.ms_after;
+        report->block_rows = r.block_rows;
     }
     return QKRYLOV_SUCCESS;
     QK_CATCH

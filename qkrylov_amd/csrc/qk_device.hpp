@@ -64,8 +64,9 @@ struct QkTables {
     int32_t lrank_u16;        // 1: lrank values fit 16 bits (LDS copy is u16)
     int32_t nlo;              // 2^NL
     int32_t sched;            // tile schedule of the single-vector apply: 0 one contiguous segment per XCD,
-                              // 1 one front over all XCDs
+                              // 1 one front over all XCDs, 2 one front of XCD-contiguous chunks
     int32_t sched_block;      // tile schedule of the multi-vector apply
+    int32_t blk_rows;         // rows per work item of the multi-vector apply (64, 32 or 16; 0: 64)
     int32_t blk_waves;        // waves per workgroup of the multi-vector apply (16, 8 or 4; 0: 16)
     int32_t signs;            // 1: some branch carries a parity mask (fermionic signs)
     int32_t apply_waves;      // waves per workgroup of the single-vector apply (4, 8 or 16; 0: 4)

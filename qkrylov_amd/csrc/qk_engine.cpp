@@ -253,6 +253,7 @@ Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpS
         T_.dnm2 = h.dnm2;
     }
     T_.blk_waves = 8;
+    T_.blk_rows = 64;
     T_.apply_waves = 16;
     // one XCD segment each for the single-vector apply (its x mostly fits the
     // caches); one front for the block apply, whose 512 B rows do not
@@ -291,6 +292,7 @@ std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& b
                                        const std::shared_ptr<const Engine>& current, int nvec, bool allow_split,
                                        hipStream_t s, TuneResult& out)
 {
+    constexpr int kKnobs = 5;
     if (nvec < 1) throw std::invalid_argument("autotune: nvec must be positive");
     if (nvec > 1 && !current->is_real()) throw std::invalid_argument("autotune: block apply needs a real Hamiltonian");
     const size_t D = (size_t)current->dim();
@@ -321,8 +323,8 @@ std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& b
     };
     out.ms_before = timed(*current);
     double best_ms = out.ms_before;
-    int best_NL = current->NL(), best_tune[4];
-    for (int k = 0; k < 4; ++k) best_tune[k] = current->tuning(k);
+    int best_NL = current->NL(), best_tune[kKnobs];
+    for (int k = 0; k < kKnobs; ++k) best_tune[k] = current->tuning(k);
     bool improved = false;
     std::vector<int> splits{current->NL()};
     if (allow_split) {
@@ -340,32 +342,57 @@ std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& b
         }
     }
     const int knob_w = nvec == 1 ? 2 : 3, knob_s = nvec == 1 ? 0 : 1;
-    for (int NL : splits) {
+    auto try_shape = [&](const std::shared_ptr<Engine>& E, int NL) {
+        double ms;
+        try {
+            ms = timed(*E);
+        } catch (const std::exception&) {
+            return;   // a launch shape that does not fit this split
+        }
+        ++out.tried;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_NL = NL;
+            for (int k = 0; k < kKnobs; ++k) best_tune[k] = E->tuning(k);
+            improved = true;
+        }
+    };
+    auto make = [&](int NL) -> std::shared_ptr<Engine> {
         EngineOptions eo;
         eo.NL = NL;
-        std::shared_ptr<Engine> E;
         try {
-            E = std::make_shared<Engine>(basis, kind, ops, eo);
+            auto E = std::make_shared<Engine>(basis, kind, ops, eo);
+            for (int k = 0; k < kKnobs; ++k) E->set_tuning(k, current->tuning(k));
+            return E;
         } catch (const std::exception&) {
-            continue;   // a split whose tables cannot be built is not a candidate
+            return nullptr;   // a split whose tables cannot be built is not a candidate
         }
-        for (int k = 0; k < 4; ++k) E->set_tuning(k, current->tuning(k));
+    };
+    // stage 1: split x waves x schedule
+    for (int NL : splits) {
+        auto E = make(NL);
+        if (!E) continue;
         for (int w : {8, 16}) {
-            for (int sc : {0, 1}) {
+            for (int sc : {0, 1, 2}) {
                 E->set_tuning(knob_w, w);
                 E->set_tuning(knob_s, sc);
-                double ms;
-                try {
-                    ms = timed(*E);
-                } catch (const std::exception&) {
-                    continue;   // a launch shape that does not fit this split
-                }
-                ++out.tried;
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best_NL = NL;
-                    for (int k = 0; k < 4; ++k) best_tune[k] = E->tuning(k);
-                    improved = true;
+                try_shape(E, NL);
+            }
+        }
+    }
+    // stage 2 (block apply): narrower work items, fewer waves, at the best split
+    if (nvec > 1) {
+        auto E = make(best_NL);
+        if (E) {
+            for (int k = 0; k < kKnobs; ++k) E->set_tuning(k, best_tune[k]);
+            for (int rows : {32, 16}) {
+                for (int w : {4, 8}) {
+                    for (int sc : {1, 2}) {
+                        E->set_tuning(4, rows);
+                        E->set_tuning(knob_w, w);
+                        E->set_tuning(knob_s, sc);
+                        try_shape(E, best_NL);
+                    }
                 }
             }
         }
@@ -375,7 +402,7 @@ std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& b
         EngineOptions eo;
         eo.NL = best_NL;
         auto E = std::make_shared<Engine>(basis, kind, ops, eo);
-        for (int k = 0; k < 4; ++k) E->set_tuning(k, best_tune[k]);
+        for (int k = 0; k < kKnobs; ++k) E->set_tuning(k, best_tune[k]);
         best = E;
     }
     out.ms = best_ms;
@@ -384,6 +411,7 @@ std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& b
     out.blk_sched = best->tuning(1);
     out.apply_waves = best->tuning(2);
     out.blk_waves = best->tuning(3);
+    out.blk_rows = best->tuning(4);
     return best;
 }
 
@@ -394,19 +422,22 @@ int Engine::tuning(int knob) const
     case 1: return T_.sched_block;
     case 2: return T_.apply_waves;
     case 3: return T_.blk_waves;
+    case 4: return T_.blk_rows;
     default: throw std::invalid_argument("unknown tuning knob");
     }
 }
 
 void Engine::set_tuning(int knob, int value) const
 {
-    const bool sched_ok = value == 0 || value == 1;
+    const bool sched_ok = value >= 0 && value <= 2;
     const bool waves_ok = value == 4 || value == 8 || value == 16;
+    const bool rows_ok = value == 16 || value == 32 || value == 64;
     switch (knob) {
     case 0: if (!sched_ok) break; T_.sched = value; return;
     case 1: if (!sched_ok) break; T_.sched_block = value; return;
     case 2: if (!waves_ok) break; T_.apply_waves = value; return;
     case 3: if (!waves_ok) break; T_.blk_waves = value; return;
+    case 4: if (!rows_ok) break; T_.blk_rows = value; return;
     default: throw std::invalid_argument("unknown tuning knob");
     }
     throw std::invalid_argument("tuning value out of range");

@@ -104,7 +104,7 @@ double now_s();
 // Launch tunings never change results; a different split changes the order
 // in which a row's entries are added, i.e. the last bits.
 struct TuneResult {
-    int NL = -1, apply_waves = 0, sched = 0, blk_waves = 0, blk_sched = 0;
+    int NL = -1, apply_waves = 0, sched = 0, blk_waves = 0, blk_sched = 0, blk_rows = 64;
     double ms = 0.0, ms_before = 0.0;
     int tried = 0;
 };

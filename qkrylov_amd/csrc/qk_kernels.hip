@@ -30,6 +30,7 @@ struct QkScal {
     uint32_t lomask;
     uint32_t ntiles;
     int32_t nbr, ndg, NL, nlo, nA, nB, sched;
+    int32_t tsh;              // block apply: each tile is 2^tsh work items of 64 >> tsh rows
     int32_t NL2, nc, KB, ntl, nB1, nB2;
     uint32_t upm2, dnm2;
     double dcr, dci;
@@ -47,6 +48,7 @@ QkScal make_scal(const QkTables* T)
     s.NL = T->NL;
     s.nlo = T->nlo;
     s.sched = T->sched;
+    s.tsh = 0;
     s.NL2 = T->NL2;
     s.nc = T->nc;
     s.KB = T->KB;
@@ -77,19 +79,31 @@ __device__ __forceinline__ int popc<uint64_t>(uint64_t x) { return __builtin_pop
 //  sched 1: all waves of the grid sweep the tiles together in one front
 //           (wave g takes tiles g, g + G, ...), so the rows in flight on all
 //           XCDs share one window of the memory-side cache, which then also
-//           holds the targets of branches that leave the block.
+//           holds the targets of branches that leave the block;
+//  sched 2: one front as in 1, cut into XCD-contiguous chunks of as many
+//           tiles as an XCD has waves: chunk j runs on XCD j % 8, so a block
+//           of consecutive tiles stays on one XCD (its L2 serves the in-block
+//           and repeated targets) while the eight XCDs still sweep one narrow
+//           window of the memory-side cache.
 __device__ __forceinline__ void tile_schedule(uint32_t ntiles, int sched, uint32_t& t0, uint32_t& tend,
                                               uint32_t& step)
 {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t w = rfl(threadIdx.x >> 6), wpb = blockDim.x >> 6;
-    if (sched == 0 && nb >= 8 && (nb & 7u) == 0) {
+    if (sched != 1 && nb >= 8 && (nb & 7u) == 0) {
         const uint32_t xcd = b & 7u, loc = b >> 3, nloc = nb >> 3;
-        const uint32_t seg = (ntiles + 7u) / 8u;
-        const uint32_t s0 = min(xcd * seg, ntiles), s1 = min(s0 + seg, ntiles);
-        t0 = s0 + loc * wpb + w;
-        tend = s1;
-        step = nloc * wpb;
+        if (sched == 0) {
+            const uint32_t seg = (ntiles + 7u) / 8u;
+            const uint32_t s0 = min(xcd * seg, ntiles), s1 = min(s0 + seg, ntiles);
+            t0 = s0 + loc * wpb + w;
+            tend = s1;
+            step = nloc * wpb;
+        } else {
+            const uint32_t chunk = nloc * wpb;
+            t0 = xcd * chunk + loc * wpb + w;
+            tend = ntiles;
+            step = 8u * chunk;
+        }
     } else {
         t0 = b * wpb + w;
         tend = ntiles;
@@ -561,15 +575,23 @@ __global__ __launch_bounds__(64 * NW) void k_apply_block(
     const size_t lds = (size_t)ld;
     double part = 0.0;
 
+    // work item it = rows [wbeg, wbeg + (64 >> tsh)) of tile it >> tsh: smaller
+    // items narrow the rows in flight (and the window of targets they share)
+    // at the cost of repeating the tile's uniform branch preparation
     uint32_t t0, tend, step;
-    tile_schedule(sc.ntiles, sc.sched, t0, tend, step);
-    for (uint32_t t = t0; t < tend; t += step) {
+    tile_schedule(sc.ntiles << sc.tsh, sc.sched, t0, tend, step);
+    const uint32_t rsub = 64u >> sc.tsh;
+    for (uint32_t it = t0; it < tend; it += step) {
+        const uint32_t t = it >> sc.tsh;
+        const uint32_t wbeg = (it & ((1u << sc.tsh) - 1u)) * rsub;
         const QkTile tl = tiles[t];
         const uint32_t hi = tl.hi;
         const uint32_t count = tl.count;
-        const bool active = lane < count;
+        if (wbeg >= count) continue;
+        const uint32_t wend = min(count, wbeg + rsub);
+        const bool active = lane >= wbeg && lane < wend;
         const uint32_t base_self = hoff[hi];
-        const uint32_t lidx = active ? lane : 0u;
+        const uint32_t lidx = lane < count ? lane : 0u;
         const uint32_t my_lr = tl.row_start - base_self + lidx;
         const uint32_t lo = lolist[tl.lo_base + lidx];
         const ST s = (static_cast<ST>(hi) << NL) | static_cast<ST>(lo);
@@ -624,8 +646,8 @@ __global__ __launch_bounds__(64 * NW) void k_apply_block(
         }
         const uint32_t excl = incl - n_j;
 
-        uint32_t j0 = 0;
-        while (j0 < count) {
+        uint32_t j0 = wbeg;
+        while (j0 < wend) {
             const uint32_t ebase = __builtin_amdgcn_readfirstlane(__shfl(excl, (int)j0, 64));
             const bool inr = (lane >= j0) && active && (incl - ebase <= (uint32_t)cap);
             const uint64_t bal = __ballot(inr);
@@ -1190,6 +1212,9 @@ struct BlockCfg {
     size_t lds;
 };
 
+// work items per tile of the block apply, as a shift: 64 rows 0, 32 rows 1, 16 rows 2
+inline int block_tsh(const QkTables* T) { return T->blk_rows == 16 ? 2 : (T->blk_rows == 32 ? 1 : 0); }
+
 inline bool block_config(const QkTables* T, BlockCfg& c)
 {
     const int nbk = T->nA + T->nB;
@@ -1208,7 +1233,7 @@ inline bool block_config(const QkTables* T, BlockCfg& c)
         c.nw = nw;
         c.cap = (int)cap;
         c.lds = shared + (size_t)nw * block_wave_bytes(nbk, T->nA, (int)cap);
-        uint32_t nb = (T->ntiles + nw - 1) / nw;
+        uint32_t nb = ((T->ntiles << block_tsh(T)) + nw - 1) / nw;
         if (nb > 256u) nb = 256u;
         if (nb >= 8) nb = (nb + 7u) & ~7u;
         if (nb == 0) nb = 1;
@@ -1247,6 +1272,7 @@ int launch_block_nw(const QkTables* T, const BlockCfg& c, const double* X, doubl
     if (LZ && npart != c.nb) return -1;
     QkScal sc = make_scal(T);
     sc.sched = T->sched_block;
+    sc.tsh = block_tsh(T);
     hipLaunchKernelGGL((k_apply_block<ST, RM, LZ, NW>), dim3(c.nb), dim3(64 * NW), c.lds, s, T->tiles, T->hoff,
                        T->lrank, T->tl, T->th, T->nth, T->lolist, T->brk, T->dg, sc, X, Y, ld, c0, nc, k, P, nrm, ld_sc, partials,
                        npart, c.cap, (int)table_bytes(T));

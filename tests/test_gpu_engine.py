@@ -327,17 +327,19 @@ def test_block_apply_long_range_vs_oracle():
 
 @pytest.mark.parametrize("name", ["heis12", "tj2x4", "longrange16"])
 def test_schedules_give_identical_results(name):
-    """The launch tuning only changes which wave handles which tile: single
-    and block applies are bitwise the same under every schedule and
-    workgroup width."""
+    """The launch tuning only changes which wave handles which rows: single
+    and block applies are bitwise the same under every schedule, workgroup
+    width and block work-item width."""
     basis, site, terms = BLOCK_MODELS[name]()
     H = qa.MatrixFreeHamiltonian(basis, site, qa.OpSum(terms))
     D = H.dimension
     Xh = np.random.default_rng(13).uniform(-1, 1, (D, 7))
     X = qa.DeviceArray.from_host(Xh)
     out = []
-    for sched, waves in ((0, 4), (1, 4), (0, 8), (1, 16)):
-        H.set_tuning(schedule=sched, block_schedule=sched, apply_waves=waves)
+    for sched, waves, rows in ((0, 4, 64), (1, 4, 32), (0, 8, 16), (1, 16, 64), (2, 8, 16), (2, 16, 32), (2, 4, 64)):
+        H.set_tuning(schedule=sched, block_schedule=sched, apply_waves=waves, block_waves=waves, block_rows=rows)
+        assert H.tuning() == dict(schedule=sched, block_schedule=sched, apply_waves=waves, block_waves=waves,
+                                  block_rows=rows)
         Y, y = qa.DeviceArray(D * 7), qa.DeviceArray(D)
         x1 = qa.DeviceArray.from_host(Xh[:, 0])
         H.apply_device(X.ptr, Y.ptr, 7)
@@ -346,9 +348,11 @@ def test_schedules_give_identical_results(name):
     for o in out[1:]:
         assert np.array_equal(out[0][0], o[0]) and np.array_equal(out[0][1], o[1])
     with pytest.raises(RuntimeError):
-        H.set_tuning(schedule=2)
+        H.set_tuning(schedule=3)
     with pytest.raises(RuntimeError):
         H.set_tuning(apply_waves=5)
+    with pytest.raises(RuntimeError):
+        H.set_tuning(block_rows=8)
 
 
 def test_autotune_keeps_results():
