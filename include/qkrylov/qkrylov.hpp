@@ -8,9 +8,15 @@
 //   MatrixFreeHamiltonian        include/qkrylov/hamiltonian/matrix_free_hamiltonian.hpp:24-77
 //   lanczos_ground_state, davidson_lowest, continued_fraction_coeffs,
 //   evaluate_spectral_function, ftlm   include/qkrylov/solvers/*.hpp
-// Differences: no Kokkos template parameter (the device is always HIP); the
-// Kokkos View overload of apply() becomes device-pointer overloads; batched
-// entry points (ftlm_batched, continued_fraction_batched) are additions.
+// The headers at the reference's paths (qkrylov/basis/spinhalf_basis.hpp,
+// qkrylov/solvers/lanczos.hpp, ...) include this file, so code written
+// against the reference compiles unchanged. Differences: the execution-space
+// template parameter of MatrixFreeHamiltonian and the solvers is kept for
+// that spelling but every space runs the one HIP engine; the Kokkos View
+// overload of apply() becomes device-pointer overloads; the names exist in
+// FP64 only (qkrylov::fp64; QKRYLOV_PRECISION_NAMESPACE is fp64, as in the
+// reference's test builds, tests/CMakeLists.txt:24); batched entry points
+// (ftlm_batched, continued_fraction_batched) are additions.
 #pragma once
 
 #include <complex>
@@ -19,10 +25,33 @@
 #include <string>
 #include <vector>
 
+#ifndef QKRYLOV_PRECISION_NAMESPACE
+#define QKRYLOV_PRECISION_NAMESPACE fp64
+#endif
+
 namespace qk {
 class BasisSpec;
 class Engine;
 }
+
+namespace qkrylov {
+
+// Execution-space tag of the engine: HIP on the current device.
+struct HipSpace {};
+
+}  // namespace qkrylov
+
+// Code written against the reference names its execution space with Kokkos
+// types (MatrixFreeHamiltonian<Kokkos::DefaultExecutionSpace>,
+// tests/test_workflow.cpp:36). Without Kokkos in the translation unit those
+// names denote the engine's tag; with Kokkos included first, its own types
+// are accepted as the (unused) template argument.
+#ifndef KOKKOS_VERSION
+namespace Kokkos {
+using DefaultExecutionSpace = qkrylov::HipSpace;
+using DefaultHostExecutionSpace = qkrylov::HipSpace;
+}  // namespace Kokkos
+#endif
 
 namespace qkrylov {
 namespace fp64 {
@@ -245,12 +274,14 @@ inline LocalOp Bdag(int i) { return {"Bdag", i}; }
 inline LocalOp B(int i) { return {"B", i}; }
 inline LocalOp N(int i) { return {"N", i}; }
 
-class MatrixFreeHamiltonian {
+// The engine-backed operator; MatrixFreeHamiltonian<ExecSpace> below is the
+// reference's spelling of it.
+class MatrixFreeHamiltonianBase {
 public:
     // split_bits: number of low state bits ranked per lane (-1: automatic)
-    MatrixFreeHamiltonian(std::shared_ptr<Basis> basis, std::shared_ptr<Site> site, const OpSum& ops,
-                          Device device = Device(), int split_bits = -1);
-    ~MatrixFreeHamiltonian();
+    MatrixFreeHamiltonianBase(std::shared_ptr<Basis> basis, std::shared_ptr<Site> site, const OpSum& ops,
+                              Device device = Device(), int split_bits = -1);
+    virtual ~MatrixFreeHamiltonianBase();
 
     // host pointers: y = H x as the reference (y[a] = sum conj(amp) x[b])
     void apply(const Complex* x, Complex* y) const;
@@ -285,6 +316,18 @@ private:
     std::shared_ptr<const qk::Engine> engine_;
 };
 
+// matrix_free_hamiltonian.hpp:24-25 of the reference: a class template over
+// the execution space. Every space is served by the same engine.
+template <class ExecSpace = HipSpace>
+class MatrixFreeHamiltonian : public MatrixFreeHamiltonianBase {
+public:
+    MatrixFreeHamiltonian(std::shared_ptr<Basis> basis, std::shared_ptr<Site> site, const OpSum& ops,
+                          Device device = Device(), int split_bits = -1)
+        : MatrixFreeHamiltonianBase(std::move(basis), std::move(site), ops, device, split_bits)
+    {
+    }
+};
+
 struct LanczosResult {
     Real energy = 0.0;
     int iterations = 0;
@@ -300,8 +343,8 @@ struct LanczosOptions {
     uint64_t seed = 1234;
 };
 
-LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian& H, int maxiter = 200, Real tol = 1.0e-12);
-LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian& H, const LanczosOptions& opt);
+LanczosResult lanczos_ground_state(const MatrixFreeHamiltonianBase& H, int maxiter = 200, Real tol = 1.0e-12);
+LanczosResult lanczos_ground_state(const MatrixFreeHamiltonianBase& H, const LanczosOptions& opt);
 
 struct DavidsonResult {
     std::vector<Real> eigenvalues;
@@ -309,7 +352,7 @@ struct DavidsonResult {
     int iterations = 0;
     bool converged = false;
 };
-DavidsonResult davidson_lowest(const MatrixFreeHamiltonian& H, int n_eig = 1, int max_subspace = 20,
+DavidsonResult davidson_lowest(const MatrixFreeHamiltonianBase& H, int n_eig = 1, int max_subspace = 20,
                                Real tol = 1.0e-8);
 
 struct DynamicsResult {
@@ -317,7 +360,7 @@ struct DynamicsResult {
     std::vector<Real> betas;
     Real norm_phi0 = 0.0;
 };
-DynamicsResult continued_fraction_coeffs(const MatrixFreeHamiltonian& H, const HostVector& phi0, int n_iter = 100);
+DynamicsResult continued_fraction_coeffs(const MatrixFreeHamiltonianBase& H, const HostVector& phi0, int n_iter = 100);
 Real evaluate_spectral_function(const Real* alphas, const Real* betas, size_t n, Real norm_phi0, Real omega, Real E0,
                                 Real eta = 0.1);
 
@@ -327,7 +370,7 @@ struct FTLMResult {
     Real internal_energy = 0.0;
     Real specific_heat = 0.0;
 };
-FTLMResult ftlm(const MatrixFreeHamiltonian& H, Real beta, int n_random = 50, int n_steps = 100);
+FTLMResult ftlm(const MatrixFreeHamiltonianBase& H, Real beta, int n_random = 50, int n_steps = 100);
 
 // ---- additions: batched solves (one multi-vector recurrence for R vectors)
 struct FTLMGridResult {
@@ -337,13 +380,39 @@ struct FTLMGridResult {
 };
 // R real Rademacher (+-1) vectors from the counter RNG (seed, column, row);
 // sequential = true runs them one at a time through the plain path.
-FTLMGridResult ftlm_batched(const MatrixFreeHamiltonian& H, int R, int M, uint64_t seed,
+FTLMGridResult ftlm_batched(const MatrixFreeHamiltonianBase& H, int R, int M, uint64_t seed,
                             const std::vector<Real>& betas, bool sequential = false);
 
 // continued fractions for several seeds at once; seeds are device arrays,
 // complex seeds occupy two real columns (group = 2)
-std::vector<DynamicsResult> continued_fraction_batched(const MatrixFreeHamiltonian& H, const std::vector<HostVector>& seeds,
+std::vector<DynamicsResult> continued_fraction_batched(const MatrixFreeHamiltonianBase& H, const std::vector<HostVector>& seeds,
                                                        int n_iter, double* seconds = nullptr);
+
+// The reference's solver templates (solvers/lanczos.hpp:20-24,
+// davidson.hpp:21-27, dynamics.hpp:21-26, ftlm.hpp:21-27) with their
+// defaults, forwarding to the engine-backed functions above.
+template <class ExecSpace>
+LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian<ExecSpace>& H, int maxiter = 200, Real tol = 1.0e-12)
+{
+    return lanczos_ground_state(static_cast<const MatrixFreeHamiltonianBase&>(H), maxiter, tol);
+}
+template <class ExecSpace>
+DavidsonResult davidson_lowest(const MatrixFreeHamiltonian<ExecSpace>& H, int n_eig = 1, int max_subspace = 20,
+                               Real tol = 1.0e-8)
+{
+    return davidson_lowest(static_cast<const MatrixFreeHamiltonianBase&>(H), n_eig, max_subspace, tol);
+}
+template <class ExecSpace>
+DynamicsResult continued_fraction_coeffs(const MatrixFreeHamiltonian<ExecSpace>& H, const HostVector& phi0,
+                                         int n_iter = 100)
+{
+    return continued_fraction_coeffs(static_cast<const MatrixFreeHamiltonianBase&>(H), phi0, n_iter);
+}
+template <class ExecSpace>
+FTLMResult ftlm(const MatrixFreeHamiltonian<ExecSpace>& H, Real beta, int n_random = 50, int n_steps = 100)
+{
+    return ftlm(static_cast<const MatrixFreeHamiltonianBase&>(H), beta, n_random, n_steps);
+}
 
 } // namespace fp64
 } // namespace qkrylov

@@ -137,7 +137,7 @@ static qk::OpSum to_qk(const OpSum& ops)
     return q;
 }
 
-MatrixFreeHamiltonian::MatrixFreeHamiltonian(std::shared_ptr<Basis> basis, std::shared_ptr<Site> site,
+MatrixFreeHamiltonianBase::MatrixFreeHamiltonianBase(std::shared_ptr<Basis> basis, std::shared_ptr<Site> site,
                                              const OpSum& ops, Device device, int split_bits)
     : device_(device), basis_(std::move(basis)), site_(std::move(site)), ops_(ops)
 {
@@ -148,9 +148,9 @@ MatrixFreeHamiltonian::MatrixFreeHamiltonian(std::shared_ptr<Basis> basis, std::
     engine_ = std::make_shared<qk::Engine>(basis_->spec(), static_cast<qk::Kind>(site_->kind()), to_qk(ops_), eo);
 }
 
-MatrixFreeHamiltonian::~MatrixFreeHamiltonian() = default;
+MatrixFreeHamiltonianBase::~MatrixFreeHamiltonianBase() = default;
 
-MatrixFreeHamiltonian::TuneReport MatrixFreeHamiltonian::autotune(int nvec, bool allow_split)
+MatrixFreeHamiltonianBase::TuneReport MatrixFreeHamiltonianBase::autotune(int nvec, bool allow_split)
 {
     qk::TuneResult r;
     engine_ = qk::autotune(basis_->spec(), static_cast<qk::Kind>(site_->kind()), to_qk(ops_), engine_, nvec,
@@ -158,10 +158,10 @@ MatrixFreeHamiltonian::TuneReport MatrixFreeHamiltonian::autotune(int nvec, bool
     return TuneReport{r.NL, r.apply_waves, r.sched, r.blk_waves, r.blk_sched, r.tried, r.ms_before, r.ms, r.blk_rows};
 }
 
-Index MatrixFreeHamiltonian::dimension() const { return static_cast<Index>(engine_->dim()); }
-bool MatrixFreeHamiltonian::is_real() const { return engine_->is_real(); }
+Index MatrixFreeHamiltonianBase::dimension() const { return static_cast<Index>(engine_->dim()); }
+bool MatrixFreeHamiltonianBase::is_real() const { return engine_->is_real(); }
 
-void MatrixFreeHamiltonian::apply(const Complex* x, Complex* y) const
+void MatrixFreeHamiltonianBase::apply(const Complex* x, Complex* y) const
 {
     const size_t D = dimension();
     if (D == 0) return;
@@ -173,20 +173,20 @@ void MatrixFreeHamiltonian::apply(const Complex* x, Complex* y) const
     hip_check(hipStreamSynchronize(s), "sync");
 }
 
-void MatrixFreeHamiltonian::apply_device(const double* x, double* y, int nvec, void* stream) const
+void MatrixFreeHamiltonianBase::apply_device(const double* x, double* y, int nvec, void* stream) const
 {
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : qk::default_stream();
     if (nvec == 1) engine_->apply(x, y, false, s);
     else engine_->apply_block(x, y, nvec, nvec, s);
 }
 
-void MatrixFreeHamiltonian::apply_device_complex(const double* x, double* y, void* stream) const
+void MatrixFreeHamiltonianBase::apply_device_complex(const double* x, double* y, void* stream) const
 {
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : qk::default_stream();
     engine_->apply(x, y, true, s);
 }
 
-HostVector MatrixFreeHamiltonian::diagonal_host() const
+HostVector MatrixFreeHamiltonianBase::diagonal_host() const
 {
     const size_t D = dimension();
     HostVector out(D);
@@ -200,7 +200,7 @@ HostVector MatrixFreeHamiltonian::diagonal_host() const
 }
 
 // ---------------------------------------------------------------- solvers
-LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian& H, const LanczosOptions& opt)
+LanczosResult lanczos_ground_state(const MatrixFreeHamiltonianBase& H, const LanczosOptions& opt)
 {
     qk::LanczosOptions o;
     o.maxiter = opt.maxiter;
@@ -225,7 +225,7 @@ LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian& H, const Lanczos
     return res;
 }
 
-LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian& H, int maxiter, Real tol)
+LanczosResult lanczos_ground_state(const MatrixFreeHamiltonianBase& H, int maxiter, Real tol)
 {
     LanczosOptions o;
     o.maxiter = maxiter;
@@ -233,7 +233,7 @@ LanczosResult lanczos_ground_state(const MatrixFreeHamiltonian& H, int maxiter, 
     return lanczos_ground_state(H, o);
 }
 
-DavidsonResult davidson_lowest(const MatrixFreeHamiltonian& H, int n_eig, int max_subspace, Real tol)
+DavidsonResult davidson_lowest(const MatrixFreeHamiltonianBase& H, int n_eig, int max_subspace, Real tol)
 {
     auto r = qk::davidson_run(*H.engine(), n_eig, max_subspace, tol, true, qk::default_stream());
     DavidsonResult out;
@@ -248,7 +248,7 @@ DavidsonResult davidson_lowest(const MatrixFreeHamiltonian& H, int n_eig, int ma
     return out;
 }
 
-DynamicsResult continued_fraction_coeffs(const MatrixFreeHamiltonian& H, const HostVector& phi0, int n_iter)
+DynamicsResult continued_fraction_coeffs(const MatrixFreeHamiltonianBase& H, const HostVector& phi0, int n_iter)
 {
     DynamicsResult out;
     if (H.dimension() == 0) return out;
@@ -267,7 +267,7 @@ Real evaluate_spectral_function(const Real* alphas, const Real* betas, size_t n,
     return qk::spectral_value(alphas, betas, n, norm_phi0, omega, E0, eta);
 }
 
-FTLMResult ftlm(const MatrixFreeHamiltonian& H, Real beta, int n_random, int n_steps)
+FTLMResult ftlm(const MatrixFreeHamiltonianBase& H, Real beta, int n_random, int n_steps)
 {
     FTLMResult res;
     res.beta = beta;
@@ -323,7 +323,7 @@ FTLMResult ftlm(const MatrixFreeHamiltonian& H, Real beta, int n_random, int n_s
     return res;
 }
 
-FTLMGridResult ftlm_batched(const MatrixFreeHamiltonian& H, int R, int M, uint64_t seed, const std::vector<Real>& bT,
+FTLMGridResult ftlm_batched(const MatrixFreeHamiltonianBase& H, int R, int M, uint64_t seed, const std::vector<Real>& bT,
                             bool sequential)
 {
     FTLMGridResult out;
@@ -369,7 +369,7 @@ FTLMGridResult ftlm_batched(const MatrixFreeHamiltonian& H, int R, int M, uint64
     return out;
 }
 
-std::vector<DynamicsResult> continued_fraction_batched(const MatrixFreeHamiltonian& H,
+std::vector<DynamicsResult> continued_fraction_batched(const MatrixFreeHamiltonianBase& H,
                                                        const std::vector<HostVector>& seeds, int n_iter,
                                                        double* seconds)
 {

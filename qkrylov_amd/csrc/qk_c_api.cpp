@@ -31,7 +31,7 @@ struct qkrylov_opsum_t {
     OpSum opsum;
 };
 struct qkrylov_hamiltonian_t {
-    std::unique_ptr<MatrixFreeHamiltonian> ptr;
+    std::unique_ptr<MatrixFreeHamiltonianBase> ptr;
 };
 
 namespace {
@@ -94,7 +94,7 @@ OpSum adjoint(const OpSum& o)
     return a;
 }
 
-const MatrixFreeHamiltonian* H_of(qkrylov_hamiltonian_h h) { return (h && h->ptr) ? h->ptr.get() : nullptr; }
+const MatrixFreeHamiltonianBase* H_of(qkrylov_hamiltonian_h h) { return (h && h->ptr) ? h->ptr.get() : nullptr; }
 
 } // namespace
 
@@ -283,7 +283,7 @@ qkrylov_hamiltonian_h qkrylov_hamiltonian_create(qkrylov_basis_h b, qkrylov_site
     if (!b || !b->ptr || !s || !s->ptr || !o) return nullptr;
     return make_handle<qkrylov_hamiltonian_t>([&] {
         auto h = std::make_unique<qkrylov_hamiltonian_t>();
-        h->ptr = std::make_unique<MatrixFreeHamiltonian>(b->ptr, s->ptr, o->opsum, Device());
+        h->ptr = std::make_unique<MatrixFreeHamiltonianBase>(b->ptr, s->ptr, o->opsum, Device());
         return h.release();
     });
 }
@@ -615,7 +615,7 @@ qkrylov_hamiltonian_h qkrylov_hamiltonian_create_ex(qkrylov_basis_h b, qkrylov_s
     // an explicit split goes through the engine directly
     return make_handle<qkrylov_hamiltonian_t>([&] {
         auto h = std::make_unique<qkrylov_hamiltonian_t>();
-        h->ptr = std::make_unique<MatrixFreeHamiltonian>(b->ptr, s->ptr, o->opsum, Device(), NL);
+        h->ptr = std::make_unique<MatrixFreeHamiltonianBase>(b->ptr, s->ptr, o->opsum, Device(), NL);
         return h.release();
     });
 }
@@ -625,7 +625,7 @@ qkrylov_hamiltonian_h qkrylov_operator_create(qkrylov_basis_h b, qkrylov_site_h 
     if (!b || !b->ptr || !s || !s->ptr || !o) return nullptr;
     return make_handle<qkrylov_hamiltonian_t>([&] {
         auto h = std::make_unique<qkrylov_hamiltonian_t>();
-        h->ptr = std::make_unique<MatrixFreeHamiltonian>(b->ptr, s->ptr, adjoint(o->opsum), Device());
+        h->ptr = std::make_unique<MatrixFreeHamiltonianBase>(b->ptr, s->ptr, adjoint(o->opsum), Device());
         return h.release();
     });
 }
