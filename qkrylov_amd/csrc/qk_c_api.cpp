@@ -551,6 +551,15 @@ int qkrylov_ftlm(qkrylov_hamiltonian_h h, float beta, int n_random, int n_steps,
 // ---------------------------------------------------------------- extensions
 const char* qkrylov_amd_last_error(void) { return g_last_error.c_str(); }
 
+} // extern "C"
+
+namespace qk {
+void set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
+} // namespace qk
+
+extern "C" {
+
+
 int qkrylov_amd_device_count(void) { return Device::gpu_count(); }
 int qkrylov_amd_set_device(int id)
 {

@@ -98,6 +98,9 @@ private:
 
 double now_s();
 
+// message returned by qkrylov_amd_last_error() on this thread
+void set_last_error(const char* msg);
+
 // Autotuner: times y = H x (nvec columns) on random vectors under candidate
 // launch tunings and, when allow_split, lo-bit splits around the current
 // one; returns the fastest engine (the current one when nothing beats it).

@@ -247,9 +247,14 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         const int max_active = (opt && opt->max_active > 0) ? opt->max_active : 32;
         const bool realtime = opt ? opt->realtime != 0 : true;
         for (int i = 0; i < nreq; ++i) {
-            if (reqs[i].L < 2 || reqs[i].L > 32 || reqs[i].maxiter <= 0 || reqs[i].cf_steps < 0)
+            if (reqs[i].L < 2 || reqs[i].L > 32 || reqs[i].maxiter <= 0 || reqs[i].cf_steps < 0) {
+                set_last_error("serve: request with L outside 2..32, maxiter <= 0 or cf_steps < 0");
                 return QKRYLOV_ERROR_INVALID_ARG;
-            if (footprint_of(reqs[i]) > budget) return QKRYLOV_ERROR_INVALID_ARG;   // could never be admitted
+            }
+            if (footprint_of(reqs[i]) > budget) {   // could never be admitted
+                set_last_error("serve: a request's device footprint exceeds the memory budget");
+                return QKRYLOV_ERROR_INVALID_ARG;
+            }
             std::memset(&results[i], 0, sizeof(results[i]));
         }
         // keep every basis table alive for the whole run: no hipFree while
@@ -361,6 +366,7 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         }
         return QKRYLOV_SUCCESS;
     } catch (const std::exception& e) {
+        set_last_error(e.what());
         return QKRYLOV_ERROR_EXCEPTION;
     }
 }

@@ -103,3 +103,30 @@ def test_full_size_apply_sampled_rows(name):
             assert abs(yr[j] - ref) <= 1e-14 * scale
     x.free()
     y.free()
+
+
+SYM_CONFIGS = dict(CONFIGS, heis_ring32=(O.SPIN, 32, dict(use_sz=1, sz2=0), lambda: models.heisenberg_chain(32),
+                                          lambda: qa.SpinHalfBasis(32, sz2=0), 601080390))
+
+
+@pytest.mark.parametrize("name", sorted(SYM_CONFIGS))
+def test_full_size_apply_is_symmetric(name):
+    """<y, H x> = <H y, x> at full size for two counter-RNG vectors: every
+    Hamiltonian here is real symmetric, so a wrong target index, sign or
+    amplitude anywhere in the basis breaks the equality (relative 1e-12)."""
+    kind, n, sec, mk_terms, mk_basis, D = SYM_CONFIGS[name]
+    H = qa.MatrixFreeHamiltonian(mk_basis(), None, qa.OpSum(mk_terms()))
+    assert H.dimension == D and H.is_real
+    L = _lib.load()
+    host = {}
+    for seed in (91, 92):
+        x, hx = qa.DeviceArray(D), qa.DeviceArray(D)
+        _lib.check(L.qkrylov_amd_fill_random(x.ptr, D, seed, 0), "fill")
+        H.apply_device(x.ptr, hx.ptr)
+        host[seed] = (x.to_host(), hx.to_host())
+        x.free()
+        hx.free()
+    (x, hx), (y, hy) = host[91], host[92]
+    a, b = float(np.dot(y, hx)), float(np.dot(hy, x))
+    scale = float(np.linalg.norm(y) * np.linalg.norm(hx))
+    assert abs(a - b) <= 1e-12 * scale, (a, b, scale)

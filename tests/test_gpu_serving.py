@@ -59,3 +59,13 @@ def test_admission_respects_budget():
     spans = sorted((o["start"], o["finish"]) for o in out)
     for (s0, f0), (s1, f1) in zip(spans, spans[1:]):
         assert s1 >= f0 - 1e-9            # never two at once
+
+
+def test_invalid_requests_are_refused_with_a_message():
+    reqs = serving.make_trace(2, rate=1000.0, L_range=(12, 12), cf_fraction=0.0, maxiter=50, seed=3)
+    bad = [dict(r) for r in reqs]
+    bad[0]["L"] = 40
+    with pytest.raises(RuntimeError, match="L outside"):
+        serving.run(bad, realtime=False)
+    with pytest.raises(RuntimeError, match="footprint exceeds"):
+        serving.run(reqs, realtime=False, hbm_budget_bytes=1024)
