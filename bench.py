@@ -223,6 +223,24 @@ def bench_serve(args):
     print(json.dumps(res), flush=True)
 
 
+def stream_copy_gbs(qa, L, gib=2.0, reps=5):
+    """Measured STREAM-copy rate of this GPU: k_copy of `gib` GiB (read +
+    write counted), synchronous calls timed on the host, best of `reps`."""
+    from qkrylov_amd import _lib
+    n = int(gib * (1 << 30) / 8)
+    a, b = qa.DeviceArray(n), qa.DeviceArray(n)
+    _lib.check(L.qkrylov_amd_fill_random(a.ptr, n, 5, 0), "fill")
+    _lib.check(L.qkrylov_amd_copy(b.ptr, a.ptr, n), "copy")
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        _lib.check(L.qkrylov_amd_copy(b.ptr, a.ptr, n), "copy")
+        best = min(best, time.perf_counter() - t0)
+    a.free()
+    b.free()
+    return 2 * n * 8 / best / 1e9
+
+
 def config_flags(args):
     """Command-line flags that reproduce this run's split and launch tuning."""
     f = ["--split-bits", str(args.split_bits), "--nvec", str(args.nvec)]
@@ -260,8 +278,8 @@ def measure_traffic(args, flags):
             cmd = ["timeout", "-s", "KILL", "300", prof, "--pmc", ctr, "-d", out, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--workload", args.workload] + flags + [
                    "--steps", "2", "--warmup", "0", "--no-lanczos",
-                   "--no-cpu", "--no-traffic", "--copy-calib-gib", str(calib_gib)]
-            log(f"[bench] traffic pass {ctr}: {' '.join(cmd[4:8])} ...")
+                   "--no-cpu", "--no-traffic", "--no-stream", "--copy-calib-gib", str(calib_gib)]
+            log(f"[bench] traffic pass {ctr}: {' '.join(cmd[3:])}")
             r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
                                text=True)
             db = os.path.join(out, "run_results.db")
@@ -305,6 +323,7 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="skip the autotuner (which times the candidate launch tunings and lo-bit splits before the "
                          "timed region and keeps the fastest)")
+    ap.add_argument("--no-stream", action="store_true", help="skip the STREAM-copy measurement")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per apply")
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
@@ -403,6 +422,12 @@ def main():
                      "traffic": None if traffic is None else int(traffic),
                      "algorithmic_bytes_per_launch": int(alg_bytes)},
     }
+
+    if not args.no_stream:
+        copy_gbs = stream_copy_gbs(qa, L)
+        out["roofline"]["stream_copy_GBs"] = round(copy_gbs, 1)
+        out["roofline"]["frac_of_stream_copy"] = round(gbs / copy_gbs, 5)
+        log(f"[bench] stream copy: {copy_gbs:.1f} GB/s (read + write of 2 GiB, same box)")
 
     if traffic is not None:
         out["roofline"]["traffic_detail"] = {

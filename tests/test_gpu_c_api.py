@@ -131,3 +131,27 @@ def test_fp32_abi_error_codes():
     L.qkrylov_site_destroy(site)
     L.qkrylov_basis_destroy(basis)
     L.qkrylov_sector_destroy(sec)
+
+
+def test_amd_copy_exact_for_any_length_and_alignment():
+    """qkrylov_amd_copy (the STREAM-copy and counter-calibration kernel)
+    copies bit for bit: odd lengths, and pointers off 16-byte alignment."""
+    L = _lib.load()
+    n = 100003
+    src = np.random.default_rng(4).standard_normal(n + 2)
+    a = L.qkrylov_amd_malloc((n + 2) * 8)
+    b = L.qkrylov_amd_malloc((n + 2) * 8)
+    assert a and b
+    try:
+        assert L.qkrylov_amd_memcpy_h2d(a, src.ctypes.data, (n + 2) * 8) == 0
+        for off_s, off_d, m in ((0, 0, n), (8, 0, n), (0, 8, n - 1), (8, 8, 1), (0, 0, 0)):
+            out = np.full(n + 2, np.nan)
+            assert L.qkrylov_amd_memcpy_h2d(b, out.ctypes.data, (n + 2) * 8) == 0
+            assert L.qkrylov_amd_copy(ctypes.c_void_p(b + off_d), ctypes.c_void_p(a + off_s), m) == 0
+            assert L.qkrylov_amd_memcpy_d2h(out.ctypes.data, b, (n + 2) * 8) == 0
+            got = out[off_d // 8: off_d // 8 + m]
+            assert np.array_equal(got, src[off_s // 8: off_s // 8 + m])
+            assert np.isnan(out[off_d // 8 + m:]).all()
+    finally:
+        L.qkrylov_amd_free(a)
+        L.qkrylov_amd_free(b)
