@@ -647,17 +647,17 @@ void RealCore::steps_graph(int k, int n)
         cap_ic = ic;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
         int rc = 0;
+        int* kd = static_cast<int*>(kdev.get());
         for (int i = 0; i < 3 && rc == 0; ++i) {
-            int* kd = static_cast<int*>(kdev.get());
-            rc |= qkl_apply_dev(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), kd, prv(), nrm.d(), partA.d(), s);
-            rc |= qkl_reduce_dev(partA.d(), npA, 0, alpha.d(), kd, 0, 0, s);
-            rc |= qkl_update_dev(D, wrk(), cur(), alpha.d(), nrm.d(), kd, partB.d(), s);
-            rc |= qkl_reduce_dev(partB.d(), NB, 1, nrm.d(), kd, 1, 1, s);
+            rc |= qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), 0, kd, i, prv(), nrm.d(), partB.d(), NB,
+                                    partA.d(), s);
+            rc |= qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), 0, kd, i, partA.d(), npA, partB.d(), s);
             const int t = ip;
             ip = ic;
             ic = iw;
             iw = t;
         }
+        rc |= qkl_add_int(kd, 3, s);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &g);
         if (rc != 0 || e != hipSuccess) {
@@ -672,24 +672,36 @@ void RealCore::steps_graph(int k, int n)
     // roles as they were
     lcheck(qkl_set_int(static_cast<int*>(kdev.get()), k, s), "set step");
     for (int r = 0; r < n / 3; ++r) hip_check(hipGraphLaunch(gexec, s), "graph launch");
+    pending = k + n;
 }
 
 void RealCore::init_norm()
 {
     lcheck(qkl_fill_zero(D, prv(), s), "zero");
     lcheck(qkl_dot_partials(0, D, cur(), cur(), partB.d(), s), "dot");
-    lcheck(qkl_reduce(partB.d(), NB, 1, 1, 1, 0, nrm.d(), 0, 1, s), "reduce");
+    lcheck(qkl_total(partB.d(), NB, 1, nrm.d(), 0, s), "total");
+    pending = -1;
 }
 
-// one step k: w = H v_k - beta v_{k-1}; alpha_k; q_{k+1}; nrm[k+1]
+void RealCore::flush()
+{
+    if (pending >= 0) lcheck(qkl_total(partB.d(), NB, 1, nrm.d(), pending, s), "total");
+    pending = -1;
+}
+
+// one step k: nrm[k] from the previous partials; w = H v_k - beta v_{k-1};
+// alpha_k; q_{k+1} and the partial sums of |q_{k+1}|^2 (nrm[k+1] is totalled
+// by the next step's apply, or by flush())
 void RealCore::step(int k, const double* basis)
 {
     // the partial buffer was sized for the launch shape of the tuning at
     // construction; a retuned engine would write past it
     if (qkl_apply_partials(E.tables()) != npA) throw std::logic_error("apply launch shape changed during a run");
-    lcheck(qkl_apply(E.tables(), 0, E.wide() ? 1 : 0, cur(), wrk(), k, prv(), nrm.d(), partA.d(), s), "apply");
-    lcheck(qkl_reduce(partA.d(), npA, 1, 1, 0, 0, alpha.d(), k, 1, s), "reduce alpha");
-    lcheck(qkl_lanczos_update(0, D, 1, 1, wrk(), cur(), alpha.d(), nrm.d(), k, 1, partB.d(), s), "update");
+    lcheck(qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), k, nullptr, 0, prv(), nrm.d(), partB.d(), NB,
+                             partA.d(), s),
+           "apply");
+    lcheck(qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), k, nullptr, 0, partA.d(), npA, partB.d(), s),
+           "update");
     if (basis) {
         // classical Gram-Schmidt against v_0..v_k (stored normalised)
         lcheck(qkl_multidot(D, k + 1, basis, D, wrk(), partM.d(), s), "multidot");
@@ -697,7 +709,7 @@ void RealCore::step(int k, const double* basis)
         lcheck(qkl_multiaxpy(D, k + 1, basis, D, hcoef.d(), -1.0, wrk(), s), "multiaxpy");
         lcheck(qkl_dot_partials(0, D, wrk(), wrk(), partB.d(), s), "dot");
     }
-    lcheck(qkl_reduce(partB.d(), NB, 1, 1, 1, 0, nrm.d(), k + 1, 1, s), "reduce beta");
+    pending = k + 1;
     const int t = ip;
     ip = ic;
     ic = iw;
@@ -753,10 +765,13 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
         }
         for (; k < kend; ++k) {
             C.step(k, reorth ? basis.d() : nullptr);
-            if (store)
+            if (store) {
+                C.flush();
                 lcheck(qkl_copy_scaled((size_t)D, C.cur(), basis.d() + (size_t)(k + 1) * D, C.nrm.d(), k + 1, s),
                        "store v");
+            }
         }
+        C.flush();
         hip_check(hipMemcpyAsync(ha.data(), C.alpha.d(), k * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
         hip_check(hipMemcpyAsync(hn.data(), C.nrm.d(), (k + 1) * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
         hip_check(hipStreamSynchronize(s), "sync");
@@ -884,6 +899,7 @@ TridiagOutput tridiag_run_dev(const Engine& E, const double* start_dev, int n_st
     const double t0 = now_s();
     C.init_norm();
     for (int k = 0; k < n_steps; ++k) C.step(k, nullptr);
+    C.flush();
     std::vector<double> ha(n_steps), hn(n_steps + 1);
     hip_check(hipMemcpyAsync(ha.data(), C.alpha.d(), n_steps * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
     hip_check(hipMemcpyAsync(hn.data(), C.nrm.d(), (n_steps + 1) * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");

@@ -147,6 +147,7 @@ struct RealCore {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int cap_ic = -1;   // rotation state the graph was captured in
+    int pending = -1;  // nrm index whose partial sums are not yet totalled
 
     RealCore(const Engine& e, int kcap, hipStream_t st, bool reorth, bool stream_ordered = false);
     ~RealCore();
@@ -156,7 +157,11 @@ struct RealCore {
     double* prv() { return Q[ip].d(); }
     double* wrk() { return Q[iw].d(); }
     void init_norm();                        // nrm[0] = |cur|, prv = 0
+    // step k leaves alpha[k] in memory and the partial sums of nrm[k+1] in
+    // partB: the next step's apply totals them; flush() does it now (before
+    // nrm[k+1] is read on the host or by another kernel)
     void step(int k, const double* basis);   // basis: normalised v_0..v_k for reorthogonalisation
+    void flush();
     // steps k .. k + n - 1 (n a multiple of 3, k + n <= kcap) as replays of
     // the captured three-step graph; the same kernels in the same order as
     // step(), so the same numbers

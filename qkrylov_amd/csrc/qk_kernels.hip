@@ -232,6 +232,38 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Fixed-order total of n partial sums, by the first wave of the workgroup:
+// lane l adds p[l], p[l + 64], ... in index order, then the butterfly of
+// wave_sum, and lane 0's value is broadcast through *sh. Every workgroup
+// (and k_total) that totals the same partials gets the same bits. All
+// threads of the workgroup must call it.
+__device__ __forceinline__ double block_total(const double* __restrict__ p, int n, double* sh)
+{
+    if (threadIdx.x < 64) {
+        const int l = (int)threadIdx.x;
+        double s = 0.0;
+        if (n <= 64 * 32) {
+            // all loads in flight at once, then the adds in index order
+            // (absent terms add +0.0, which leaves the sum unchanged)
+            double v[32];
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                const int i = l + 64 * j;
+                v[j] = p[i < n ? i : 0];
+                v[j] = i < n ? v[j] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < 32; ++j) s += v[j];
+        } else {
+            for (int i = l; i < n; i += 64) s += p[i];
+        }
+        s = wave_sum(s);
+        if (l == 0) *sh = s;
+    }
+    __syncthreads();
+    return *sh;
+}
+
 // ------------------------------------------------------------------ apply
 // Per-wave LDS scratch of the apply kernel: for every branch the tile's
 // uniform data (target block offset; the two values lo & lcm may take for
@@ -294,12 +326,14 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
                                                    const QkBranchK* __restrict__ brk,
                                                    const QkDiagTerm* __restrict__ dg, QkScal sc,
                                                    const double* __restrict__ x, double* __restrict__ y, int lz_k,
-                                                   const double* __restrict__ prev, const double* __restrict__ nrm,
+                                                   const double* __restrict__ prev, double* __restrict__ nrm,
                                                    double* __restrict__ partials, int table_bytes,
-                                                   const int* __restrict__ kdev)
+                                                   const int* __restrict__ kdev, int koff,
+                                                   const double* __restrict__ nrm_part, int n_nrm_part)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double wsum[16];
+    __shared__ double tot_sh;
     const int nA = sc.nA, nB = sc.nB, nbk = sc.nA + sc.nB;
     uint4* lb4 = reinterpret_cast<uint4*>(smem + table_bytes);
     uint32_t* lbm = reinterpret_cast<uint32_t*>(lb4 + nB);
@@ -324,12 +358,22 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
     const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
 
     // Lanczos step: from the argument, or from the device step counter
-    // when the launch is replayed from a captured graph
-    const int lzk = (LZ && kdev) ? *kdev : lz_k;
+    // (plus koff) when the launch is replayed from a captured graph. With
+    // nrm_part, nrm[k] = |q_k| is totalled here from the previous update's
+    // partial sums (and stored by workgroup 0) instead of by a separate
+    // reduction launch.
+    const int lzk = (LZ && kdev) ? *kdev + koff : lz_k;
     double inv_n = 1.0, cprev = 0.0;
     if (LZ) {
-        inv_n = 1.0 / nrm[lzk];
-        if (lzk > 0) cprev = nrm[lzk] / nrm[lzk - 1];
+        double nk;
+        if (nrm_part) {
+            nk = sqrt(block_total(nrm_part, n_nrm_part, &tot_sh));
+            if (blockIdx.x == 0 && threadIdx.x == 0) nrm[lzk] = nk;
+        } else {
+            nk = nrm[lzk];
+        }
+        inv_n = 1.0 / nk;
+        if (lzk > 0) cprev = nk / nrm[lzk - 1];
     }
     double part = 0.0;
 
@@ -768,13 +812,24 @@ __global__ __launch_bounds__(64 * NW) void k_apply_block(
 // Single vector: thread-strided. Block: lane = column within a 64-wide row.
 template <bool CPLX>
 __global__ __launch_bounds__(QK_TPB) void k_update1(size_t n, double* __restrict__ w, const double* __restrict__ q,
-                                                     const double* __restrict__ alpha,
-                                                     const double* __restrict__ nrm, int k,
-                                                     double* __restrict__ partials, const int* __restrict__ kdev)
+                                                     double* __restrict__ alpha, const double* __restrict__ nrm,
+                                                     int k, double* __restrict__ partials,
+                                                     const int* __restrict__ kdev, int koff,
+                                                     const double* __restrict__ a_part, int n_a_part)
 {
     __shared__ double wsum[QK_WPB];
-    const int kk = kdev ? *kdev : k;
-    const double c = alpha[kk] / nrm[kk];
+    __shared__ double tot_sh;
+    const int kk = kdev ? *kdev + koff : k;
+    // with a_part, alpha[k] is totalled here from the apply's partial sums
+    // (workgroup 0 stores it) instead of by a separate reduction launch
+    double a;
+    if (a_part) {
+        a = block_total(a_part, n_a_part, &tot_sh);
+        if (blockIdx.x == 0 && threadIdx.x == 0) alpha[kk] = a;
+    } else {
+        a = alpha[kk];
+    }
+    const double c = a / nrm[kk];
     double part = 0.0;
     const size_t nel = CPLX ? 2 * n : n;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nel; i += (size_t)gridDim.x * blockDim.x) {
@@ -966,6 +1021,21 @@ __global__ __launch_bounds__(QK_TPB) void k_copy(size_t n, const double* __restr
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride)
         __builtin_nontemporal_store(__builtin_nontemporal_load(s2 + i), d2 + i);
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[n - 1] = src[n - 1];
+}
+
+// out[idx] = total of n partial sums (sqrt of it when root), in the order
+// of block_total
+__global__ __launch_bounds__(64) void k_total(const double* __restrict__ p, int n, int root, double* __restrict__ out,
+                                              int idx)
+{
+    __shared__ double tot_sh;
+    const double t = block_total(p, n, &tot_sh);
+    if (threadIdx.x == 0) out[idx] = root ? sqrt(t) : t;
+}
+
+__global__ void k_add_int(int* __restrict__ p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] += v;
 }
 
 __global__ void k_set_int(int* __restrict__ p, int v)
@@ -1264,16 +1334,25 @@ inline int grid_for(size_t n)
 
 inline int rc(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
+// Lanczos-form extras of the single-vector apply: the device step counter
+// (plus offset) of graph replays, and the partial sums that total to nrm[k]
+struct LzArgs {
+    const int* kdev = nullptr;
+    int koff = 0;
+    const double* nrm_part = nullptr;
+    int n_nrm_part = 0;
+};
+
 template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
-int launch_apply(const QkTables* T, const double* x, double* y, int k, const double* prev, const double* nrm,
-                 double* partials, hipStream_t s, const int* kdev = nullptr)
+int launch_apply(const QkTables* T, const double* x, double* y, int k, const double* prev, double* nrm,
+                 double* partials, hipStream_t s, const LzArgs& lz)
 {
     ApplyCfg c;
     if (!apply_config(T, c)) return -2;
     hipLaunchKernelGGL((k_apply<ST, CPLX, RM, LZ, SIGNS>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff,
                        T->lrank, T->tl, T->th, T->nth,
                        T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T),
-                       kdev);
+                       lz.kdev, lz.koff, lz.nrm_part, lz.n_nrm_part);
     return rc(hipGetLastError());
 }
 
@@ -1301,25 +1380,25 @@ int launch_block(const QkTables* T, const BlockCfg& c, const double* X, double* 
 }
 
 template <typename ST, bool C, bool LZ, bool SG>
-int apply_rm(const QkTables* T, const double* x, double* y, int k, const double* prev, const double* nrm,
-             double* partials, hipStream_t s, const int* kdev)
+int apply_rm(const QkTables* T, const double* x, double* y, int k, const double* prev, double* nrm,
+             double* partials, hipStream_t s, const LzArgs& lz)
 {
     switch (T->rank_mode) {
-    case 2: return launch_apply<ST, C, 2, LZ, SG>(T, x, y, k, prev, nrm, partials, s, kdev);
-    case 1: return launch_apply<ST, C, 1, LZ, SG>(T, x, y, k, prev, nrm, partials, s, kdev);
-    default: return launch_apply<ST, C, 0, LZ, SG>(T, x, y, k, prev, nrm, partials, s, kdev);
+    case 2: return launch_apply<ST, C, 2, LZ, SG>(T, x, y, k, prev, nrm, partials, s, lz);
+    case 1: return launch_apply<ST, C, 1, LZ, SG>(T, x, y, k, prev, nrm, partials, s, lz);
+    default: return launch_apply<ST, C, 0, LZ, SG>(T, x, y, k, prev, nrm, partials, s, lz);
     }
 }
 
 template <typename ST, bool C>
-int apply_any(const QkTables* T, bool lz, const double* x, double* y, int k, const double* prev, const double* nrm,
-              double* partials, hipStream_t s, const int* kdev)
+int apply_any(const QkTables* T, bool lzf, const double* x, double* y, int k, const double* prev, double* nrm,
+              double* partials, hipStream_t s, const LzArgs& lz)
 {
     const bool sg = T->signs != 0;
-    if (lz) return sg ? apply_rm<ST, C, true, true>(T, x, y, k, prev, nrm, partials, s, kdev)
-                      : apply_rm<ST, C, true, false>(T, x, y, k, prev, nrm, partials, s, kdev);
-    return sg ? apply_rm<ST, C, false, true>(T, x, y, k, prev, nrm, partials, s, kdev)
-              : apply_rm<ST, C, false, false>(T, x, y, k, prev, nrm, partials, s, kdev);
+    if (lzf) return sg ? apply_rm<ST, C, true, true>(T, x, y, k, prev, nrm, partials, s, lz)
+                       : apply_rm<ST, C, true, false>(T, x, y, k, prev, nrm, partials, s, lz);
+    return sg ? apply_rm<ST, C, false, true>(T, x, y, k, prev, nrm, partials, s, lz)
+              : apply_rm<ST, C, false, false>(T, x, y, k, prev, nrm, partials, s, lz);
 }
 
 template <typename ST, bool LZ>
@@ -1356,42 +1435,52 @@ int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y,
     if (T->ntiles == 0) return 0;
     const bool lz = lz_k >= 0;
     if (lz && cplx) return -1;
+    // without nrm_part the kernel only reads nrm
+    double* nw = const_cast<double*>(nrm);
+    const LzArgs none;
     if (wide) {
-        return cplx ? apply_any<uint64_t, true>(T, lz, x, y, lz_k, prev, nrm, partials, s, nullptr)
-                    : apply_any<uint64_t, false>(T, lz, x, y, lz_k, prev, nrm, partials, s, nullptr);
+        return cplx ? apply_any<uint64_t, true>(T, lz, x, y, lz_k, prev, nw, partials, s, none)
+                    : apply_any<uint64_t, false>(T, lz, x, y, lz_k, prev, nw, partials, s, none);
     }
-    return cplx ? apply_any<uint32_t, true>(T, lz, x, y, lz_k, prev, nrm, partials, s, nullptr)
-                : apply_any<uint32_t, false>(T, lz, x, y, lz_k, prev, nrm, partials, s, nullptr);
+    return cplx ? apply_any<uint32_t, true>(T, lz, x, y, lz_k, prev, nw, partials, s, none)
+                : apply_any<uint32_t, false>(T, lz, x, y, lz_k, prev, nw, partials, s, none);
 }
 
-// Real single-vector Lanczos step kernels reading the step k from a device
-// counter (for launches captured in a graph and replayed): the fused apply,
-// the update, and the reduction that writes out[*kdev + idx] (and, when
-// advance, moves the counter to the next step).
-int qkl_apply_dev(const QkTables* T, int wide, const double* x, double* y, const int* kdev, const double* prev,
-                  const double* nrm, double* partials, qkl_stream sv)
+// The two kernels of a real single-vector Lanczos step with the
+// reductions folded in (see qk_launch.h).
+int qkl_lanczos_apply(const QkTables* T, int wide, const double* x, double* y, int k, const int* kdev, int koff,
+                      const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
+                      qkl_stream sv)
 {
     hipStream_t s = (hipStream_t)sv;
-    if (T->ntiles == 0 || !kdev) return -1;
-    return wide ? apply_any<uint64_t, false>(T, true, x, y, 0, prev, nrm, partials, s, kdev)
-                : apply_any<uint32_t, false>(T, true, x, y, 0, prev, nrm, partials, s, kdev);
+    if (T->ntiles == 0 || !nrm_part || n_nrm_part <= 0) return -1;
+    LzArgs lz;
+    lz.kdev = kdev;
+    lz.koff = koff;
+    lz.nrm_part = nrm_part;
+    lz.n_nrm_part = n_nrm_part;
+    return wide ? apply_any<uint64_t, false>(T, true, x, y, k, prev, nrm, partials, s, lz)
+                : apply_any<uint32_t, false>(T, true, x, y, k, prev, nrm, partials, s, lz);
 }
 
-int qkl_update_dev(size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, const int* kdev,
-                   double* partials, qkl_stream sv)
+int qkl_lanczos_update1(size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
+                        const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, qkl_stream sv)
 {
-    if (!kdev) return -1;
+    if (!a_part || n_a_part <= 0) return -1;
     hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, (hipStream_t)sv, nrows, w, q, alpha,
-                       nrm, 0, partials, kdev);
+                       nrm, k, partials, kdev, koff, a_part, n_a_part);
     return rc(hipGetLastError());
 }
 
-int qkl_reduce_dev(const double* partials, int npart, int mode, double* out, int* kdev, int idx, int advance,
-                   qkl_stream sv)
+int qkl_total(const double* partials, int npart, int root, double* out, int idx, qkl_stream sv)
 {
-    if (!kdev) return -1;
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(QK_TPB), 0, (hipStream_t)sv, partials, npart, 1, mode, 0, out, idx, 1,
-                       kdev, advance);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, (hipStream_t)sv, partials, npart, root, out, idx);
+    return rc(hipGetLastError());
+}
+
+int qkl_add_int(int* p, int v, qkl_stream sv)
+{
+    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, (hipStream_t)sv, p, v);
     return rc(hipGetLastError());
 }
 
@@ -1416,12 +1505,14 @@ int qkl_lanczos_update(int cplx, size_t nrows, int ncol, int ld, double* w, cons
 {
     hipStream_t s = (hipStream_t)sv;
     if (ncol == 1 && ld == 1) {
+        // without a_part the kernel only reads alpha
+        double* aw = const_cast<double*>(alpha);
         if (cplx)
-            hipLaunchKernelGGL(k_update1<true>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, alpha, nrm, k,
-                               partials, (const int*)nullptr);
+            hipLaunchKernelGGL(k_update1<true>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, aw, nrm, k,
+                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0);
         else
-            hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, alpha, nrm, k,
-                               partials, (const int*)nullptr);
+            hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, aw, nrm, k,
+                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0);
     } else {
         if (cplx || ncol > 64) return -1;
         hipLaunchKernelGGL(k_update_block, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, ncol, ld, w, q, alpha, nrm,

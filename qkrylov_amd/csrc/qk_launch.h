@@ -34,17 +34,26 @@ int qkl_apply_block(const qk::QkTables* T, int wide, const double* X, double* Y,
                     qkl_stream s);
 int qkl_block_partials(const qk::QkTables* T);
 
-/* The real single-vector Lanczos step with the step k read from a device
- * counter *kdev (launches that are captured once in a graph and replayed):
- * fused apply, update, and a one-workgroup reduction into out[*kdev + idx]
- * that, when advance, moves the counter to the next step. */
-int qkl_apply_dev(const qk::QkTables* T, int wide, const double* x, double* y, const int* kdev, const double* prev,
-                  const double* nrm, double* partials, qkl_stream s);
-int qkl_update_dev(size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, const int* kdev,
-                   double* partials, qkl_stream s);
-int qkl_reduce_dev(const double* partials, int npart, int mode, double* out, int* kdev, int idx, int advance,
-                   qkl_stream s);
-
+/* The real single-vector Lanczos step as two launches, each totalling the
+ * partial sums the other wrote instead of a separate reduction launch:
+ *   qkl_lanczos_apply:   nrm[k] = sqrt(total of nrm_part) (the previous
+ *                        update's, or the start vector's, sums of squares;
+ *                        workgroup 0 stores it), then the fused apply of
+ *                        qkl_apply with Lanczos epilogue (partials of alpha);
+ *   qkl_lanczos_update1: alpha[k] = total of a_part (workgroup 0 stores it),
+ *                        then qkl_lanczos_update for one vector.
+ * Totals are fixed-order (qkl_total computes the same bits). k is taken
+ * from *kdev + koff when kdev is set (launches captured once in a graph and
+ * replayed), else from the argument. */
+int qkl_lanczos_apply(const qk::QkTables* T, int wide, const double* x, double* y, int k, const int* kdev, int koff,
+                      const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
+                      qkl_stream s);
+int qkl_lanczos_update1(size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
+                        const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, qkl_stream s);
+/* out[idx] = total of partials[0..npart) (sqrt when root), in the order of
+ * the two launches above; *p += v on the stream (one thread) */
+int qkl_total(const double* partials, int npart, int root, double* out, int idx, qkl_stream s);
+int qkl_add_int(int* p, int v, qkl_stream s);
 
 /* q_{k+1} = w - alpha[k] * q_k / nrm[k]; partials of |q_{k+1}|^2 (ncol columns
  * interleaved with stride ld; ncol = 1, ld = 1 for a single vector; cplx for

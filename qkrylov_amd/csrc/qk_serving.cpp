@@ -117,6 +117,7 @@ void enqueue_gs_chunk(Job& j)
         j.k += n3;
     }
     for (; j.k < kend; ++j.k) j.C->step(j.k, nullptr);
+    j.C->flush();
     copy_scalars(j, j.k);
     hip_check(hipEventRecord(j.ev, j.s), "event");
     j.inflight = true;
@@ -202,6 +203,7 @@ void advance(Job& j, qkrylov_serve_result_t& res, double* cf_a, double* cf_b, in
             if (n3 > 0) j.C->steps_graph(0, n3);
             for (int kk = n3; kk < j.rq.cf_steps; ++kk) j.C->step(kk, nullptr);
         }
+        j.C->flush();
         copy_scalars(j, j.rq.cf_steps);
         hip_check(hipMemcpyAsync(j.ha->p + j.rq.cf_steps + 1, j.psi2.d(), sizeof(double), hipMemcpyDeviceToHost, j.s),
                   "d2h");
