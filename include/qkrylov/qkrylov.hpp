@@ -13,10 +13,12 @@
 // against the reference compiles unchanged. Differences: the execution-space
 // template parameter of MatrixFreeHamiltonian and the solvers is kept for
 // that spelling but every space runs the one HIP engine; the Kokkos View
-// overload of apply() becomes device-pointer overloads; the names exist in
-// FP64 only (qkrylov::fp64; QKRYLOV_PRECISION_NAMESPACE is fp64, as in the
-// reference's test builds, tests/CMakeLists.txt:24); batched entry points
-// (ftlm_batched, continued_fraction_batched) are additions.
+// overload of apply() becomes device-pointer overloads; qkrylov::fp32
+// (qkrylov_fp32.hpp) has float scalars and vectors at its boundary over the
+// same FP64 engine; batched entry points (ftlm_batched,
+// continued_fraction_batched) are additions. QKRYLOV_PRECISION_NAMESPACE is
+// fp64 with QKRYLOV_DOUBLE_PRECISION defined and fp32 otherwise, as in the
+// reference (core/types.hpp:15-19).
 #pragma once
 
 #include <complex>
@@ -26,7 +28,11 @@
 #include <vector>
 
 #ifndef QKRYLOV_PRECISION_NAMESPACE
+#ifdef QKRYLOV_DOUBLE_PRECISION
 #define QKRYLOV_PRECISION_NAMESPACE fp64
+#else
+#define QKRYLOV_PRECISION_NAMESPACE fp32
+#endif
 #endif
 
 namespace qk {
@@ -416,3 +422,5 @@ FTLMResult ftlm(const MatrixFreeHamiltonian<ExecSpace>& H, Real beta, int n_rand
 
 } // namespace fp64
 } // namespace qkrylov
+
+#include "qkrylov/qkrylov_fp32.hpp"
