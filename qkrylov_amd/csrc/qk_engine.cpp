@@ -446,7 +446,7 @@ void Engine::set_tuning(int knob, int value) const
 void Engine::apply(const double* x, double* y, bool cplx, hipStream_t s) const
 {
     if (!cplx && !is_real()) throw std::invalid_argument("complex Hamiltonian needs complex vectors");
-    lcheck(qkl_apply(&T_, cplx ? 1 : 0, wide() ? 1 : 0, x, y, -1, nullptr, nullptr, nullptr, s), "apply");
+    lcheck(qkl_apply(&T_, cplx ? 1 : 0, wide() ? 1 : 0, x, y, s), "apply");
 }
 
 void Engine::apply_block(const double* X, double* Y, int ld, int ncols, hipStream_t s) const

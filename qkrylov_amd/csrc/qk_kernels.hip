@@ -358,20 +358,15 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
     const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
 
     // Lanczos step: from the argument, or from the device step counter
-    // (plus koff) when the launch is replayed from a captured graph. With
-    // nrm_part, nrm[k] = |q_k| is totalled here from the previous update's
-    // partial sums (and stored by workgroup 0) instead of by a separate
-    // reduction launch.
+    // (plus koff) when the launch is replayed from a captured graph;
+    // nrm[k] = |q_k| is totalled here from the previous update's partial
+    // sums (and stored by workgroup 0) instead of by a separate reduction
+    // launch.
     const int lzk = (LZ && kdev) ? *kdev + koff : lz_k;
     double inv_n = 1.0, cprev = 0.0;
     if (LZ) {
-        double nk;
-        if (nrm_part) {
-            nk = sqrt(block_total(nrm_part, n_nrm_part, &tot_sh));
-            if (blockIdx.x == 0 && threadIdx.x == 0) nrm[lzk] = nk;
-        } else {
-            nk = nrm[lzk];
-        }
+        const double nk = sqrt(block_total(nrm_part, n_nrm_part, &tot_sh));
+        if (blockIdx.x == 0 && threadIdx.x == 0) nrm[lzk] = nk;
         inv_n = 1.0 / nk;
         if (lzk > 0) cprev = nk / nrm[lzk - 1];
     }
@@ -880,14 +875,8 @@ __global__ __launch_bounds__(QK_TPB) void k_update_block(size_t nrows, int ncol,
 // One workgroup per result group. Fixed order: thread-strided sums, then a
 // fixed tree, then columns of the group added in index order.
 __global__ __launch_bounds__(QK_TPB) void k_reduce(const double* __restrict__ partials, int npart, int group,
-                                                    int mode, int cplx, double* __restrict__ out, int idx, int ld_sc,
-                                                    int* __restrict__ kdev, int advance)
+                                                    int mode, int cplx, double* __restrict__ out, int idx, int ld_sc)
 {
-    // with a device step counter the output index is counter + idx; the
-    // launch that advances the counter has one workgroup, and its thread 0
-    // writes the counter after every thread has read it
-    const int k0 = kdev ? *kdev : 0;
-    idx += k0;
     __shared__ double red[QK_TPB];
     __shared__ double red_i[QK_TPB];
     const int g = blockIdx.x;
@@ -927,7 +916,6 @@ __global__ __launch_bounds__(QK_TPB) void k_reduce(const double* __restrict__ pa
                 o[1] = tot_i;
             }
         }
-        if (advance && kdev) kdev[0] = k0 + 1;
     }
 }
 
@@ -1428,22 +1416,17 @@ int qkl_block_partials(const QkTables* T)
 }
 int qkl_blas_partials(void) { return QK_BLAS_BLOCKS; }
 
-int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y, int lz_k, const double* prev,
-              const double* nrm, double* partials, qkl_stream sv)
+int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y, qkl_stream sv)
 {
     hipStream_t s = (hipStream_t)sv;
     if (T->ntiles == 0) return 0;
-    const bool lz = lz_k >= 0;
-    if (lz && cplx) return -1;
-    // without nrm_part the kernel only reads nrm
-    double* nw = const_cast<double*>(nrm);
     const LzArgs none;
     if (wide) {
-        return cplx ? apply_any<uint64_t, true>(T, lz, x, y, lz_k, prev, nw, partials, s, none)
-                    : apply_any<uint64_t, false>(T, lz, x, y, lz_k, prev, nw, partials, s, none);
+        return cplx ? apply_any<uint64_t, true>(T, false, x, y, -1, nullptr, nullptr, nullptr, s, none)
+                    : apply_any<uint64_t, false>(T, false, x, y, -1, nullptr, nullptr, nullptr, s, none);
     }
-    return cplx ? apply_any<uint32_t, true>(T, lz, x, y, lz_k, prev, nw, partials, s, none)
-                : apply_any<uint32_t, false>(T, lz, x, y, lz_k, prev, nw, partials, s, none);
+    return cplx ? apply_any<uint32_t, true>(T, false, x, y, -1, nullptr, nullptr, nullptr, s, none)
+                : apply_any<uint32_t, false>(T, false, x, y, -1, nullptr, nullptr, nullptr, s, none);
 }
 
 // The two kernels of a real single-vector Lanczos step with the
@@ -1527,7 +1510,7 @@ int qkl_reduce(const double* partials, int npart, int ncol, int group, int mode,
     hipStream_t s = (hipStream_t)sv;
     if (group <= 0 || ncol % group) return -1;
     hipLaunchKernelGGL(k_reduce, dim3(ncol / group), dim3(QK_TPB), 0, s, partials, npart, group, mode, cplx, out, idx,
-                       ld_sc, (int*)nullptr, 0);
+                       ld_sc);
     return rc(hipGetLastError());
 }
 

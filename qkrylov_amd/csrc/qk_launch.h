@@ -19,11 +19,8 @@ int qkl_apply_partials(const qk::QkTables* T);
 int qkl_blas_partials(void);
 
 /* y = H x (pull form). cplx: interleaved complex x/y. wide: 64-bit states.
- * Lanczos epilogue when lz_k >= 0 (real only):
- *   w = (H x) / nrm[k] - (k > 0 ? nrm[k] * prev / nrm[k-1] : 0) -> y
- *   partials[b] = sum_rows (x / nrm[k]) * w        (deterministic order)   */
-int qkl_apply(const qk::QkTables* T, int cplx, int wide, const double* x, double* y,
-              int lz_k, const double* prev, const double* nrm, double* partials, qkl_stream s);
+ * The Lanczos form is qkl_lanczos_apply below. */
+int qkl_apply(const qk::QkTables* T, int cplx, int wide, const double* x, double* y, qkl_stream s);
 
 /* Block apply over nc <= 64 real columns: X[row * ld + c0 + c]. Lanczos
  * epilogue per column when lz_k >= 0 (scalars per column, stride ld_sc):
@@ -38,8 +35,10 @@ int qkl_block_partials(const qk::QkTables* T);
  * partial sums the other wrote instead of a separate reduction launch:
  *   qkl_lanczos_apply:   nrm[k] = sqrt(total of nrm_part) (the previous
  *                        update's, or the start vector's, sums of squares;
- *                        workgroup 0 stores it), then the fused apply of
- *                        qkl_apply with Lanczos epilogue (partials of alpha);
+ *                        workgroup 0 stores it), then y = H x with the
+ *                        Lanczos epilogue
+ *                          w = (H x) / nrm[k] - (k > 0 ? nrm[k] prev / nrm[k-1] : 0)
+ *                          partials[b] = sum_rows (x / nrm[k]) w  (of alpha);
  *   qkl_lanczos_update1: alpha[k] = total of a_part (workgroup 0 stores it),
  *                        then qkl_lanczos_update for one vector.
  * Totals are fixed-order (qkl_total computes the same bits). k is taken
