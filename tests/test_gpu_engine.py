@@ -438,3 +438,33 @@ def test_two_level_lo_rank(kind, n, sec, terms, splits):
             x1, y1 = qa.DeviceArray.from_host(Xh[:, cidx]), qa.DeviceArray(om.dim)
             H.apply_device(x1.ptr, y1.ptr, 1)
             assert np.array_equal(Yh[:, cidx], y1.to_host()), (split, cidx)
+
+
+def test_degenerate_sizes_follow_the_reference():
+    """lanczos.cpp:109, davidson.cpp:94, dynamics.cpp:20, ftlm.cpp:131: an
+    empty basis returns the default result. With one state Lanczos stops
+    after one step (beta = 0 < 1e-15, converged) at E = H_00; with
+    maxiter = 1 it stops after alpha_0, unconverged (lanczos.cpp:155-161)."""
+    site, terms = qa.SpinHalfSite(), models.heisenberg_chain(4)
+    H0 = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(4, sz2=10), site, qa.OpSum(terms))
+    assert H0.dimension == 0
+    r = qa.lanczos_ground_state(H0)
+    assert (r.energy, r.iterations, r.converged, len(r.eigenvector)) == (0.0, 0, False, 0)
+    d = qa.davidson_lowest(H0)
+    assert d.iterations == 0 and not d.converged
+    cf = qa.continued_fraction_coeffs(H0, np.zeros(0, dtype=np.complex128), 20)
+    assert len(cf.alphas) == 0 and len(cf.betas) == 0 and cf.norm_phi0 == 0.0
+    f = qa.ftlm(H0, 0.5, 4, 10)
+    assert (f.beta, f.partition_function, f.internal_energy, f.specific_heat) == (0.5, 0.0, 0.0, 0.0)
+    assert len(H0.apply(np.zeros(0))) == 0
+
+    H1 = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(4, sz2=4), site, qa.OpSum(terms))
+    assert H1.dimension == 1
+    h00 = H1.apply(np.ones(1))[0].real
+    r = qa.lanczos_ground_state(H1)
+    assert r.iterations == 1 and r.converged and r.energy == h00
+    assert abs(abs(r.eigenvector[0]) - 1.0) < 1e-15
+
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(8, sz2=0), site, qa.OpSum(models.heisenberg_chain(8)))
+    r = qa.lanczos_ground_state(H, maxiter=1)
+    assert r.iterations == 1 and not r.converged and len(r.alphas) == 1 and r.energy == r.alphas[0]
