@@ -583,7 +583,10 @@ HostTables build_tables(const BasisSpec& basis, int NL)
     if (!basis.spinless() && (NL & 1)) throw std::invalid_argument("NL must be even for two-species bases");
     if (NL > 20) throw std::invalid_argument("NL too large");
     const int NH = nbits - NL;
-    if (NH > 28) throw std::invalid_argument("basis too wide for the device tables (more than 48 bits)");
+    if (NH > 28)
+        throw std::invalid_argument("split NL = " + std::to_string(NL) + " leaves " + std::to_string(NH) +
+                                    " high bits; at most 28 are supported (the block-offset table has "
+                                    "2^(bits - NL) entries): use a larger split");
     if (basis.dim() >= (int64_t(1) << 32)) throw std::invalid_argument("dimension exceeds 2^32 rows");
     T.NL = NL;
     T.NH = NH;
