@@ -471,10 +471,11 @@ def test_degenerate_sizes_follow_the_reference():
 
 
 def test_wide_states_and_size_limit():
-    """64-bit states: a 40-site chain with three down spins (D = C(40, 3)),
+    """64-bit states: a 40-site chain with three down spins (D = C(40, 3))
+    and an 18-site Hubbard ring with 2 + 2 electrons (36 bits, signed hops),
     states bit-exact against the oracle's counting unrank and the apply
-    against its CSR, for two splits; a basis of 2^32 rows or more is refused
-    with a message (row indices are 32-bit)."""
+    against its CSR, for two splits; a basis of 2^32 rows or more, and a
+    split leaving more than 28 high bits, are refused with a message."""
     terms = models.heisenberg_chain(40)
     om = O.Model(O.SPIN, 40, O.sector(use_sz=1, sz2=34), O.SPIN, terms, enumerate=False)
     ref = np.array([om.state(i) for i in range(om.dim)], dtype=np.uint64)
@@ -486,7 +487,21 @@ def test_wide_states_and_size_limit():
         assert H.dimension == om.dim == 9880
         assert np.array_equal(H.states(0, H.dimension).astype(np.uint64), ref)
         assert np.max(np.abs(H.apply(xh).real - yref)) < 1e-13 * np.max(np.abs(yref))
+    # fermions on 36 bits: the periodic hops cross both 32-bit halves with
+    # Jordan-Wigner signs
+    terms = models.hubbard_chain(18, 1.0, 4.0)
+    om = O.Model(O.HUBBARD, 18, O.sector(use_nup=1, nup=2, use_ndn=1, ndn=2), O.HUBBARD, terms, enumerate=False)
+    ref = np.array([om.state(i) for i in range(om.dim)], dtype=np.uint64)
+    xh = np.random.default_rng(18).uniform(-1, 1, om.dim)
+    yref = om.apply(xh + 0j).real
+    for split in (-1, 14):
+        H = qa.MatrixFreeHamiltonian(qa.HubbardBasis(18, nup=2, ndn=2), qa.HubbardSite(), qa.OpSum(terms),
+                                     split_bits=split)
+        assert H.dimension == om.dim == 23409
+        assert np.array_equal(H.states(0, H.dimension).astype(np.uint64), ref)
+        assert np.max(np.abs(H.apply(xh).real - yref)) < 1e-13 * np.max(np.abs(yref))
     with pytest.raises(ValueError, match="2\\^32"):
         qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(36, sz2=0), qa.SpinHalfSite(), qa.OpSum(models.heisenberg_chain(36)))
     with pytest.raises(ValueError, match="at most 28"):
-        qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(40, sz2=34), qa.SpinHalfSite(), qa.OpSum(terms), split_bits=8)
+        qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(40, sz2=34), qa.SpinHalfSite(), qa.OpSum(models.heisenberg_chain(40)),
+                                 split_bits=8)
