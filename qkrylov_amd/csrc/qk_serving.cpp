@@ -253,6 +253,11 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                 set_last_error("serve: request with L outside 2..32, maxiter <= 0 or cf_steps < 0");
                 return QKRYLOV_ERROR_INVALID_ARG;
             }
+            if (dim_of(reqs[i]) == 0) {
+                set_last_error("serve: request whose Sz sector is empty (sz2 out of range or of the wrong parity "
+                               "for L)");
+                return QKRYLOV_ERROR_INVALID_ARG;
+            }
             if (footprint_of(reqs[i]) > budget) {   // could never be admitted
                 set_last_error("serve: a request's device footprint exceeds the memory budget");
                 return QKRYLOV_ERROR_INVALID_ARG;
@@ -271,17 +276,36 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
             BasisSpec b(Kind::SpinHalf, reqs[i].L, s);
             keep[key] = basis_tables(b, choose_NL(b));
         }
-        std::vector<hipStream_t> streams(max_active);
-        std::vector<hipEvent_t> events(max_active);
+        // streams, events and jobs are released on every exit; on an error
+        // the streams are drained first, so no job's memory is freed while
+        // its kernels may still run
+        std::vector<hipStream_t> streams;
+        std::vector<hipEvent_t> events;
+        std::vector<std::unique_ptr<Job>> active;
+        struct Release {
+            std::vector<hipStream_t>& st;
+            std::vector<hipEvent_t>& ev;
+            std::vector<std::unique_ptr<Job>>& jobs;
+            ~Release()
+            {
+                for (auto s : st) (void)hipStreamSynchronize(s);
+                jobs.clear();
+                for (auto s : st) (void)hipStreamDestroy(s);
+                for (auto e : ev) (void)hipEventDestroy(e);
+            }
+        } release{streams, events, active};
         for (int i = 0; i < max_active; ++i) {
-            hip_check(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking), "stream");
-            hip_check(hipEventCreateWithFlags(&events[i], hipEventDisableTiming), "event");
+            hipStream_t st = nullptr;
+            hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+            streams.push_back(st);
+            hipEvent_t ev = nullptr;
+            hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+            events.push_back(ev);
         }
         std::vector<int> order(nreq);
         for (int i = 0; i < nreq; ++i) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival < reqs[b].arrival; });
         std::deque<int> pending(order.begin(), order.end());
-        std::vector<std::unique_ptr<Job>> active;
         std::vector<int> free_slots;
         for (int i = max_active - 1; i >= 0; --i) free_slots.push_back(i);
         std::vector<int> slot_of;
@@ -346,11 +370,6 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
             if (!progress) std::this_thread::yield();
         }
         const double wall = now_s() - t0;
-        for (int i = 0; i < max_active; ++i) {
-            (void)hipStreamSynchronize(streams[i]);
-            (void)hipStreamDestroy(streams[i]);
-            (void)hipEventDestroy(events[i]);
-        }
         if (totals) {
             std::vector<double> lat(nreq);
             for (int i = 0; i < nreq; ++i) lat[i] = results[i].latency;

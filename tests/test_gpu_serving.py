@@ -69,3 +69,11 @@ def test_invalid_requests_are_refused_with_a_message():
         serving.run(bad, realtime=False)
     with pytest.raises(RuntimeError, match="footprint exceeds"):
         serving.run(reqs, realtime=False, hbm_budget_bytes=1024)
+    for sz2 in (1, 14):   # wrong parity for 12 sites; beyond the all-up state
+        bad = [dict(r) for r in reqs]
+        bad[1]["sz2"] = sz2
+        with pytest.raises(RuntimeError, match="sector is empty"):
+            serving.run(bad, realtime=False)
+    # the loop still serves after refused runs
+    out, tot = serving.run(reqs, realtime=False)
+    assert len(out) == 2 and all(r["iterations"] > 0 for r in out) and tot["solves_per_s"] > 0
