@@ -1,6 +1,7 @@
 """The reference's own Python tests and examples, run against this package
 on the GPU (bindings/python/tests/test_basic.py, test_workflow.py,
-examples/heisenberg_python.py, dynamics_example.py, hdf5_storage.py),
+tests/python/test_scipy.py, examples/heisenberg_python.py,
+dynamics_example.py, hdf5_storage.py),
 plus to_sparse against the textbook Kronecker matrix, dtype handling and
 storage of solver results."""
 import math
@@ -154,3 +155,29 @@ def test_storage_example_flow(tmp_path):
     back = qk.load_result(tmp_path / "dav.npz")
     assert np.array_equal(back.eigenvalues, d.eigenvalues)
     assert all(np.array_equal(a, b) for a, b in zip(back.eigenvectors, d.eigenvectors))
+
+
+def test_reference_scipy_flow():
+    """tests/python/test_scipy.py: the two-site Sz = 0 Hamiltonian from
+    operator algebra, as a scipy LinearOperator and a sparse matrix, a host
+    copy by to(device="cpu") and H @ x equal to H.apply(x); plus the values
+    those objects must hold (singlet -3/4, triplet 1/4)."""
+    basis = qk.SpinHalfBasis(2, sz=0)
+    site = qk.SpinHalfSite()
+    os_ = qk.OpSum()
+    os_ += 1.0 * qk.Sz(0) * qk.Sz(1) + 0.5 * (qk.Sp(0) * qk.Sm(1) + qk.Sm(0) * qk.Sp(1))
+    H = qk.MatrixFreeHamiltonian(basis, site, os_)
+    assert H.dimension > 0
+    H_op = H.aslinearoperator()
+    assert H_op.shape == (H.dimension, H.dimension)
+    H_mat = H.to_sparse()
+    assert H_mat.shape == (H.dimension, H.dimension)
+    H2 = H.to(device="cpu")
+    assert H2.dimension == H.dimension
+    x = np.random.rand(H.dimension)
+    y1 = H.apply(x)
+    y2 = H2 @ x
+    np.testing.assert_allclose(y1, y2)
+    np.testing.assert_allclose(H_op @ x, y1)
+    np.testing.assert_allclose(H_mat @ x, y1)
+    np.testing.assert_allclose(np.linalg.eigvalsh(H_mat.toarray()), [-0.75, 0.25], atol=1e-15)
