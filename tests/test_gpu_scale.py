@@ -6,7 +6,9 @@ properties and sampled rows:
     t-J 4x5 2 holes (D = 9,237,800): states and row entries of sampled rows
     bit-exact against the oracle's counting rank and Phase-1 row build, and
     sampled rows of a full apply against the oracle's row sums over the same
-    counter-RNG input vector."""
+    counter-RNG input vector;
+  - t-J 4x5: batched FTLM against sequential runs over the same vectors, and
+    the block continued fraction against single-vector runs, within 1e-8."""
 import ctypes
 
 import numpy as np
@@ -150,3 +152,21 @@ def test_full_size_ftlm_batched_equals_sequential():
         assert np.all(np.abs(b[key] - s[key]) <= 1e-8 * np.maximum(np.abs(s[key]), 1e-300)), key
     e0 = qa.lanczos_ground_state(H, maxiter=200, tol=1e-10, reorthogonalize=False, return_state=False).energy
     assert np.all(b["Z"] > 0) and np.all(b["E"] > e0 - 1e-8 * abs(e0))
+
+
+def test_full_size_block_cf_equals_single():
+    """C4 at full size: the continued fraction of two complex seeds as one
+    block recurrence (four real columns) against the single-vector
+    continued_fraction_coeffs of each seed: coefficients and norms within
+    1e-8 relative over 100 steps."""
+    H = qa.MatrixFreeHamiltonian(qa.TJBasis(20, nup=9, ndn=9), qa.TJSite(), qa.OpSum(models.tj_model(4, 5, 1.0, 0.4)))
+    D = H.dimension
+    rng = np.random.default_rng(19)
+    seeds = rng.uniform(-1, 1, (2, D)) + 1j * rng.uniform(-1, 1, (2, D))
+    block, _ = qa.continued_fraction_batched(H, seeds, 100)
+    for s in range(2):
+        one = qa.continued_fraction_coeffs(H, seeds[s], 100)
+        assert len(block[s].alphas) == len(one.alphas) == 100
+        assert abs(block[s].norm_phi0 - one.norm_phi0) <= 1e-12 * one.norm_phi0
+        assert np.max(np.abs(block[s].alphas - one.alphas)) < 1e-8 * np.max(np.abs(one.alphas))
+        assert np.max(np.abs(block[s].betas - one.betas)) < 1e-8 * np.max(np.abs(one.betas))
