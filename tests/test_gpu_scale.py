@@ -1,7 +1,8 @@
 """Full-size parity at the benchmark configurations, through size-independent
 properties and sampled rows:
   - L=20 Heisenberg ring (D = 184,756): Lanczos E0 vs the Bethe ansatz and
-    vs the oracle's Lanczos, 1e-10 relative;
+    vs the oracle's Lanczos, 1e-10 relative; L=32 (D = 601,080,390) vs the
+    Bethe ansatz;
   - J1-J2 4x8 (D = 601,080,390), Hubbard 16 sites 8/8 (D = 165,636,900),
     t-J 4x5 2 holes (D = 9,237,800): states and row entries of sampled rows
     bit-exact against the oracle's counting rank and Phase-1 row build, and
@@ -45,6 +46,18 @@ def test_heisenberg_ring20_lanczos_vs_bethe_and_oracle():
     rr = om.lanczos(300, 1e-12, start=O.start_vector(om.dim, 1234) + 0j)
     assert abs(r.energy - rr["energy"]) < 1e-10 * abs(rr["energy"])
     assert r.iterations == rr["iterations"]
+
+
+def test_heisenberg_ring32_full_size_vs_bethe():
+    """The 32-site Heisenberg ring at full size (D = 601,080,390, the same D
+    as C2): the plain three-term Lanczos (no stored basis, graph-replayed
+    steps) reaches the Bethe-ansatz ground-state energy within 1e-10."""
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(32, sz2=0), qa.SpinHalfSite(), qa.OpSum(models.heisenberg_chain(32)))
+    assert H.dimension == 601080390
+    e_b = TB.bethe_xxx_ring(32)
+    r = qa.lanczos_ground_state(H, maxiter=300, tol=1e-12, reorthogonalize=False, return_state=False)
+    print(f"ring32: E = {r.energy!r} after {r.iterations} iterations; Bethe {e_b!r}", flush=True)
+    assert r.converged and abs(r.energy - e_b) < 1e-10 * abs(e_b)
 
 
 CONFIGS = {
