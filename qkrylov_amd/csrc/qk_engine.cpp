@@ -643,21 +643,27 @@ void RealCore::steps_graph(int k, int n)
         graph = nullptr;
     }
     if (!gexec) {
-        if (kdev.bytes() == 0) kdev = DevBuf(sizeof(int));
+        if (kdev.bytes() == 0) kdev = DevBuf(2 * sizeof(int));
         cap_ic = ic;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
         int rc = 0;
+        // six steps, two turns of the vector rotation: steps 0-2 read
+        // counter 0 and the third update stores counter 1 = counter 0 + 3;
+        // steps 3-5 read counter 1 and the sixth update stores counter 0 for
+        // the next replay. No launch writes a counter it reads.
         int* kd = static_cast<int*>(kdev.get());
-        for (int i = 0; i < 3 && rc == 0; ++i) {
-            rc |= qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), 0, kd, i, prv(), nrm.d(), partB.d(), NB,
-                                    partA.d(), s);
-            rc |= qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), 0, kd, i, partA.d(), npA, partB.d(), s);
+        for (int i = 0; i < 6 && rc == 0; ++i) {
+            int* kc = kd + i / 3;
+            int* kn = (i % 3 == 2) ? kd + (1 - i / 3) : nullptr;
+            rc |= qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), 0, kc, i % 3, prv(), nrm.d(), partB.d(),
+                                    NB, partA.d(), s);
+            rc |= qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), 0, kc, i % 3, partA.d(), npA, partB.d(), kn,
+                                      s);
             const int t = ip;
             ip = ic;
             ic = iw;
             iw = t;
         }
-        rc |= qkl_add_int(kd, 3, s);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &g);
         if (rc != 0 || e != hipSuccess) {
@@ -668,11 +674,12 @@ void RealCore::steps_graph(int k, int n)
         hip_check(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "graph instantiate");
     }
     // the counter is set by a one-thread kernel outside the graph (no host
-    // staging, so the caller never waits); three rotations leave the vector
-    // roles as they were
+    // staging, so the caller never waits); six rotations leave the vector
+    // roles as they were; a remaining turn of three steps runs eagerly
     lcheck(qkl_set_int(static_cast<int*>(kdev.get()), k, s), "set step");
-    for (int r = 0; r < n / 3; ++r) hip_check(hipGraphLaunch(gexec, s), "graph launch");
-    pending = k + n;
+    for (int r = 0; r < n / 6; ++r) hip_check(hipGraphLaunch(gexec, s), "graph launch");
+    pending = k + n / 6 * 6;
+    for (int i = k + n / 6 * 6; i < k + n; ++i) step(i, nullptr);
 }
 
 void RealCore::init_norm()
@@ -700,7 +707,7 @@ void RealCore::step(int k, const double* basis)
     lcheck(qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), k, nullptr, 0, prv(), nrm.d(), partB.d(), NB,
                              partA.d(), s),
            "apply");
-    lcheck(qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), k, nullptr, 0, partA.d(), npA, partB.d(), s),
+    lcheck(qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), k, nullptr, 0, partA.d(), npA, partB.d(), nullptr, s),
            "update");
     if (basis) {
         // classical Gram-Schmidt against v_0..v_k (stored normalised)

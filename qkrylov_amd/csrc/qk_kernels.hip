@@ -810,7 +810,8 @@ __global__ __launch_bounds__(QK_TPB) void k_update1(size_t n, double* __restrict
                                                      double* __restrict__ alpha, const double* __restrict__ nrm,
                                                      int k, double* __restrict__ partials,
                                                      const int* __restrict__ kdev, int koff,
-                                                     const double* __restrict__ a_part, int n_a_part)
+                                                     const double* __restrict__ a_part, int n_a_part,
+                                                     int* __restrict__ knext)
 {
     __shared__ double wsum[QK_WPB];
     __shared__ double tot_sh;
@@ -820,7 +821,12 @@ __global__ __launch_bounds__(QK_TPB) void k_update1(size_t n, double* __restrict
     double a;
     if (a_part) {
         a = block_total(a_part, n_a_part, &tot_sh);
-        if (blockIdx.x == 0 && threadIdx.x == 0) alpha[kk] = a;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            alpha[kk] = a;
+            // a graph's last step of a turn advances the other step counter
+            // (one this launch does not read) to the next turn
+            if (knext) *knext = kk - koff + 3;
+        }
     } else {
         a = alpha[kk];
     }
@@ -1019,11 +1025,6 @@ __global__ __launch_bounds__(64) void k_total(const double* __restrict__ p, int 
     __shared__ double tot_sh;
     const double t = block_total(p, n, &tot_sh);
     if (threadIdx.x == 0) out[idx] = root ? sqrt(t) : t;
-}
-
-__global__ void k_add_int(int* __restrict__ p, int v)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] += v;
 }
 
 __global__ void k_set_int(int* __restrict__ p, int v)
@@ -1447,23 +1448,18 @@ int qkl_lanczos_apply(const QkTables* T, int wide, const double* x, double* y, i
 }
 
 int qkl_lanczos_update1(size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
-                        const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, qkl_stream sv)
+                        const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, int* knext,
+                        qkl_stream sv)
 {
     if (!a_part || n_a_part <= 0) return -1;
     hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, (hipStream_t)sv, nrows, w, q, alpha,
-                       nrm, k, partials, kdev, koff, a_part, n_a_part);
+                       nrm, k, partials, kdev, koff, a_part, n_a_part, knext);
     return rc(hipGetLastError());
 }
 
 int qkl_total(const double* partials, int npart, int root, double* out, int idx, qkl_stream sv)
 {
     hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, (hipStream_t)sv, partials, npart, root, out, idx);
-    return rc(hipGetLastError());
-}
-
-int qkl_add_int(int* p, int v, qkl_stream sv)
-{
-    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, (hipStream_t)sv, p, v);
     return rc(hipGetLastError());
 }
 
@@ -1492,10 +1488,10 @@ int qkl_lanczos_update(int cplx, size_t nrows, int ncol, int ld, double* w, cons
         double* aw = const_cast<double*>(alpha);
         if (cplx)
             hipLaunchKernelGGL(k_update1<true>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, aw, nrm, k,
-                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0);
+                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0, (int*)nullptr);
         else
             hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, aw, nrm, k,
-                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0);
+                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0, (int*)nullptr);
     } else {
         if (cplx || ncol > 64) return -1;
         hipLaunchKernelGGL(k_update_block, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, ncol, ld, w, q, alpha, nrm,

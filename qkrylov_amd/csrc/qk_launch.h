@@ -40,7 +40,8 @@ int qkl_block_partials(const qk::QkTables* T);
  *                          w = (H x) / nrm[k] - (k > 0 ? nrm[k] prev / nrm[k-1] : 0)
  *                          partials[b] = sum_rows (x / nrm[k]) w  (of alpha);
  *   qkl_lanczos_update1: alpha[k] = total of a_part (workgroup 0 stores it),
- *                        then qkl_lanczos_update for one vector.
+ *                        then qkl_lanczos_update for one vector; with knext
+ *                        set, workgroup 0 also stores k - koff + 3 there.
  * Totals are fixed-order (qkl_total computes the same bits). k is taken
  * from *kdev + koff when kdev is set (launches captured once in a graph and
  * replayed), else from the argument. */
@@ -48,11 +49,11 @@ int qkl_lanczos_apply(const qk::QkTables* T, int wide, const double* x, double* 
                       const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
                       qkl_stream s);
 int qkl_lanczos_update1(size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
-                        const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, qkl_stream s);
+                        const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, int* knext,
+                        qkl_stream s);
 /* out[idx] = total of partials[0..npart) (sqrt when root), in the order of
- * the two launches above; *p += v on the stream (one thread) */
+ * the two launches above */
 int qkl_total(const double* partials, int npart, int root, double* out, int idx, qkl_stream s);
-int qkl_add_int(int* p, int v, qkl_stream s);
 
 /* q_{k+1} = w - alpha[k] * q_k / nrm[k]; partials of |q_{k+1}|^2 (ncol columns
  * interleaved with stride ld; ncol = 1, ld = 1 for a single vector; cplx for
