@@ -133,7 +133,7 @@ typedef struct {
     int want_vector;
     uint64_t seed;        /* start vector uniform[-1,1) from (seed, index) */
     int check_every;      /* 0: automatic */
-    int graph_mode;       /* 0: automatic (replay a captured three-step graph below
+    int graph_mode;       /* 0: automatic (replay a captured six-step graph below
                            * 2^22 rows when no basis is stored), 1: always, 2: never */
 } qkrylov_lanczos_options_t;
 

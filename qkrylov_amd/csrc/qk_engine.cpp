@@ -756,7 +756,7 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
     if (store) lcheck(qkl_copy_scaled((size_t)D, C.cur(), basis.d(), C.nrm.d(), 0, s), "store v0");
 
     LanczosReplay rp{kmax, o.tol, o.fixed_steps, {}, {}};
-    // launch-bound sizes replay a captured three-step graph between host checks
+    // launch-bound sizes replay a captured six-step graph between host checks
     const bool graph = !store && !reorth && kmax >= 3 && (o.graph > 0 || (o.graph < 0 && D < (1 << 22)));
     int check = o.check_every > 0 ? o.check_every : (D < (1 << 22) ? (graph ? 12 : 8) : 1);
     if (o.fixed_steps) check = kmax;

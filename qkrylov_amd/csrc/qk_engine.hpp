@@ -163,8 +163,8 @@ struct RealCore {
     void step(int k, const double* basis);   // basis: normalised v_0..v_k for reorthogonalisation
     void flush();
     // steps k .. k + n - 1 (n a multiple of 3, k + n <= kcap) as replays of
-    // the captured three-step graph; the same kernels in the same order as
-    // step(), so the same numbers
+    // the captured six-step graph (a remaining three steps eagerly); the
+    // same kernels in the same order as step(), so the same numbers
     void steps_graph(int k, int n);
 };
 

@@ -384,7 +384,7 @@ def test_autotune_keeps_results():
 
 @pytest.mark.parametrize("check_every", [0, 8])
 def test_lanczos_graph_replay_equals_eager(check_every):
-    """The captured three-step graph replays the eager launches exactly: the
+    """The captured six-step graph replays the eager launches exactly: the
     same alpha/beta bit for bit, with host reads and eager steps between the
     replays (check_every = 8 leaves the vector rotation off the graph's
     capture state, which forces a second capture)."""
