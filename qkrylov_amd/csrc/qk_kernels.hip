@@ -383,6 +383,9 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
         const uint32_t my_lr = tl.row_start - base_self + lidx;
         const uint32_t row = tl.row_start + lidx;
         const uint32_t lo = lolist[tl.lo_base + lidx];
+        // the Lanczos epilogue's q_{k-1} entry, loaded now so that its
+        // latency hides behind the tile's gathers
+        const double pv = (LZ && lzk > 0) ? __builtin_nontemporal_load(prev + row) : 0.0;
         const ST s = (static_cast<ST>(hi) << NL) | static_cast<ST>(lo);
 
         // (1) the uniform part of every branch, one branch per lane
@@ -467,7 +470,7 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
         if (active) {
             if (LZ) {
                 double w = ar * inv_n;
-                if (lzk > 0) w = fma(-cprev, __builtin_nontemporal_load(prev + row), w);
+                if (lzk > 0) w = fma(-cprev, pv, w);
                 __builtin_nontemporal_store(w, y + row);
                 part = fma(xr * inv_n, w, part);
             } else if (CPLX) {
