@@ -64,6 +64,9 @@ QkScal make_scal(const QkTables* T)
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// two doubles as one 16 B vector access
+typedef double qk_double2 __attribute__((ext_vector_type(2)));
+
 template <typename ST>
 __device__ __forceinline__ int popc(ST x);
 template <>
@@ -836,10 +839,33 @@ __global__ __launch_bounds__(QK_TPB) void k_update1(size_t n, double* __restrict
     const double c = a / nrm[kk];
     double part = 0.0;
     const size_t nel = CPLX ? 2 * n : n;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nel; i += (size_t)gridDim.x * blockDim.x) {
-        const double v = fma(-c, q[i], w[i]);
-        w[i] = v;
-        part = fma(v, v, part);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (((reinterpret_cast<uintptr_t>(w) | reinterpret_cast<uintptr_t>(q)) & 15u) == 0) {
+        // 16 B per lane, then an odd last element
+        const size_t npair = nel / 2;
+        qk_double2* w2 = reinterpret_cast<qk_double2*>(w);
+        const qk_double2* q2 = reinterpret_cast<const qk_double2*>(q);
+        for (size_t i = tid; i < npair; i += stride) {
+            const qk_double2 wv = w2[i], qv = q2[i];
+            qk_double2 v;
+            v.x = fma(-c, qv.x, wv.x);
+            v.y = fma(-c, qv.y, wv.y);
+            w2[i] = v;
+            part = fma(v.x, v.x, part);
+            part = fma(v.y, v.y, part);
+        }
+        if ((nel & 1) && tid == 0) {
+            const double v = fma(-c, q[nel - 1], w[nel - 1]);
+            w[nel - 1] = v;
+            part = fma(v, v, part);
+        }
+    } else {
+        for (size_t i = tid; i < nel; i += stride) {
+            const double v = fma(-c, q[i], w[i]);
+            w[i] = v;
+            part = fma(v, v, part);
+        }
     }
     part = wave_sum(part);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = part;
@@ -1004,7 +1030,6 @@ __global__ __launch_bounds__(QK_TPB) void k_copy_scaled(size_t n, const double* 
 
 // 16 B per lane with streaming (nontemporal) loads and stores: the fastest
 // of the plain copy shapes measured on MI355X (about 5.7 TB/s for 2 GiB)
-typedef double qk_double2 __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(QK_TPB) void k_copy(size_t n, const double* __restrict__ src, double* __restrict__ dst)
 {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
