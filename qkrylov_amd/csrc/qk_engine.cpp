@@ -758,7 +758,10 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
     LanczosReplay rp{kmax, o.tol, o.fixed_steps, {}, {}};
     // launch-bound sizes replay a captured six-step graph between host checks
     const bool graph = !store && !reorth && kmax >= 3 && (o.graph > 0 || (o.graph < 0 && D < (1 << 22)));
-    int check = o.check_every > 0 ? o.check_every : (D < (1 << 22) ? (graph ? 12 : 8) : 1);
+    // host convergence checks: every 8-12 steps while launches dominate,
+    // every 4 up to 2^25 rows (a read-back costs a tenth of a step there),
+    // every step beyond; the replay fixes the iteration count either way
+    int check = o.check_every > 0 ? o.check_every : (D < (1 << 22) ? (graph ? 12 : 8) : (D < (1 << 25) ? 4 : 1));
     if (o.fixed_steps) check = kmax;
     std::vector<double> ha(kmax + 2), hn(kmax + 3);
     const double t0 = now_s();
