@@ -122,7 +122,10 @@ def test_full_size_apply_sampled_rows(name):
 
 
 SYM_CONFIGS = dict(CONFIGS, heis_ring32=(O.SPIN, 32, dict(use_sz=1, sz2=0), lambda: models.heisenberg_chain(32),
-                                          lambda: qa.SpinHalfBasis(32, sz2=0), 601080390))
+                                          lambda: qa.SpinHalfBasis(32, sz2=0), 601080390),
+                   # D above 2^31: row indices use the upper half of the unsigned 32-bit range
+                   heis_ring34=(O.SPIN, 34, dict(use_sz=1, sz2=0), lambda: models.heisenberg_chain(34),
+                                lambda: qa.SpinHalfBasis(34, sz2=0), 2333606220))
 
 
 @pytest.mark.parametrize("name", sorted(SYM_CONFIGS))
