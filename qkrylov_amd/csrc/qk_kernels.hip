@@ -439,18 +439,18 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
 
         // (3) taken hi-only branches: coalesced gathers of the tile's own
         // lo-rank range in each target block, four in flight
-        for (uint32_t i = 0; i < ntaken; i += 8) {
-            uint32_t rt[8];
-            double cr[8], ci[8];
+        for (uint32_t i = 0; i < ntaken; i += 4) {
+            uint32_t rt[4];
+            double cr[4], ci[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 4; ++u) {
                 const bool in = (i + u) < ntaken;
                 const uint32_t k = in ? i + u : 0u;
                 rt[u] = in ? a_base[k] + my_lr : row;
                 cr[u] = in ? a_cr[k] : 0.0;
                 ci[u] = in ? a_ci[k] : 0.0;
             }
-            accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
+            accumulate<CPLX, 4>(x, rt, cr, ci, ar, ai);
         }
 
         // (4) remaining branches: per-lane test of lo & lcm against the two
