@@ -614,7 +614,7 @@ RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool
     for (auto& q : Q) q = mk(D * sizeof(double));
     alpha = mk((kcap + 2) * sizeof(double));
     nrm = mk((kcap + 3) * sizeof(double));
-    npA = qkl_apply_partials(E.tables());
+    npA = qkl_apply_partials(E.tables(), E.wide() ? 1 : 0);
     if (npA <= 0) throw std::runtime_error("apply: the operator tables do not fit in LDS");
     NB = qkl_blas_partials();
     partA = mk(npA * sizeof(double));
@@ -634,7 +634,7 @@ RealCore::~RealCore()
 void RealCore::steps_graph(int k, int n)
 {
     if (n % 3 != 0 || k + n > kcap) throw std::logic_error("graph steps: n must be a multiple of 3 within the cap");
-    if (qkl_apply_partials(E.tables()) != npA) throw std::logic_error("apply launch shape changed during a run");
+    if (qkl_apply_partials(E.tables(), E.wide() ? 1 : 0) != npA) throw std::logic_error("apply launch shape changed during a run");
     if (gexec && ic != cap_ic) {
         // eager steps moved the rotation: capture again for the current roles
         (void)hipGraphExecDestroy(gexec);
@@ -656,7 +656,7 @@ void RealCore::steps_graph(int k, int n)
             int* kc = kd + i / 3;
             int* kn = (i % 3 == 2) ? kd + (1 - i / 3) : nullptr;
             rc |= qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), 0, kc, i % 3, prv(), nrm.d(), partB.d(),
-                                    NB, partA.d(), s);
+                                    NB, partA.d(), npA, s);
             rc |= qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), 0, kc, i % 3, partA.d(), npA, partB.d(), kn,
                                       s);
             const int t = ip;
@@ -703,9 +703,9 @@ void RealCore::step(int k, const double* basis)
 {
     // the partial buffer was sized for the launch shape of the tuning at
     // construction; a retuned engine would write past it
-    if (qkl_apply_partials(E.tables()) != npA) throw std::logic_error("apply launch shape changed during a run");
+    if (qkl_apply_partials(E.tables(), E.wide() ? 1 : 0) != npA) throw std::logic_error("apply launch shape changed during a run");
     lcheck(qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), k, nullptr, 0, prv(), nrm.d(), partB.d(), NB,
-                             partA.d(), s),
+                             partA.d(), npA, s),
            "apply");
     lcheck(qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), k, nullptr, 0, partA.d(), npA, partB.d(), nullptr, s),
            "update");

@@ -12,6 +12,10 @@
 // fixed-order: per-workgroup partials, then one workgroup sums them.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include <math.h>
@@ -246,17 +250,22 @@ __device__ __forceinline__ double block_total(const double* __restrict__ p, int 
         const int l = (int)threadIdx.x;
         double s = 0.0;
         if (n <= 64 * 32) {
-            // all loads in flight at once, then the adds in index order
-            // (absent terms add +0.0, which leaves the sum unchanged)
-            double v[32];
+            // batches of 16 loads in flight, then their adds in index order
+            // (absent terms add +0.0); a batch holds 1024 partials, which
+            // covers every launch shape but the smallest workgroups. Sixteen
+            // doubles per lane keep this prologue's register peak below the
+            // apply loop's, so it does not lower the apply's occupancy.
+            for (int j0 = 0; j0 < 32 && 64 * j0 < n; j0 += 16) {
+                double v[16];
 #pragma unroll
-            for (int j = 0; j < 32; ++j) {
-                const int i = l + 64 * j;
-                v[j] = p[i < n ? i : 0];
-                v[j] = i < n ? v[j] : 0.0;
+                for (int j = 0; j < 16; ++j) {
+                    const int i = l + 64 * (j0 + j);
+                    v[j] = p[i < n ? i : 0];
+                    v[j] = i < n ? v[j] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) s += v[j];
             }
-#pragma unroll
-            for (int j = 0; j < 32; ++j) s += v[j];
         } else {
             for (int i = l; i < n; i += 64) s += p[i];
         }
@@ -268,14 +277,14 @@ __device__ __forceinline__ double block_total(const double* __restrict__ p, int 
 }
 
 // ------------------------------------------------------------------ apply
-// Per-wave LDS scratch of the apply kernel: for every branch the tile's
-// uniform data (target block offset; the two values lo & lcm may take for
-// the branch to apply, given what the tile's hi allows, or an impossible
+// Per-wave LDS scratch of the apply kernel: for every per-lane branch the
+// tile's uniform data (target block offset; the two values lo & lcm may take
+// for the branch to apply, given what the tile's hi allows, or an impossible
 // value; hi parity), and the compacted list of the hi-only branches the
-// tile's hi takes.
-__host__ __device__ __forceinline__ size_t apply_wave_bytes(int nbk, int nA)
+// tile's hi takes (value, imaginary part for complex vectors, block offset).
+__host__ __device__ __forceinline__ size_t apply_wave_bytes(int nB, int nA, bool cplx)
 {
-    return (((size_t)nbk * 16 + (size_t)nA * 20) + 15) & ~(size_t)15;
+    return (((size_t)nB * 16 + (size_t)nA * (cplx ? 20 : 12)) + 15) & ~(size_t)15;
 }
 
 // Per-workgroup LDS copy of the lo side of the per-lane branches: 16 B of
@@ -353,11 +362,11 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
 
     const uint32_t lane = threadIdx.x & 63u;
     const int NL = sc.NL;
-    unsigned char* wb = smem + table_bytes + apply_branch_bytes(nB) + (threadIdx.x >> 6) * apply_wave_bytes(nbk, nA);
-    uint4* tb = reinterpret_cast<uint4*>(wb);                          // [nbk] {block offset, va, vb, hpar}
-    double* a_cr = reinterpret_cast<double*>(wb + (size_t)nbk * 16);   // [nA] taken hi-only values
-    double* a_ci = a_cr + nA;
-    uint32_t* a_base = reinterpret_cast<uint32_t*>(a_ci + nA);         // [nA] their block offsets
+    unsigned char* wb = smem + table_bytes + apply_branch_bytes(nB) + (threadIdx.x >> 6) * apply_wave_bytes(nB, nA, CPLX);
+    uint4* tb = reinterpret_cast<uint4*>(wb);                          // [nB] {block offset, va, vb, hpar}
+    double* a_cr = reinterpret_cast<double*>(wb + (size_t)nB * 16);    // [nA] taken hi-only values
+    double* a_ci = a_cr + nA;                                          // [nA] (complex only)
+    uint32_t* a_base = reinterpret_cast<uint32_t*>(a_cr + (CPLX ? 2 : 1) * nA);   // [nA] their block offsets
     const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
 
     // Lanczos step: from the argument, or from the device step counter
@@ -407,7 +416,7 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
                 // lo & lcm < 2^NL <= 2^20, so ~lcm is a value it never takes
                 const uint32_t never = ~B.lcm;
                 base = hoff[hi ^ B.hF];
-                tb[b] = make_uint4(base, u0 ? B.lcv : never, u1 ? (B.lcv ^ B.lalt) : never, hpar);
+                if (b >= nA) tb[b - nA] = make_uint4(base, u0 ? B.lcv : never, u1 ? (B.lcv ^ B.lalt) : never, hpar);
                 takeA = (b < nA) && (u0 | u1);
                 vr = hpar ? -B.re : B.re;
                 vi = hpar ? -B.im : B.im;
@@ -418,7 +427,7 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
                 const uint32_t pos = ntaken + (uint32_t)__popcll(m & lt_mask);
                 a_base[pos] = base;
                 a_cr[pos] = vr;
-                a_ci[pos] = vi;
+                if (CPLX) a_ci[pos] = vi;
             }
             ntaken += (uint32_t)__popcll(m);
         }
@@ -448,7 +457,7 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
                 const uint32_t k = in ? i + u : 0u;
                 rt[u] = in ? a_base[k] + my_lr : row;
                 cr[u] = in ? a_cr[k] : 0.0;
-                ci[u] = in ? a_ci[k] : 0.0;
+                ci[u] = (CPLX && in) ? a_ci[k] : 0.0;
             }
             accumulate<CPLX, 4>(x, rt, cr, ci, ar, ai);
         }
@@ -458,13 +467,13 @@ __global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles
         // two-level rank, three runs by rank kind
         const OwnRank orank = own_rank<RM>(smem, lo, sc);
         if (RM == 2) {
-            lane_branches<CPLX, RM, SIGNS, 1>(0, sc.nB1, smem, lb4, lbm, lbre, lbim, tb + nA, lo, row, orank, sc, x, ar, ai);
-            lane_branches<CPLX, RM, SIGNS, 2>(sc.nB1, sc.nB1 + sc.nB2, smem, lb4, lbm, lbre, lbim, tb + nA, lo, row, orank,
+            lane_branches<CPLX, RM, SIGNS, 1>(0, sc.nB1, smem, lb4, lbm, lbre, lbim, tb, lo, row, orank, sc, x, ar, ai);
+            lane_branches<CPLX, RM, SIGNS, 2>(sc.nB1, sc.nB1 + sc.nB2, smem, lb4, lbm, lbre, lbim, tb, lo, row, orank,
                                               sc, x, ar, ai);
-            lane_branches<CPLX, RM, SIGNS, 0>(sc.nB1 + sc.nB2, nB, smem, lb4, lbm, lbre, lbim, tb + nA, lo, row, orank, sc,
+            lane_branches<CPLX, RM, SIGNS, 0>(sc.nB1 + sc.nB2, nB, smem, lb4, lbm, lbre, lbim, tb, lo, row, orank, sc,
                                               x, ar, ai);
         } else {
-            lane_branches<CPLX, RM, SIGNS, 0>(0, nB, smem, lb4, lbm, lbre, lbim, tb + nA, lo, row, orank, sc, x, ar, ai);
+            lane_branches<CPLX, RM, SIGNS, 0>(0, nB, smem, lb4, lbm, lbre, lbim, tb, lo, row, orank, sc, x, ar, ai);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
@@ -1271,24 +1280,53 @@ inline size_t table_bytes(const QkTables* T)
     return ((size_t)T->nlo * (T->rank_mode == 1 ? 2 : 4) + 15) & ~(size_t)15;
 }
 
+// Workgroups of kernel f (64 * wpb threads, lds dynamic LDS bytes) that one
+// CU holds at once, by the runtime's occupancy calculator (registers, LDS,
+// wave slots), and the number of CUs; cached per kernel, shape and device.
+inline void residency(const void* f, int wpb, size_t lds, int& per_cu, int& ncu)
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t, int>, std::pair<int, int>> cache;
+    const auto key = std::make_tuple(f, wpb, lds, dev);
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        int n = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64 * wpb, lds) != hipSuccess || n < 1) n = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+        (void)hipGetLastError();
+        it = cache.emplace(key, std::make_pair(n, cus)).first;
+    }
+    per_cu = it->second.first;
+    ncu = it->second.second;
+}
+
 // Single-vector apply launch shape: the preferred number of waves per
 // workgroup (4, 8 or 16; they share one LDS copy of the tables), or fewer
-// when the LDS of that many waves exceeds 160 KB; the grid holds 8192
-// waves at most, a multiple of 8 workgroups (XCD segments).
+// when the LDS of that many waves exceeds 160 KB. The grid is a whole
+// number of full residencies of the chip (so no partial last round of
+// workgroups leaves a tail) of at most 8192 waves, a multiple of 8
+// workgroups (XCD segments). f is the kernel instance to be launched.
 struct ApplyCfg {
     int wpb, nb;
     size_t lds;
 };
 
-inline bool apply_config(const QkTables* T, ApplyCfg& c)
+inline bool apply_config(const QkTables* T, bool cplx, const void* f, ApplyCfg& c)
 {
     int wpb = (T->apply_waves == 8 || T->apply_waves == 16) ? T->apply_waves : 4;
     for (; wpb >= 1; wpb /= 2) {
         const size_t lds = table_bytes(T) + apply_branch_bytes(T->nB) +
-                           (size_t)wpb * apply_wave_bytes(T->nA + T->nB, T->nA);
+                           (size_t)wpb * apply_wave_bytes(T->nB, T->nA, cplx);
         if (lds > 160 * 1024) continue;
+        int per_cu = 1, ncu = 256;
+        residency(f, wpb, lds, per_cu, ncu);
+        const uint32_t resident = (uint32_t)per_cu * (uint32_t)ncu;
+        const uint32_t rounds = std::max(1u, (8192u / (uint32_t)wpb) / resident);
         uint32_t nb = (T->ntiles + wpb - 1) / wpb;
-        const uint32_t maxb = 8192u / (uint32_t)wpb;
+        const uint32_t maxb = resident * rounds;
         if (nb > maxb) nb = maxb;
         if (nb >= 8) nb = (nb + 7u) & ~7u;
         if (nb == 0) nb = 1;
@@ -1358,14 +1396,22 @@ struct LzArgs {
     int koff = 0;
     const double* nrm_part = nullptr;
     int n_nrm_part = 0;
+    int n_part = 0;           // partial sums the caller holds room for
 };
+
+template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
+const void* apply_fn()
+{
+    return reinterpret_cast<const void*>(&k_apply<ST, CPLX, RM, LZ, SIGNS>);
+}
 
 template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
 int launch_apply(const QkTables* T, const double* x, double* y, int k, const double* prev, double* nrm,
                  double* partials, hipStream_t s, const LzArgs& lz)
 {
     ApplyCfg c;
-    if (!apply_config(T, c)) return -2;
+    if (!apply_config(T, CPLX, apply_fn<ST, CPLX, RM, LZ, SIGNS>(), c)) return -2;
+    if (LZ && lz.n_part != c.nb) return -1;   // one partial sum per workgroup
     hipLaunchKernelGGL((k_apply<ST, CPLX, RM, LZ, SIGNS>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff,
                        T->lrank, T->tl, T->th, T->nth,
                        T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T),
@@ -1429,14 +1475,27 @@ int block_rm(const QkTables* T, const BlockCfg& c, const double* X, double* Y, i
     }
 }
 
+// the Lanczos-form instance apply_rm launches for these tables
+template <typename ST>
+const void* lz_apply_fn(const QkTables* T)
+{
+    const bool sg = T->signs != 0;
+    switch (T->rank_mode) {
+    case 2: return sg ? apply_fn<ST, false, 2, true, true>() : apply_fn<ST, false, 2, true, false>();
+    case 1: return sg ? apply_fn<ST, false, 1, true, true>() : apply_fn<ST, false, 1, true, false>();
+    default: return sg ? apply_fn<ST, false, 0, true, true>() : apply_fn<ST, false, 0, true, false>();
+    }
+}
+
 } // namespace
 
 extern "C" {
 
-int qkl_apply_partials(const QkTables* T)
+int qkl_apply_partials(const QkTables* T, int wide)
 {
     ApplyCfg c;
-    return apply_config(T, c) ? c.nb : -2;
+    const void* f = wide ? lz_apply_fn<uint64_t>(T) : lz_apply_fn<uint32_t>(T);
+    return apply_config(T, false, f, c) ? c.nb : -2;
 }
 int qkl_block_partials(const QkTables* T)
 {
@@ -1462,11 +1521,12 @@ int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y,
 // reductions folded in (see qk_launch.h).
 int qkl_lanczos_apply(const QkTables* T, int wide, const double* x, double* y, int k, const int* kdev, int koff,
                       const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
-                      qkl_stream sv)
+                      int n_part, qkl_stream sv)
 {
     hipStream_t s = (hipStream_t)sv;
     if (T->ntiles == 0 || !nrm_part || n_nrm_part <= 0) return -1;
     LzArgs lz;
+    lz.n_part = n_part;
     lz.kdev = kdev;
     lz.koff = koff;
     lz.nrm_part = nrm_part;

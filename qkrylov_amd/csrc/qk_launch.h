@@ -14,8 +14,10 @@ extern "C" {
 
 typedef void* qkl_stream;
 
-/* Number of partial sums (per reduced quantity) the reducing kernels write. */
-int qkl_apply_partials(const qk::QkTables* T);
+/* Number of partial sums (per reduced quantity) the reducing kernels write:
+ * qkl_apply_partials for qkl_lanczos_apply on these tables (wide: 64-bit
+ * states), qkl_blas_partials for the BLAS-1 and update kernels. */
+int qkl_apply_partials(const qk::QkTables* T, int wide);
 int qkl_blas_partials(void);
 
 /* y = H x (pull form). cplx: interleaved complex x/y. wide: 64-bit states.
@@ -44,10 +46,11 @@ int qkl_block_partials(const qk::QkTables* T);
  *                        set, workgroup 0 also stores k - koff + 3 there.
  * Totals are fixed-order (qkl_total computes the same bits). k is taken
  * from *kdev + koff when kdev is set (launches captured once in a graph and
- * replayed), else from the argument. */
+ * replayed), else from the argument. n_part is the room in partials; it
+ * must equal qkl_apply_partials(T, wide) or the launch is refused. */
 int qkl_lanczos_apply(const qk::QkTables* T, int wide, const double* x, double* y, int k, const int* kdev, int koff,
                       const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
-                      qkl_stream s);
+                      int n_part, qkl_stream s);
 int qkl_lanczos_update1(size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
                         const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, int* knext,
                         qkl_stream s);
