@@ -330,8 +330,12 @@ __device__ __forceinline__ void lane_branches(int gbeg, int gend, const unsigned
     }
 }
 
+// The real Lanczos form is held to 80 VGPRs (six waves per SIMD): left
+// alone, its epilogue's partial dot raises the register count past the
+// plain apply's and costs waves per CU in the 8- and 4-wave shapes. Other
+// instances keep the compiler's choice.
 template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
-__global__ __launch_bounds__(1024) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CPLX) ? 6 : 1))) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
                                                    const uint32_t* __restrict__ lrank, const uint16_t* __restrict__ tl,
                                                    const uint32_t* __restrict__ th, int nth,
                                                    const uint32_t* __restrict__ lolist,
