@@ -1297,10 +1297,16 @@ inline void residency(const void* f, int wpb, size_t lds, int& per_cu, int& ncu)
     std::lock_guard<std::mutex> g(mu);
     auto it = cache.find(key);
     if (it == cache.end()) {
+        // on a failed query: one workgroup per CU and the MI355X's 256 CUs
+        // (the grid size changes only the speed); the query's own error is
+        // cleared so that the launch after it does not report it
         int n = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64 * wpb, lds) != hipSuccess || n < 1) n = 1;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
-        (void)hipGetLastError();
+        bool failed = false;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, 64 * wpb, lds) != hipSuccess) failed = true;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) failed = true;
+        if (failed) (void)hipGetLastError();
+        if (n < 1) n = 1;
+        if (cus < 1) cus = 256;
         it = cache.emplace(key, std::make_pair(n, cus)).first;
     }
     per_cu = it->second.first;
