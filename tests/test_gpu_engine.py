@@ -355,6 +355,28 @@ def test_schedules_give_identical_results(name):
         H.set_tuning(block_rows=8)
 
 
+@pytest.mark.parametrize("name", ["heis22", "hubbard12"])
+def test_lanczos_under_every_launch_shape(name):
+    """The Lanczos-form apply writes one partial sum per workgroup, and its
+    grid follows the occupancy of the launched instance (more tiles than one
+    full residency of the chip here, for a spin and a fermion model): under
+    every workgroup width and schedule the run completes and reaches the same
+    energy; only the order in which the alpha and beta totals add differs."""
+    basis, site, terms = {
+        "heis22": lambda: (qa.SpinHalfBasis(22, sz2=0), qa.SpinHalfSite(), models.heisenberg_chain(22)),
+        "hubbard12": lambda: (qa.HubbardBasis(12, nup=6, ndn=6), qa.HubbardSite(), models.hubbard_chain(12, 1.0, 4.0)),
+    }[name]()
+    H = qa.MatrixFreeHamiltonian(basis, site, qa.OpSum(terms))
+    assert H.dimension > 600_000
+    es = []
+    for sched, waves in ((0, 4), (1, 8), (2, 16), (2, 8), (1, 4), (0, 16)):
+        H.set_tuning(schedule=sched, apply_waves=waves)
+        r = qa.lanczos_ground_state(H, 300, 1e-12)
+        assert r.converged
+        es.append(r.energy)
+    assert max(es) - min(es) < 1e-11 * abs(es[0])
+
+
 def test_autotune_keeps_results():
     """Autotuning launch shapes leaves the apply bitwise unchanged; letting it
     move the lo-bit split changes only the summation order of a row."""
