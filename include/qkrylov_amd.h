@@ -202,6 +202,23 @@ int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, const qkryl
                       qkrylov_serve_result_t* results, double* cf_alphas, double* cf_betas, int cf_cap,
                       double* totals);
 
+/* ---- host eigensolvers used by the Krylov drivers (no device work; usable
+ * without a GPU) ---- */
+/* Symmetric tridiagonal (alpha[0..n), beta[0..n-1)): eigenvalues ascending
+ * in evals; want 0: none, 1: evecs[i] = first component of eigenvector i,
+ * 2: evecs[r * n + i] = component r of eigenvector i (implicit QL). The
+ * quantities of the reference's tridiag_ground_state_full and
+ * diagonalize_tridiag_components (src/solvers/lanczos.cpp:25-98,
+ * src/solvers/ftlm.cpp:61-119). */
+int qkrylov_amd_tridiag_eigen(int n, const double* alpha, const double* beta, double* evals, double* evecs, int want);
+/* lowest eigenvalue only (Sturm bisection) */
+int qkrylov_amd_tridiag_lowest(int n, const double* alpha, const double* beta, double* lowest);
+/* Hermitian n x n (row-major, interleaved complex): ascending eigenvalues,
+ * evecs[2 * (r * n + i)] = component r of eigenvector i (Householder to
+ * tridiagonal + QL); the small eigenproblem of the reference's
+ * solve_small_hermitian (src/solvers/davidson.cpp:22-82). */
+int qkrylov_amd_hermitian_eigen(int n, const double* a, double* evals, double* evecs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -168,6 +168,10 @@ SIGNATURES = {
     "qkrylov_cf_batched_f64": (c_int, [c_vp, c_int, p_dbl, c_int, p_dbl, p_dbl, p_dbl, p_int, p_dbl]),
     "qkrylov_serve_run": (c_int, [P(ServeRequest), c_int, P(ServeOptions), P(ServeResult), p_dbl, p_dbl, c_int,
                                   p_dbl]),
+    # host eigensolvers
+    "qkrylov_amd_tridiag_eigen": (c_int, [c_int, p_dbl, p_dbl, p_dbl, p_dbl, c_int]),
+    "qkrylov_amd_tridiag_lowest": (c_int, [c_int, p_dbl, p_dbl, p_dbl]),
+    "qkrylov_amd_hermitian_eigen": (c_int, [c_int, p_dbl, p_dbl, p_dbl]),
 }
 
 _lib = None

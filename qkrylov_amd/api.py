@@ -42,83 +42,123 @@ def _vec_dtype(dtype):
     return np.complex64 if np.dtype(dtype) == np.float32 else np.complex128
 
 
-class LocalOpExpr:
-    def __init__(self, name, site):
-        self.name, self.site = _op_name(name), int(site)
+_SCALARS = (int, float, complex, np.number)
+
+
+def _expand(x):
+    """Operator expression or scalar -> tuple of (coefficient, factors)."""
+    if isinstance(x, OpProduct):
+        return ((x.coeff, x.factors),)
+    if isinstance(x, OpTermSum):
+        return x.terms
+    if isinstance(x, _SCALARS):
+        return ((complex(x), ()),)
+    return None
+
+
+def _combine(terms, single):
+    return OpProduct(*terms[0]) if single and len(terms) == 1 else OpTermSum(terms)
+
+
+class OpProduct:
+    """An ordered product coeff * f0 * f1 * ... of local operators, each
+    factor a (name, site) pair; the first factor acts first, as in the
+    reference's OperatorTerm. A local operator such as Sz(0) is the
+    one-factor product with coefficient one. Products multiply by
+    concatenating factors and multiplying coefficients; + and - make an
+    OpTermSum (bindings/python/qkrylov/operators.py:25-105 define the same
+    algebra)."""
+
+    __slots__ = ("coeff", "factors")
+
+    def __init__(self, coeff, factors):
+        self.coeff = complex(coeff)
+        self.factors = tuple((_op_name(n), int(i)) for n, i in factors)
+
+    @property
+    def name(self):
+        return self.factors[0][0]
+
+    @property
+    def site(self):
+        return self.factors[0][1]
 
     def __mul__(self, other):
-        if isinstance(other, LocalOpExpr):
-            return TermExpr(1.0, [self, other])
-        if isinstance(other, (int, float, complex)):
-            return TermExpr(complex(other), [self])
-        return NotImplemented
+        rhs = _expand(other)
+        if rhs is None:
+            return NotImplemented
+        out = tuple((self.coeff * c, self.factors + f) for c, f in rhs)
+        return _combine(out, not isinstance(other, OpTermSum))
 
     def __rmul__(self, other):
-        if isinstance(other, (int, float, complex)):
-            return TermExpr(complex(other), [self])
-        return NotImplemented
+        if not isinstance(other, _SCALARS):
+            return NotImplemented
+        return OpProduct(complex(other) * self.coeff, self.factors)
 
-
-class TermExpr:
-    def __init__(self, coeff, ops):
-        self.coeff, self.ops = complex(coeff), list(ops)
-
-    def __mul__(self, other):
-        if isinstance(other, LocalOpExpr):
-            return TermExpr(self.coeff, self.ops + [other])
-        if isinstance(other, TermExpr):
-            return TermExpr(self.coeff * other.coeff, self.ops + other.ops)
-        if isinstance(other, (int, float, complex)):
-            return TermExpr(self.coeff * complex(other), self.ops)
-        return NotImplemented
-
-    def __rmul__(self, other):
-        if isinstance(other, (int, float, complex)):
-            return TermExpr(self.coeff * complex(other), self.ops)
-        return NotImplemented
+    def __neg__(self):
+        return OpProduct(-self.coeff, self.factors)
 
     def __add__(self, other):
-        if isinstance(other, TermExpr):
-            return OpSumExpr([self, other])
-        if isinstance(other, OpSumExpr):
-            return OpSumExpr([self] + other.terms)
-        return NotImplemented
+        rhs = _expand(other)
+        if rhs is None or isinstance(other, _SCALARS):
+            return NotImplemented
+        return OpTermSum(((self.coeff, self.factors),) + rhs)
 
     def __sub__(self, other):
-        if isinstance(other, TermExpr):
-            return OpSumExpr([self, TermExpr(-other.coeff, other.ops)])
-        return NotImplemented
+        rhs = _expand(other)
+        if rhs is None or isinstance(other, _SCALARS):
+            return NotImplemented
+        return OpTermSum(((self.coeff, self.factors),) + tuple((-c, f) for c, f in rhs))
+
+    def __repr__(self):
+        return f"OpProduct({self.coeff}, {list(self.factors)})"
 
 
-class OpSumExpr:
+class OpTermSum:
+    """An ordered sum of OpProduct terms, kept as (coefficient, factors)
+    pairs; terms are never merged or reordered."""
+
+    __slots__ = ("terms",)
+
     def __init__(self, terms):
-        self.terms = list(terms)
+        self.terms = tuple((complex(c), tuple(f)) for c, f in terms)
 
     def __add__(self, other):
-        if isinstance(other, TermExpr):
-            return OpSumExpr(self.terms + [other])
-        if isinstance(other, OpSumExpr):
-            return OpSumExpr(self.terms + other.terms)
-        return NotImplemented
+        rhs = _expand(other)
+        if rhs is None or isinstance(other, _SCALARS):
+            return NotImplemented
+        return OpTermSum(self.terms + rhs)
 
     def __sub__(self, other):
-        if isinstance(other, TermExpr):
-            return OpSumExpr(self.terms + [TermExpr(-other.coeff, other.ops)])
-        if isinstance(other, OpSumExpr):
-            return OpSumExpr(self.terms + [TermExpr(-t.coeff, t.ops) for t in other.terms])
-        return NotImplemented
+        rhs = _expand(other)
+        if rhs is None or isinstance(other, _SCALARS):
+            return NotImplemented
+        return OpTermSum(self.terms + tuple((-c, f) for c, f in rhs))
+
+    def __neg__(self):
+        return OpTermSum(tuple((-c, f) for c, f in self.terms))
 
     def __mul__(self, other):
-        if isinstance(other, (int, float, complex)):
-            return OpSumExpr([t * other for t in self.terms])
-        return NotImplemented
+        rhs = _expand(other)
+        if rhs is None:
+            return NotImplemented
+        return OpTermSum(tuple((c1 * c2, f1 + f2) for c1, f1 in self.terms for c2, f2 in rhs))
 
-    __rmul__ = __mul__
+    def __rmul__(self, other):
+        if not isinstance(other, _SCALARS):
+            return NotImplemented
+        return OpTermSum(tuple((complex(other) * c, f) for c, f in self.terms))
+
+    def __len__(self):
+        return len(self.terms)
+
+    def __repr__(self):
+        return f"OpTermSum({len(self.terms)} terms)"
 
 
 def _mk(name):
     def f(i):
-        return LocalOpExpr(name, i)
+        return OpProduct(1.0, ((name, i),))
     f.__name__ = name
     return f
 
@@ -162,11 +202,9 @@ class OpSum:
                 raise ValueError("Term must be a tuple of (coeff, op1, site1, ...)")
             self.add_term(other[0], *other[1:])
             return self
-        if isinstance(other, TermExpr):
-            other = OpSumExpr([other])
-        if isinstance(other, OpSumExpr):
-            for t in other.terms:
-                self.add(t.coeff, [(o.name, o.site) for o in t.ops])
+        if isinstance(other, (OpProduct, OpTermSum)):
+            for c, fs in _expand(other):
+                self.add(c, fs)
             return self
         if isinstance(other, OpSum):
             self.terms += other.terms
@@ -351,7 +389,7 @@ class MatrixFreeHamiltonian:
     build does; the arithmetic stays float64."""
 
     def __init__(self, basis, site=None, ops=None, device="cpu", dtype=None, split_bits=-1, operator=False):
-        if ops is None and isinstance(site, (OpSum, OpSumExpr, TermExpr)):
+        if ops is None and isinstance(site, (OpSum, OpTermSum, OpProduct)):
             site, ops = None, site
         if site is None:
             site = basis.site_class()

@@ -23,6 +23,7 @@ SOURCES = [
     "qk_api.cpp",
     "qk_c_api.cpp",
     "qk_serving.cpp",
+    "qk_eigen.cpp",
 ]
 
 

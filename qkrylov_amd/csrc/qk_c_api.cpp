@@ -6,11 +6,13 @@
 #include "qkrylov/qkrylov.hpp"
 #include "qkrylov_amd.h"
 
+#include "qk_eigen.hpp"
 #include "qk_engine.hpp"
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -869,6 +871,44 @@ int qkrylov_cf_batched_f64(qkrylov_hamiltonian_h h, int nseeds, const double* se
         for (size_t k = 0; k < r[s].betas.size() && k < (size_t)n_iter; ++k) betas[(size_t)s * n_iter + k] = r[s].betas[k];
         norms[s] = r[s].norm_phi0;
         if (n_alpha) n_alpha[s] = (int)r[s].alphas.size();
+    }
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
+/* ---- host eigensolvers of the Krylov drivers (no device work) ---- */
+int qkrylov_amd_tridiag_eigen(int n, const double* alpha, const double* beta, double* evals, double* evecs, int want)
+{
+    if (n < 0 || (n > 0 && (!alpha || !evals)) || (n > 1 && !beta) || want < 0 || want > 2 ||
+        (want > 0 && n > 0 && !evecs))
+        return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    qk::symtri_eig(alpha, beta, n, evals, evecs, want);
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
+int qkrylov_amd_tridiag_lowest(int n, const double* alpha, const double* beta, double* lowest)
+{
+    if (n <= 0 || !alpha || (n > 1 && !beta) || !lowest) return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    *lowest = qk::symtri_lowest(alpha, beta, n);
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
+int qkrylov_amd_hermitian_eigen(int n, const double* a, double* evals, double* evecs)
+{
+    if (n < 0 || (n > 0 && (!a || !evals || !evecs))) return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    std::vector<std::complex<double>> A((size_t)n * n), V;
+    for (size_t i = 0; i < (size_t)n * n; ++i) A[i] = {a[2 * i], a[2 * i + 1]};
+    std::vector<double> w;
+    qk::herm_eig(n, A, w, V);
+    for (int i = 0; i < n; ++i) evals[i] = w[i];
+    for (size_t i = 0; i < (size_t)n * n; ++i) {
+        evecs[2 * i] = V[i].real();
+        evecs[2 * i + 1] = V[i].imag();
     }
     return QKRYLOV_SUCCESS;
     QK_CATCH

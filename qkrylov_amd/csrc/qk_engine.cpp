@@ -1,7 +1,8 @@
-// qk_engine.cpp -- Engine (device tables), tridiagonal QR, and the Krylov
+// qk_engine.cpp -- Engine (device tables), the tridiagonal helpers, and the Krylov
 // drivers: Lanczos ground state, plain three-term recurrences (FTLM / CF),
 // the block recurrence for batched FTLM / CF, and Davidson.
 #include "qk_engine.hpp"
+#include "qk_eigen.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -470,57 +471,7 @@ void Engine::apply_diag_operator(const double* x, bool xcplx, double* y, hipStre
     lcheck(qkl_apply_diag_terms(&T_, wide() ? 1 : 0, xcplx ? 1 : 0, x, y, s), "apply_diag_operator");
 }
 
-// ---------------------------------------------------------------- tridiagonal QR
-
-void tridiag_qr(const double* alpha, const double* beta, int n, double* d, double* z)
-{
-    std::vector<double> e(std::max(n, 1), 0.0);
-    for (int i = 0; i < n; ++i) d[i] = alpha[i];
-    for (int i = 0; i + 1 < n; ++i) e[i] = beta[i];
-    if (z) {
-        std::fill(z, z + (size_t)n * n, 0.0);
-        for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
-    }
-    for (int sweep = 0; sweep < 1000; ++sweep) {
-        for (int i = 0; i + 1 < n; ++i)
-            if (std::abs(e[i]) < 1e-14 * (std::abs(d[i]) + std::abs(d[i + 1]))) e[i] = 0.0;
-        int m = n - 1;
-        while (m > 0 && e[m - 1] == 0.0) --m;
-        if (m == 0) break;
-        int l = m - 1;
-        while (l > 0 && e[l - 1] != 0.0) --l;
-        // Wilkinson shift from the trailing 2x2 block
-        const double b = (d[m - 1] - d[m]) / 2.0;
-        const double c = e[m - 1] * e[m - 1];
-        const double r0 = std::sqrt(b * b + c);
-        const double mu = (b > 0) ? d[m] - c / (b + r0) : d[m] - c / (b - r0);
-        double p = d[l] - mu, g = e[l];
-        for (int i = l; i < m; ++i) {
-            const double r = std::hypot(p, g);
-            const double cs = p / r, sn = g / r;
-            if (i > l) e[i - 1] = r;
-            const double f = cs * d[i] + sn * e[i];
-            const double gn = cs * e[i] + sn * d[i + 1];
-            const double h = sn * d[i] - cs * e[i];
-            const double k = sn * e[i] - cs * d[i + 1];
-            d[i] = cs * f + sn * gn;
-            e[i] = cs * h + sn * k;
-            d[i + 1] = sn * h - cs * k;
-            if (z) {
-                for (int j = 0; j < n; ++j) {
-                    const double z1 = z[(size_t)j * n + i], z2 = z[(size_t)j * n + i + 1];
-                    z[(size_t)j * n + i] = cs * z1 + sn * z2;
-                    z[(size_t)j * n + i + 1] = sn * z1 - cs * z2;
-                }
-            }
-            if (i < m - 1) {
-                p = e[i];
-                g = sn * e[i + 1];
-                e[i + 1] = -cs * e[i + 1];
-            }
-        }
-    }
-}
+// ---------------------------------------------------------------- tridiagonal
 
 double tridiag_min(const double* alpha, const double* beta, int n, std::vector<double>* evec)
 {
@@ -528,22 +479,12 @@ double tridiag_min(const double* alpha, const double* beta, int n, std::vector<d
         if (evec) evec->clear();
         return 0.0;
     }
-    if (n == 1) {
-        if (evec) evec->assign(1, 1.0);
-        return alpha[0];
-    }
-    std::vector<double> d(n);
-    std::vector<double> z;
-    if (evec) z.resize((size_t)n * n);
-    tridiag_qr(alpha, beta, n, d.data(), evec ? z.data() : nullptr);
-    int mi = 0;
-    for (int i = 1; i < n; ++i)
-        if (d[i] < d[mi]) mi = i;
-    if (evec) {
-        evec->resize(n);
-        for (int i = 0; i < n; ++i) (*evec)[i] = z[(size_t)i * n + mi];
-    }
-    return d[mi];
+    if (!evec) return symtri_lowest(alpha, beta, n);
+    std::vector<double> w(n), z((size_t)n * n);
+    symtri_eig(alpha, beta, n, w.data(), z.data(), kAllVectors);
+    evec->resize(n);
+    for (int i = 0; i < n; ++i) (*evec)[i] = z[(size_t)i * n];
+    return w[0];
 }
 
 // ---------------------------------------------------------------- helpers
@@ -1043,15 +984,9 @@ void ftlm_observables(const std::vector<std::vector<double>>& alphas, const std:
     for (size_t r = 0; r < R; ++r) {
         const int n = (int)alphas[r].size();
         if (n == 0) continue;
+        // eigenvalues and first eigenvector components of the r-th tridiagonal
         std::vector<double> ev(n), fc(n);
-        if (n == 1) {
-            ev[0] = alphas[r][0];
-            fc[0] = 1.0;
-        } else {
-            std::vector<double> z((size_t)n * n);
-            tridiag_qr(alphas[r].data(), betas[r].data(), n, ev.data(), z.data());
-            for (int i = 0; i < n; ++i) fc[i] = z[i];   // z[0][i]
-        }
+        symtri_eig(alphas[r].data(), betas[r].data(), n, ev.data(), fc.data(), kFirstRow);
         for (size_t t = 0; t < nT; ++t)
             for (int i = 0; i < n; ++i) {
                 const double w = norm2[r] * fc[i] * fc[i] * std::exp(-bT[t] * ev[i]);
@@ -1089,56 +1024,6 @@ double spectral_value(const double* alphas, const double* betas, size_t n, doubl
 namespace {
 
 using cd = std::complex<double>;
-
-// Jacobi rotations for a small Hermitian matrix: the same rotation sequence
-// as the reference's solve_small_hermitian (src/solvers/davidson.cpp:22-82)
-void small_hermitian(int n, std::vector<cd> A, std::vector<double>& ev, std::vector<std::vector<cd>>& vecs)
-{
-    std::vector<cd> V((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
-    auto a = [&](int i, int j) -> cd& { return A[(size_t)i * n + j]; };
-    auto v = [&](int i, int j) -> cd& { return V[(size_t)i * n + j]; };
-    for (int it = 0; it < 1000; ++it) {
-        double mo = 0.0;
-        int p = 0, q = 0;
-        for (int i = 0; i < n; ++i)
-            for (int j = i + 1; j < n; ++j)
-                if (std::abs(a(i, j)) > mo) {
-                    mo = std::abs(a(i, j));
-                    p = i;
-                    q = j;
-                }
-        if (mo < 1e-15) break;
-        const cd app = a(p, p), aqq = a(q, q), apq = a(p, q);
-        const double phi = 0.5 * std::atan2(2.0 * std::abs(apq), std::real(app - aqq));
-        const cd c = std::cos(phi);
-        const cd s = std::sin(phi) * apq / std::abs(apq);
-        for (int i = 0; i < n; ++i) {
-            const cd vip = v(i, p), viq = v(i, q);
-            v(i, p) = c * vip + s * viq;
-            v(i, q) = -std::conj(s) * vip + c * viq;
-            if (i != p && i != q) {
-                const cd aip = a(i, p), aiq = a(i, q);
-                a(i, p) = a(p, i) = c * aip + s * aiq;
-                a(i, q) = a(q, i) = -std::conj(s) * aip + c * aiq;
-            }
-        }
-        a(p, p) = c * c * app + s * std::conj(s) * aqq + 2.0 * std::real(c * s * std::conj(apq));
-        a(q, q) = s * std::conj(s) * app + c * c * aqq - 2.0 * std::real(c * s * std::conj(apq));
-        a(p, q) = a(q, p) = 0.0;
-    }
-    std::vector<int> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return std::real(a(i, i)) < std::real(a(j, j)); });
-    ev.clear();
-    vecs.clear();
-    for (int i : idx) {
-        ev.push_back(std::real(a(i, i)));
-        std::vector<cd> col(n);
-        for (int j = 0; j < n; ++j) col[j] = v(j, i);
-        vecs.push_back(col);
-    }
-}
 
 } // namespace
 
@@ -1191,9 +1076,13 @@ DavidsonOutput davidson_run(const Engine& E, int n_eig, int max_subspace, double
                 Hs[(size_t)i * m + j] = dotc(V[i].d(), HV[j].d());
                 if (i != j) Hs[(size_t)j * m + i] = std::conj(Hs[(size_t)i * m + j]);
             }
+        // Ritz pairs of the projection: es[k][i] = component i of the k-th
         std::vector<double> ev;
-        std::vector<std::vector<cd>> es;
-        small_hermitian(m, Hs, ev, es);
+        std::vector<cd> Vs;
+        herm_eig(m, Hs, ev, Vs);
+        std::vector<std::vector<cd>> es(m, std::vector<cd>(m));
+        for (int k = 0; k < m; ++k)
+            for (int i = 0; i < m; ++i) es[k][i] = Vs[(size_t)i * m + k];
         bool all_conv = true;
         std::vector<DevBuf> corr;
         for (int k = 0; k < n_eig; ++k) {

@@ -169,11 +169,10 @@ struct RealCore {
 };
 
 // ---------------------------------------------------------------- tridiagonal
-// Implicit symmetric QR with Wilkinson shift, deflation test
-// |e_i| < 1e-14 (|d_i| + |d_{i+1}|), at most 1000 sweeps; the same steps as
-// the reference's tridiag_ground_state_full (src/solvers/lanczos.cpp:25-98).
-// evals: all eigenvalues in QR order; z (n*n row-major, optional).
-void tridiag_qr(const double* alpha, const double* beta, int n, double* d, double* z);
+// Lowest eigenvalue of the Lanczos tridiagonal (alpha, beta) and, when evec
+// is given, its unit eigenvector: the quantities of the reference's
+// tridiag_ground_state_full (src/solvers/lanczos.cpp:25-98), computed by
+// Sturm bisection (value only) or implicit QL (qk_eigen.hpp).
 double tridiag_min(const double* alpha, const double* beta, int n, std::vector<double>* evec);
 
 // ---------------------------------------------------------------- Lanczos
