@@ -394,6 +394,25 @@ FTLMGridResult ftlm_batched(const MatrixFreeHamiltonianBase& H, int R, int M, ui
 std::vector<DynamicsResult> continued_fraction_batched(const MatrixFreeHamiltonianBase& H, const std::vector<HostVector>& seeds,
                                                        int n_iter, double* seconds = nullptr);
 
+// Dynamical structure factor driver: for each operator O_q (built with
+// operator semantics, y = O_q x, e.g. S^z_q = N^-1/2 sum_r e^{iq.r} S^z_r),
+// the seed phi_q = O_q psi0 is formed on the device, all seeds run as one
+// block continued fraction of n_iter steps (a complex seed is two real
+// columns), and S_q(omega) = -1/pi Im <phi_q| (omega + E0 + i eta - H)^-1
+// |phi_q> is evaluated on the grid. The reference leaves seed building to
+// the caller (tests/test_dynamics.cpp:41-46) and provides the two halves as
+// continued_fraction_coeffs / evaluate_spectral_function
+// (solvers/dynamics.hpp:20-38).
+struct StructureFactorResult {
+    std::vector<DynamicsResult> cf;               // per operator
+    std::vector<std::vector<Real>> spectrum;      // spectrum[q][w]
+    double seconds = 0.0;                          // block recurrence wall time
+};
+StructureFactorResult structure_factor(const MatrixFreeHamiltonianBase& H,
+                                       const std::vector<const MatrixFreeHamiltonianBase*>& operators,
+                                       const HostVector& psi0, Real E0, int n_iter, const std::vector<Real>& omega,
+                                       Real eta);
+
 // The reference's solver templates (solvers/lanczos.hpp:20-24,
 // davidson.hpp:21-27, dynamics.hpp:21-26, ftlm.hpp:21-27) with their
 // defaults, forwarding to the engine-backed functions above.

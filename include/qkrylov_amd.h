@@ -167,6 +167,22 @@ int qkrylov_ftlm_batched_f64(qkrylov_hamiltonian_h h, int R, int M, uint64_t see
 int qkrylov_cf_batched_f64(qkrylov_hamiltonian_h h, int nseeds, const double* seeds_complex, int n_iter,
                            double* alphas, double* betas, double* norms, int* n_alpha, double* seconds);
 
+/* ---- dynamical structure factor S_q(omega) for nops operators O_q (handles
+ * from qkrylov_operator_create, e.g. S^z_q = N^-1/2 sum_r e^{iq.r} S^z_r):
+ * seeds phi_q = O_q psi0 formed on the device, one block continued fraction
+ * of n_iter steps over all seeds, then
+ * S_q(omega_w) = -1/pi Im <phi_q|(omega_w + E0 + i eta - H)^-1|phi_q>.
+ * psi0: host, dim doubles (psi0_complex = 0) or 2*dim interleaved.
+ * spectrum: nops*nomega (row q at q*nomega); alphas/betas (optional):
+ * nops*n_iter; norms, n_alpha (optional): nops. The reference has no such
+ * driver: it builds phi0 by hand (tests/test_dynamics.cpp:41-46) and calls
+ * continued_fraction_coeffs / evaluate_spectral_function
+ * (include/qkrylov/solvers/dynamics.hpp:20-38) per seed. ---- */
+int qkrylov_structure_factor_f64(qkrylov_hamiltonian_h h, const qkrylov_hamiltonian_h* operators, int nops,
+                                 const double* psi0, int psi0_complex, double E0, int n_iter, const double* omega,
+                                 int nomega, double eta, double* spectrum, double* alphas, double* betas,
+                                 double* norms, int* n_alpha, double* seconds);
+
 /* ---- serving loop (continuous batching of ground-state and S^zz(omega)
  * solves of XXZ rings): see qkrylov_amd/csrc/qk_serving.cpp ---- */
 typedef struct {

@@ -168,6 +168,8 @@ SIGNATURES = {
     "qkrylov_cf_batched_f64": (c_int, [c_vp, c_int, p_dbl, c_int, p_dbl, p_dbl, p_dbl, p_int, p_dbl]),
     "qkrylov_serve_run": (c_int, [P(ServeRequest), c_int, P(ServeOptions), P(ServeResult), p_dbl, p_dbl, c_int,
                                   p_dbl]),
+    "qkrylov_structure_factor_f64": (c_int, [c_vp, P(c_vp), c_int, p_dbl, c_int, c_double, c_int, p_dbl, c_int,
+                                             c_double, p_dbl, p_dbl, p_dbl, p_dbl, p_int, p_dbl]),
     # host eigensolvers
     "qkrylov_amd_tridiag_eigen": (c_int, [c_int, p_dbl, p_dbl, p_dbl, p_dbl, c_int]),
     "qkrylov_amd_tridiag_lowest": (c_int, [c_int, p_dbl, p_dbl, p_dbl]),

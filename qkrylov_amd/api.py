@@ -669,6 +669,51 @@ def continued_fraction_batched(H, seeds, n_iter=100):
     return out, sec.value
 
 
+class StructureFactorResult:
+    def __init__(self, omega, spectrum, cf, seconds):
+        self.omega, self.spectrum, self.cf, self.seconds = omega, spectrum, cf, seconds
+
+    def __repr__(self):
+        return f"StructureFactorResult(operators={len(self.cf)}, omega points={len(self.omega)})"
+
+
+def structure_factor(H, psi0, operators, omega, E0, eta=0.1, n_iter=100):
+    """S_q(omega) = -1/pi Im <phi_q|(omega + E0 + i eta - H)^-1|phi_q> with
+    phi_q = O_q psi0 for every operator in `operators` (OpSum, operator
+    expressions, or MatrixFreeHamiltonian handles built with operator=True;
+    O_q acts as written, not as its adjoint). The seeds are formed on the
+    device and run as one block continued fraction of n_iter steps. Returns
+    spectrum[q, w] and the continued fractions (DynamicsResult per q)."""
+    L = _lib.load()
+    handles, keep = [], []
+    for o in operators:
+        if isinstance(o, MatrixFreeHamiltonian):
+            op = o
+        else:
+            op = MatrixFreeHamiltonian(H.basis, H.site, o, operator=True)
+        keep.append(op)
+        handles.append(op._h)
+    nq = len(handles)
+    if nq == 0:
+        raise ValueError("no operators given")
+    psi = np.ascontiguousarray(psi0, dtype=np.complex128)
+    if psi.ndim != 1 or psi.shape[0] != H.dimension:
+        raise ValueError(f"psi0 has shape {psi.shape}, expected ({H.dimension},)")
+    om = np.ascontiguousarray(omega, dtype=np.float64).ravel()
+    spec = np.zeros((nq, max(len(om), 1)))
+    al, be = np.zeros(nq * n_iter), np.zeros(nq * n_iter)
+    nrm = np.zeros(nq)
+    na = np.zeros(nq, dtype=np.int32)
+    sec = ctypes.c_double()
+    harr = (ctypes.c_void_p * nq)(*handles)
+    check(L.qkrylov_structure_factor_f64(H._h, harr, nq, _dp(psi.view(np.float64)), 1, float(E0), int(n_iter),
+                                         _dp(om), len(om), float(eta), _dp(spec), _dp(al), _dp(be), _dp(nrm),
+                                         na.ctypes.data_as(_lib.p_int), ctypes.byref(sec)), "structure_factor")
+    al, be = al.reshape(nq, n_iter), be.reshape(nq, n_iter)
+    cf = [DynamicsResult(al[q, :na[q]].copy(), be[q, :max(na[q] - 1, 0)].copy(), nrm[q]) for q in range(nq)]
+    return StructureFactorResult(om, spec[:, :len(om)], cf, sec.value)
+
+
 # ---------------------------------------------------------------- device memory
 class DeviceArray:
     """Minimal device buffer of float64 owned by the engine's allocator."""

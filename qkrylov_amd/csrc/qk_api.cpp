@@ -410,5 +410,68 @@ std::vector<DynamicsResult> continued_fraction_batched(const MatrixFreeHamiltoni
     return out;
 }
 
+StructureFactorResult structure_factor(const MatrixFreeHamiltonianBase& H,
+                                       const std::vector<const MatrixFreeHamiltonianBase*>& ops,
+                                       const HostVector& psi0, Real E0, int n_iter, const std::vector<Real>& omega,
+                                       Real eta)
+{
+    StructureFactorResult out;
+    const size_t D = H.dimension();
+    if (psi0.size() != D) throw std::invalid_argument("psi0 has the wrong length");
+    for (const auto* op : ops) {
+        if (!op) throw std::invalid_argument("null operator");
+        if (op->dimension() != H.dimension()) throw std::invalid_argument("operator acts on another basis");
+    }
+    if (n_iter <= 0) throw std::invalid_argument("n_iter must be positive");
+    if (D == 0 || ops.empty()) return out;
+    const auto& E = *H.engine();
+    hipStream_t s = qk::default_stream();
+    DevBuf psi(2 * D * sizeof(double)), tmp(2 * D * sizeof(double));
+    hip_check(hipMemcpyAsync(psi.get(), psi0.data(), 2 * D * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    auto lc = [](int rc, const char* what) {
+        if (rc != 0) throw std::runtime_error(std::string("kernel launch failed: ") + what);
+    };
+    if (E.is_real()) {
+        // up to 32 complex seeds per block (64 real columns)
+        for (size_t g0 = 0; g0 < ops.size(); g0 += 32) {
+            const int ng = (int)std::min<size_t>(32, ops.size() - g0), nc = 2 * ng;
+            DevBuf X(D * (size_t)nc * sizeof(double));
+            for (int g = 0; g < ng; ++g) {
+                ops[g0 + g]->engine()->apply(psi.d(), tmp.d(), true, s);
+                lc(qkl_copy_strided(D, 2, tmp.d(), 2, X.d() + 2 * g, (size_t)nc, s), "pack seed");
+            }
+            double sec = 0.0;
+            auto bt = qk::block_tridiag_run(E, X.d(), nc, 2, n_iter, s, &sec);
+            out.seconds += sec;
+            for (int g = 0; g < ng; ++g) {
+                DynamicsResult d;
+                if (bt.norm0[g] >= 1e-15) {
+                    d.alphas = bt.alphas[g];
+                    d.betas = bt.betas[g];
+                    d.norm_phi0 = bt.norm0[g];
+                }
+                out.cf.push_back(std::move(d));
+            }
+        }
+    } else {
+        HostVector seed(D);
+        for (const auto* op : ops) {
+            op->engine()->apply(psi.d(), tmp.d(), true, s);
+            hip_check(hipMemcpyAsync(seed.data(), tmp.get(), 2 * D * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+            hip_check(hipStreamSynchronize(s), "sync");
+            const double t0 = qk::now_s();
+            out.cf.push_back(continued_fraction_coeffs(H, seed, n_iter));
+            out.seconds += qk::now_s() - t0;
+        }
+    }
+    for (const auto& d : out.cf) {
+        std::vector<Real> sp(omega.size());
+        for (size_t w = 0; w < omega.size(); ++w)
+            sp[w] = qk::spectral_value(d.alphas.data(), d.betas.data(), d.alphas.size(), d.norm_phi0, omega[w], E0, eta);
+        out.spectrum.push_back(std::move(sp));
+    }
+    return out;
+}
+
 } // namespace fp64
 } // namespace qkrylov

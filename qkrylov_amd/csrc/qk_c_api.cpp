@@ -876,6 +876,38 @@ int qkrylov_cf_batched_f64(qkrylov_hamiltonian_h h, int nseeds, const double* se
     QK_CATCH
 }
 
+int qkrylov_structure_factor_f64(qkrylov_hamiltonian_h h, const qkrylov_hamiltonian_h* operators, int nops,
+                                 const double* psi0, int psi0_complex, double E0, int n_iter, const double* omega,
+                                 int nomega, double eta, double* spectrum, double* alphas, double* betas,
+                                 double* norms, int* n_alpha, double* seconds)
+{
+    const auto* H = H_of(h);
+    if (!H || !operators || nops <= 0 || !psi0 || n_iter <= 0 || nomega < 0 || (nomega > 0 && (!omega || !spectrum)))
+        return QKRYLOV_ERROR_INVALID_ARG;
+    std::vector<const MatrixFreeHamiltonianBase*> ops(nops);
+    for (int q = 0; q < nops; ++q) {
+        ops[q] = H_of(operators[q]);
+        if (!ops[q]) return QKRYLOV_ERROR_INVALID_ARG;
+    }
+    QK_TRY
+    const size_t D = H->dimension();
+    HostVector p(D);
+    for (size_t i = 0; i < D; ++i) p[i] = psi0_complex ? Complex(psi0[2 * i], psi0[2 * i + 1]) : Complex(psi0[i], 0.0);
+    std::vector<Real> om(omega, omega + nomega);
+    auto r = structure_factor(*H, ops, p, E0, n_iter, om, eta);
+    for (int q = 0; q < nops; ++q) {
+        const auto& d = r.cf[q];
+        for (int w = 0; w < nomega; ++w) spectrum[(size_t)q * nomega + w] = r.spectrum[q][w];
+        if (alphas) for (size_t k = 0; k < d.alphas.size() && k < (size_t)n_iter; ++k) alphas[(size_t)q * n_iter + k] = d.alphas[k];
+        if (betas) for (size_t k = 0; k < d.betas.size() && k < (size_t)n_iter; ++k) betas[(size_t)q * n_iter + k] = d.betas[k];
+        if (norms) norms[q] = d.norm_phi0;
+        if (n_alpha) n_alpha[q] = (int)d.alphas.size();
+    }
+    if (seconds) *seconds = r.seconds;
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
 /* ---- host eigensolvers of the Krylov drivers (no device work) ---- */
 int qkrylov_amd_tridiag_eigen(int n, const double* alpha, const double* beta, double* evals, double* evecs, int want)
 {

@@ -1058,6 +1058,18 @@ __global__ __launch_bounds__(QK_TPB) void k_copy(size_t n, const double* __restr
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[n - 1] = src[n - 1];
 }
 
+// dst[r * dst_ld + j] = src[r * src_ld + j] for r < nrows, j < width (packs
+// vectors into columns of a row-major block and back)
+__global__ __launch_bounds__(QK_TPB) void k_copy_strided(size_t nrows, int width, const double* __restrict__ src,
+                                                          size_t src_ld, double* __restrict__ dst, size_t dst_ld)
+{
+    const size_t n = nrows * (size_t)width;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i / (size_t)width, j = i - r * (size_t)width;
+        dst[r * dst_ld + j] = src[r * src_ld + j];
+    }
+}
+
 // out[idx] = total of n partial sums (sqrt of it when root), in the order
 // of block_total
 __global__ __launch_bounds__(64) void k_total(const double* __restrict__ p, int n, int root, double* __restrict__ out,
@@ -1649,6 +1661,15 @@ int qkl_fill_random(int kind, size_t nrows, uint64_t seed, uint64_t counter0, in
 int qkl_copy(size_t n, const double* src, double* dst, qkl_stream sv)
 {
     hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, src, dst);
+    return rc(hipGetLastError());
+}
+
+int qkl_copy_strided(size_t nrows, int width, const double* src, size_t src_ld, double* dst, size_t dst_ld,
+                     qkl_stream sv)
+{
+    if (width <= 0 || (size_t)width > src_ld || (size_t)width > dst_ld) return -1;
+    hipLaunchKernelGGL(k_copy_strided, dim3(grid_for(nrows * (size_t)width)), dim3(QK_TPB), 0, (hipStream_t)sv, nrows,
+                       width, src, src_ld, dst, dst_ld);
     return rc(hipGetLastError());
 }
 

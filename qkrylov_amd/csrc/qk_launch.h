@@ -83,6 +83,9 @@ int qkl_fill_zero(size_t n_doubles, double* x, qkl_stream s);
 /* *p = v on the stream (a one-thread kernel: no host staging, graph-safe) */
 int qkl_set_int(int* p, int v, qkl_stream s);
 int qkl_copy(size_t n_doubles, const double* src, double* dst, qkl_stream s);
+/* dst[r * dst_ld + j] = src[r * src_ld + j], r < nrows, j < width <= both ld */
+int qkl_copy_strided(size_t nrows, int width, const double* src, size_t src_ld, double* dst, size_t dst_ld,
+                     qkl_stream s);
 
 /* h[j] partials: partials[j * npart + b] = sum_i V[j][i] u[i], j < m (real) */
 int qkl_multidot(size_t n, int m, const double* V, size_t ldv, const double* u, double* partials, qkl_stream s);

@@ -1,0 +1,124 @@
+"""The S(q, omega) driver (qkrylov_structure_factor_f64 / structure_factor):
+seeds phi_q = S^z_q psi0 formed on the device, one block continued fraction
+over all q, and the spectrum on an omega grid. Checked against the dense
+resolvent -1/pi Im <phi_q|(omega + E0 + i eta - H)^-1|phi_q> of the same seed
+(the continued fraction of n >= the sector dimension reproduces it up to
+rounding), against the oracle's continued fraction on the same seed, and
+against seeds built by hand through the operator apply."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import qkrylov_amd as qa
+from qkrylov_amd import models
+
+pytestmark = pytest.mark.gpu
+
+TOL_S = 1e-8   # north-star tolerance for S(omega) observables
+
+
+def dense_spectrum(Hd, phi, omega, E0, eta):
+    w, V = np.linalg.eigh(Hd)
+    c2 = np.abs(V.conj().T @ phi) ** 2
+    return np.array([-1.0 / math.pi * np.imag(np.sum(c2 / (om + E0 + 1j * eta - w))) for om in omega])
+
+
+def tj_setup(nup, ndn):
+    terms = models.tj_model(4, 5, 1.0, 0.4, periodic=False)
+    sec = dict(use_nup=1, nup=nup, use_ndn=1, ndn=ndn)
+    om = O.Model(O.TJ, 20, O.sector(**sec), O.TJ, terms, enumerate=False)
+    H = qa.MatrixFreeHamiltonian(qa.TJBasis(20, nup=nup, ndn=ndn), qa.TJSite(), qa.OpSum(terms))
+    assert H.dimension == om.dim
+    return H, om
+
+
+def test_structure_factor_tj4x5_all_q_vs_dense_resolvent():
+    """t-J on the 4x5 open lattice of the C4 workload, one up and one down
+    electron (D = 380): all 20 plane-wave S^z_q, 100 steps, eta = 0.05."""
+    H, om = tj_setup(1, 1)
+    Hd = om.dense()
+    E0 = np.linalg.eigvalsh(Hd)[0]
+    g = qa.lanczos_ground_state(H, maxiter=300, tol=1e-13, return_state=True)
+    assert abs(g.energy - E0) < 1e-10 * abs(E0)
+    psi = np.asarray(g.eigenvector, dtype=np.complex128)
+    qs = models.lattice_momenta(4, 5)
+    ops = [qa.OpSum(models.szq_terms(4, 5, q)) for q in qs]
+    omega = np.linspace(-0.5, 4.0, 46)
+    r = qa.structure_factor(H, psi, ops, omega, E0, eta=0.05, n_iter=100)
+    assert r.spectrum.shape == (20, 46) and len(r.cf) == 20
+    for k, q in enumerate(qs):
+        oq = O.Model(O.TJ, 20, O.sector(use_nup=1, nup=1, use_ndn=1, ndn=1), O.TJ, models.szq_terms(4, 5, q),
+                     enumerate=False)
+        # the oracle stores conj(amp) per row: for a diagonal operator O psi
+        # is conj(diag) * psi
+        phi = np.conj(np.diag(oq.dense())) * psi
+        ex = dense_spectrum(Hd, phi, omega, E0, 0.05)
+        err = np.max(np.abs(r.spectrum[k] - ex) / np.maximum(1.0, np.abs(ex)))
+        assert err < TOL_S, (q, err)
+        assert abs(r.cf[k].norm_phi0 - np.linalg.norm(phi)) < 1e-12 * max(1.0, np.linalg.norm(phi))
+        # the first steps agree with the oracle's recurrence on the same seed
+        al, be, nrm = om.cf(phi, 20)
+        n = min(20, len(al), len(r.cf[k].alphas))
+        assert np.allclose(r.cf[k].alphas[:n], al[:n], rtol=1e-9, atol=1e-9)
+    # q = 0: S^z_total commutes with H; psi0 has Sz = 0, so the seed vanishes
+    assert r.cf[0].norm_phi0 < 1e-12 and np.max(np.abs(r.spectrum[0])) < 1e-12
+
+
+def test_structure_factor_equals_hand_built_seeds():
+    """The driver's device-built seeds give the same block recurrence as
+    seeds formed by the operator apply on the host side and passed to the
+    batched continued fraction (same columns, same kernels: same numbers)."""
+    H, om = tj_setup(2, 1)
+    g = qa.lanczos_ground_state(H, maxiter=300, tol=1e-12, reorthogonalize=False, return_state=True)
+    psi = np.asarray(g.eigenvector, dtype=np.complex128)
+    qs = models.lattice_momenta(4, 5)[1:7]
+    basis = qa.TJBasis(20, nup=2, ndn=1)
+    seeds = []
+    for q in qs:
+        op = qa.MatrixFreeHamiltonian(basis, qa.TJSite(), qa.OpSum(models.szq_terms(4, 5, q)), operator=True)
+        seeds.append(op.apply(psi))
+    cf, _ = qa.continued_fraction_batched(H, np.array(seeds), 60)
+    omega = [0.1, 0.7, 1.9]
+    r = qa.structure_factor(H, psi, [models.szq_terms(4, 5, q) for q in qs], omega, g.energy, eta=0.05, n_iter=60)
+    for k in range(len(qs)):
+        assert np.array_equal(r.cf[k].alphas, cf[k].alphas)
+        assert np.array_equal(r.cf[k].betas, cf[k].betas)
+        for j, w in enumerate(omega):
+            assert r.spectrum[k, j] == qa.evaluate_spectral_function(cf[k], w, g.energy, 0.05)
+
+
+def test_structure_factor_spin_ring_sum_rule_and_resolvent():
+    """Heisenberg ring L = 10 (D = 252): S^zz(q, omega) for every q against
+    the dense resolvent; the total weight equals |phi_q|^2 (the norm the
+    continued fraction carries)."""
+    L = 10
+    terms = models.heisenberg_chain(L)
+    om = O.Model(O.SPIN, L, O.sector(use_sz=1, sz2=0), O.SPIN, terms)
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(L, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms))
+    Hd = om.dense()
+    w, V = np.linalg.eigh(Hd)
+    E0, psi = w[0], V[:, 0]
+    qs = [(2 * math.pi * k / L, 0.0) for k in range(L)]
+    ops = [models.szq_terms(1, L, q, two_species=False) for q in qs]
+    omega = np.linspace(0.0, 3.5, 36)
+    r = qa.structure_factor(H, psi, ops, omega, E0, eta=0.1, n_iter=260)
+    for k, q in enumerate(qs):
+        oq = O.Model(O.SPIN, L, O.sector(use_sz=1, sz2=0), O.SPIN, ops[k])
+        phi = np.conj(np.diag(oq.dense())) * psi
+        ex = dense_spectrum(Hd, phi, omega, E0, 0.1)
+        assert np.max(np.abs(r.spectrum[k] - ex) / np.maximum(1.0, np.abs(ex))) < TOL_S
+        assert abs(r.cf[k].norm_phi0 ** 2 - np.vdot(phi, phi).real) < 1e-12
+
+
+def test_structure_factor_rejects_bad_input():
+    H, _ = tj_setup(1, 1)
+    with pytest.raises(ValueError):
+        qa.structure_factor(H, np.zeros(H.dimension + 1), [models.szq_terms(4, 5, (0.0, 0.0))], [0.0], 0.0)
+    with pytest.raises(ValueError):
+        qa.structure_factor(H, np.zeros(H.dimension), [], [0.0], 0.0)
+    other = qa.MatrixFreeHamiltonian(qa.TJBasis(20, nup=2, ndn=1), qa.TJSite(),
+                                     qa.OpSum(models.szq_terms(4, 5, (0.0, 0.0))), operator=True)
+    with pytest.raises(RuntimeError):
+        qa.structure_factor(H, np.zeros(H.dimension), [other], [0.0], 0.0)
