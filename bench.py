@@ -129,12 +129,13 @@ def cpu_baseline(name, terms, D, budget_s):
                 rows_per_s=round(rate_rows, 1))
 
 
-def bench_ftlm(args):
+def ftlm_leg(args, with_cf=True):
     """Batched FTLM + S^zz(q, omega) on the t-J 4x5 open lattice (2 holes):
     R = 64 Rademacher vectors, M = 100 steps, 64 temperatures T = 0.05..5,
     one multi-vector recurrence vs the same 64 vectors run one at a time;
-    then the ground state and 20 plane-wave S^z_q seeds as one 40-column
-    block continued fraction (100 steps, eta = 0.05)."""
+    then the ground state and the 20 plane-wave S^z_q seeds through the
+    structure-factor driver (one 40-column block continued fraction, 100
+    steps, eta = 0.05)."""
     import numpy as np
 
     import qkrylov_amd as qa
@@ -162,67 +163,151 @@ def bench_ftlm(args):
     rel = None
     if seq is not None:
         rel = max(float(np.max(np.abs(b[k] - seq[k]) / np.maximum(np.abs(seq[k]), 1e-300))) for k in ("Z", "E", "Cv"))
-    # ground state and S^zz(q, omega) seeds
-    g = qa.lanczos_ground_state(Hs, maxiter=300, tol=1e-10, reorthogonalize=False, return_state=True)
-    psi = g.eigenvector
-    basis = qa.TJBasis(20, nup=9, ndn=9)
-    seeds = []
-    for q in models.lattice_momenta(4, 5):
-        op = qa.MatrixFreeHamiltonian(basis, qa.TJSite(), qa.OpSum(models.szq_terms(4, 5, q)), operator=True)
-        seeds.append(op.apply(psi + 0j))
-        del op
-    cf, cf_s = qa.continued_fraction_batched(H, np.array(seeds), 100)
-    omega = np.linspace(0.0, 4.0, 201)
-    S = [[qa.evaluate_spectral_function(r, w, g.energy, 0.05) for w in omega] for r in cf]
-    log(f"[bench-ftlm] E0={g.energy:.10f} ({g.iterations} it), CF 20 q in {cf_s:.3f}s")
     out = {
-        "metric": "batched FTLM solves/s (R=64 vectors x M=100 steps, FP64)",
-        "value": round(R / b["seconds"], 3), "unit": "random-vector solves/s", "n_gpus": 1, "steps": M,
-        "warmup": 0, "ms_per_step": round(1e3 * b["seconds"] / M, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (Rademacher vectors from the counter RNG)",
-        "config": {"workload": WORKLOADS["tj4x5"]["desc"], "D": D, "R": R, "M": M, "temperatures": 64,
-                   "NL": H.info["NL"], "tuning": H.tuning(), "autotune": tune},
+        "workload": WORKLOADS["tj4x5"]["desc"] + ", FTLM R=64 Rademacher vectors, M=100, 64 T in 0.05..5",
+        "D": D, "R": R, "M": M, "temperatures": 64, "NL": H.info["NL"], "tuning": H.tuning(), "autotune": tune,
+        "solves_per_s": round(R / b["seconds"], 3),
         "batched_seconds": round(b["seconds"], 4),
         "sequential_seconds": round(seq["seconds"], 4) if seq else None,
         "speedup_vs_sequential": round(seq["seconds"] / b["seconds"], 3) if seq else None,
         "max_rel_diff_batched_vs_sequential": rel,
         "Z_E_Cv_at_T=0.05,1,5": [[float(b[k][i]) for k in ("Z", "E", "Cv")] for i in (0, 12, 63)],
-        "ground_state": {"energy": g.energy, "iterations": g.iterations, "seconds": g.report["loop_seconds"]},
-        "szz_cf": {"q_points": 20, "steps": 100, "seconds": round(cf_s, 4), "eta": 0.05,
-                   "S_peak_q_pi": float(max(S[len(S) // 2]))},
         "roofline": {"bound": "hbm", "achieved": round(16.0 * D * R * M / b["seconds"] / 1e9, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(16.0 * D * R * M / b["seconds"] / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
                      "note": "block recurrence wall time incl. reductions; 16 B x R per row per step"},
     }
-    print(json.dumps(out), flush=True)
+    if with_cf:
+        g = qa.lanczos_ground_state(Hs, maxiter=300, tol=1e-10, reorthogonalize=False, return_state=True)
+        psi = g.eigenvector
+        ops = [qa.OpSum(models.szq_terms(4, 5, q)) for q in models.lattice_momenta(4, 5)]
+        omega = np.linspace(0.0, 4.0, 201)
+        t0 = time.perf_counter()
+        sf = qa.structure_factor(H, psi, ops, omega, g.energy, eta=0.05, n_iter=100)
+        t_sf = time.perf_counter() - t0
+        log(f"[bench-ftlm] E0={g.energy:.10f} ({g.iterations} it), S(q, omega) 20 q in {sf.seconds:.3f}s "
+            f"({t_sf:.3f}s with seeds and spectra)")
+        out["ground_state"] = {"energy": g.energy, "iterations": g.iterations,
+                               "seconds": round(g.report["loop_seconds"], 4)}
+        out["szz_structure_factor"] = {"q_points": 20, "steps": 100, "eta": 0.05, "omega_points": 201,
+                                       "block_cf_seconds": round(sf.seconds, 4),
+                                       "driver_seconds": round(t_sf, 4),
+                                       "S_peak_q_pi": float(np.max(sf.spectrum[len(ops) // 2]))}
+    return out
 
 
-def bench_serve(args):
+def bench_ftlm(args):
+    o = ftlm_leg(args)
+    res = {
+        "metric": "batched FTLM solves/s (R=64 vectors x M=100 steps, FP64)",
+        "value": o["solves_per_s"], "unit": "random-vector solves/s", "n_gpus": 1, "steps": o["M"],
+        "warmup": 0, "ms_per_step": round(1e3 * o["batched_seconds"] / o["M"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Rademacher vectors from the counter RNG)",
+        "config": {"workload": o["workload"], "D": o["D"], "R": o["R"], "M": o["M"], "temperatures": 64,
+                   "NL": o["NL"], "tuning": o["tuning"], "autotune": o["autotune"]},
+    }
+    res.update({k: v for k, v in o.items() if k not in ("workload", "D", "R", "M", "NL", "tuning", "autotune")})
+    print(json.dumps(res), flush=True)
+
+
+def serve_leg(args):
     """Serving stream of XXZ-ring solves: Poisson arrivals at 50 req/s,
     L in 12..28, Delta in [0.5, 1.5], |Sz| <= 2, 70 % ground states
-    (tol 1e-10), 30 % ground state + S^zz(pi, omega) (100 steps)."""
+    (tol 1e-10), 30 % ground state + S^zz(pi, omega) (100 steps); then the
+    same requests all present at t = 0 (capacity)."""
     from qkrylov_amd import serving
     reqs = serving.make_trace(args.serve_n, rate=50.0, seed=2024)
     out, tot = serving.run(reqs, realtime=True, max_active=args.serve_active)
     log(f"[bench-serve] {len(reqs)} requests: {tot}")
-    # capacity: the same requests all present at t = 0
     _, sat = serving.run(reqs, realtime=False, max_active=args.serve_active)
     log(f"[bench-serve] saturated: {sat}")
-    res = {
-        "metric": "serving solves/s with p50/p99 latency (FP64)",
-        "value": round(tot["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1, "steps": len(reqs), "warmup": 0,
-        "ms_per_step": round(1e3 * tot["wall_s"] / len(reqs), 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic seeded Poisson trace",
-        "config": {"workload": "XXZ ring serving stream, 50 req/s Poisson, L 12..28", "requests": len(reqs),
-                   "max_active": args.serve_active},
+    return {
+        "workload": "XXZ ring serving stream, 50 req/s Poisson, L 12..28, Delta 0.5..1.5, |Sz| <= 2, "
+                    "70% ground state / 30% + S^zz CF (100 steps)",
+        "requests": len(reqs), "max_active": args.serve_active,
+        "solves_per_s": round(tot["solves_per_s"], 3),
         "p50_latency_s": round(tot["p50_s"], 5), "p99_latency_s": round(tot["p99_s"], 5),
         "wall_s": round(tot["wall_s"], 3),
         "converged": int(sum(o["converged"] for o in out)),
         "saturated": {"solves_per_s": round(float(sat["solves_per_s"]), 3), "p50_latency_s": round(float(sat["p50_s"]), 5),
                       "p99_latency_s": round(float(sat["p99_s"]), 5), "wall_s": round(float(sat["wall_s"]), 3)},
     }
+
+
+def bench_serve(args):
+    o = serve_leg(args)
+    res = {
+        "metric": "serving solves/s with p50/p99 latency (FP64)",
+        "value": o["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": o["requests"], "warmup": 0,
+        "ms_per_step": round(1e3 * o["wall_s"] / o["requests"], 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic seeded Poisson trace",
+        "config": {"workload": o["workload"], "requests": o["requests"], "max_active": o["max_active"]},
+    }
+    res.update({k: v for k, v in o.items() if k not in ("workload", "requests", "max_active")})
     print(json.dumps(res), flush=True)
+
+
+def lanczos_c1_leg():
+    """Plain Lanczos ground state of the Heisenberg ring L = 20 (D = 184,756):
+    start uniform[-1,1) from seed 1234, tol 1e-12 on |E_k - E_{k-1}|, at most
+    300 iterations; without reorthogonalisation (graph replay) and with the
+    reference's full reorthogonalisation. Energies against the Bethe ansatz."""
+    import qkrylov_amd as qa
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import textbook as TB
+    H, _ = make_hamiltonian("heis_ring20")
+    e_bethe = float(TB.bethe_xxx_ring(20))
+    res = {"workload": WORKLOADS["heis_ring20"]["desc"] + ", tol 1e-12, <= 300 iterations", "D": H.dimension,
+           "bethe_energy": e_bethe}
+    for name, reorth in (("plain", False), ("reorthogonalized", True)):
+        qa.lanczos_ground_state(H, maxiter=300, tol=1e-12, reorthogonalize=reorth, return_state=False)   # warm
+        r = qa.lanczos_ground_state(H, maxiter=300, tol=1e-12, reorthogonalize=reorth, return_state=False)
+        rep = r.report
+        res[name] = {"iterations": r.iterations, "converged": bool(r.converged), "energy": r.energy,
+                     "rel_err_vs_bethe": abs(r.energy - e_bethe) / abs(e_bethe),
+                     "loop_seconds": round(rep["loop_seconds"], 5),
+                     "iters_per_s": round(rep["steps_run"] / rep["loop_seconds"], 1)}
+        log(f"[bench-c1] {name}: {r.iterations} it, E={r.energy:.12f} (Bethe {e_bethe:.12f}), "
+            f"{res[name]['iters_per_s']} it/s")
+    return res
+
+
+def cpu_lanczos_c1(budget_s=20.0):
+    """The oracle's restatement of the reference's lanczos_ground_state
+    (complex CSR gather SpMV + classical Gram-Schmidt against every stored
+    vector + tridiagonal QR each step) on the full Heisenberg ring L = 20,
+    tol 1e-12, at most 300 iterations, on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    terms = build_terms("heis_ring20")
+    kind, sec = oracle_sector("heis_ring20")
+    m = O.Model(kind, 20, sec, kind, terms, enumerate=False)
+    t0 = time.perf_counter()
+    m.build_csr()
+    t_build = time.perf_counter() - t0
+    start = O.start_vector(m.dim, 1234) + 0j
+    t0 = time.perf_counter()
+    r = m.lanczos(300, 1e-12, start=start, reorth=True)
+    el = time.perf_counter() - t0
+    return {"iterations": r["iterations"], "energy": r["energy"], "seconds": round(el, 3),
+            "iters_per_s": round(r["iterations"] / el, 2), "csr_build_seconds": round(t_build, 3),
+            "algorithm": "oracle restatement of src/solvers/lanczos.cpp:101-202 (CGS reorthogonalisation)"}
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES")}
 
 
 def stream_copy_gbs(qa, L, gib=2.0, reps=5):
@@ -280,7 +365,7 @@ def measure_traffic(args, flags):
             cmd = ["timeout", "-s", "KILL", "300", prof, "--pmc", ctr, "-d", out, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--workload", args.workload] + flags + [
                    "--steps", "2", "--warmup", "0", "--no-lanczos",
-                   "--no-cpu", "--no-traffic", "--no-stream", "--copy-calib-gib", str(calib_gib)]
+                   "--no-cpu", "--no-traffic", "--no-stream", "--no-extras", "--copy-calib-gib", str(calib_gib)]
             log(f"[bench] traffic pass {ctr}: {' '.join(cmd[3:])}")
             r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
                                text=True)
@@ -312,6 +397,8 @@ def main():
     ap.add_argument("--lanczos-iters", type=int, default=None)
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the C1 Lanczos, C4 batched FTLM / S(q, omega) and C5 serving legs of the default run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--nvec", type=int, default=1,
                     help="columns of the applied block (1: single-vector kernel; >1: row-major block kernel)")
@@ -331,6 +418,10 @@ def main():
     ap.add_argument("--copy-calib-gib", type=float, default=0.0,
                     help="also run one plain copy kernel of this many GiB (counter calibration)")
     args = ap.parse_args()
+    # the reference's recommended OpenMP placement for its CPU path
+    # (bindings/julia/README.md:343-347), used by the oracle's timed runs
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "threads")
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU runs are not part of this "
                                    "build (the apply would shard by ranges of output rows cut at block boundaries, "
@@ -455,9 +546,22 @@ def main():
                           "bytes_per_iter_alg": int(48 * D),
                           "achieved_GBs_alg": round(48 * D * ips / 1e9, 2)}
 
+    # the rest of the metric on its own workloads: C1 Lanczos it/s, C4
+    # batched FTLM + S(q, omega), C5 serving (each with its own setup,
+    # outside the apply's timed region)
+    if not args.no_extras and args.nvec == 1:
+        del H
+        out["lanczos_c1"] = lanczos_c1_leg()
+        out["ftlm_c4"] = ftlm_leg(args)
+        out["serving_c5"] = serve_leg(args)
+
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload, terms, D, args.cpu_seconds)
+        out["cpu_baseline"]["host"] = host_info()
         log(f"[bench] cpu baseline: {out['cpu_baseline']['value']} GB/s-eq on {out['cpu_baseline']['cores']} threads")
+        if not args.no_extras:
+            out["cpu_baseline"]["lanczos_c1_reference"] = cpu_lanczos_c1()
+            log(f"[bench] cpu reference Lanczos (C1): {out['cpu_baseline']['lanczos_c1_reference']}")
     print(json.dumps(out), flush=True)
 
 

@@ -679,8 +679,9 @@ class StructureFactorResult:
 
 def structure_factor(H, psi0, operators, omega, E0, eta=0.1, n_iter=100):
     """S_q(omega) = -1/pi Im <phi_q|(omega + E0 + i eta - H)^-1|phi_q> with
-    phi_q = O_q psi0 for every operator in `operators` (OpSum, operator
-    expressions, or MatrixFreeHamiltonian handles built with operator=True;
+    phi_q = O_q psi0 for every operator in `operators` (OpSum, term lists
+    [(coeff, [(op, site), ...])], operator expressions, or
+    MatrixFreeHamiltonian handles built with operator=True;
     O_q acts as written, not as its adjoint). The seeds are formed on the
     device and run as one block continued fraction of n_iter steps. Returns
     spectrum[q, w] and the continued fractions (DynamicsResult per q)."""
@@ -690,6 +691,8 @@ def structure_factor(H, psi0, operators, omega, E0, eta=0.1, n_iter=100):
         if isinstance(o, MatrixFreeHamiltonian):
             op = o
         else:
+            if isinstance(o, (list, tuple)):
+                o = OpSum(list(o))     # [(coeff, [(op, site), ...]), ...]
             op = MatrixFreeHamiltonian(H.basis, H.site, o, operator=True)
         keep.append(op)
         handles.append(op._h)

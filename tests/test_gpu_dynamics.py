@@ -59,8 +59,10 @@ def test_structure_factor_tj4x5_all_q_vs_dense_resolvent():
         assert err < TOL_S, (q, err)
         assert abs(r.cf[k].norm_phi0 - np.linalg.norm(phi)) < 1e-12 * max(1.0, np.linalg.norm(phi))
         # the first steps agree with the oracle's recurrence on the same seed
-        al, be, nrm = om.cf(phi, 20)
-        n = min(20, len(al), len(r.cf[k].alphas))
+        # (D = 380: orthogonality is lost, and the two roundings part, from
+        # about step 18 on)
+        al, be, nrm = om.cf(phi, 12)
+        n = min(12, len(al), len(r.cf[k].alphas))
         assert np.allclose(r.cf[k].alphas[:n], al[:n], rtol=1e-9, atol=1e-9)
     # q = 0: S^z_total commutes with H; psi0 has Sz = 0, so the seed vanishes
     assert r.cf[0].norm_phi0 < 1e-12 and np.max(np.abs(r.spectrum[0])) < 1e-12
