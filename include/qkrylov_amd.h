@@ -153,6 +153,22 @@ typedef struct {
 int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_options_t* opt, const double* start,
                             double* alphas, double* betas, double* vector, qkrylov_lanczos_report_t* report);
 
+/* The driver of qkrylov_lanczos_run_f64 without any device allocation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Every device buffer (three Krylov vectors, alpha/norm arrays, partial
+ * sums, the stored basis when reorthogonalize is set, the Ritz pass) comes
+ * from one static 64 MiB workspace per GPU that the library holds until the
+ * process ends; sizes that do not fit are refused with -2 and a message
+ * (roughly dim <= 2.0e6 with want_vector, 2.7e6 without, far less with
+ * reorthogonalize). Real Hamiltonians only; no graph replay (graph_mode is
+ * ignored). Results equal qkrylov_lanczos_run_f64's. Calls on one GPU share
+ * its workspace: this library serialises them within a process, so calls
+ * that overlap on one GPU wait for each other; the workspace keeps the last
+ * call's contents and every buffer is written before it is read. */
+int qkrylov_lanczos_run_noalloc_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_options_t* opt, const double* start,
+                                    double* alphas, double* betas, double* vector, qkrylov_lanczos_report_t* report);
+
 /* ---- batched FTLM: R Rademacher vectors (counter RNG: column r, row i)
  * run as one multi-vector recurrence of M steps (sequential = 1 runs the
  * same vectors one at a time); observables for nT inverse temperatures.

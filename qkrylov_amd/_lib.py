@@ -163,6 +163,8 @@ SIGNATURES = {
     "qkrylov_hamiltonian_states": (c_int, [c_vp, c_u64, c_u64, p_u64]),
     "qkrylov_hamiltonian_time_apply": (c_int, [c_vp, c_int, c_int, c_int, p_dbl]),
     "qkrylov_lanczos_run_f64": (c_int, [c_vp, P(LanczosOptions), p_dbl, p_dbl, p_dbl, p_dbl, P(LanczosReport)]),
+    "qkrylov_lanczos_run_noalloc_f64": (c_int, [c_vp, P(LanczosOptions), p_dbl, p_dbl, p_dbl, p_dbl,
+                                                P(LanczosReport)]),
     "qkrylov_ftlm_batched_f64": (c_int, [c_vp, c_int, c_int, c_u64, p_dbl, c_int, c_int, p_dbl, p_dbl, p_dbl,
                                          p_dbl, p_dbl, p_int, p_dbl]),
     "qkrylov_cf_batched_f64": (c_int, [c_vp, c_int, p_dbl, c_int, p_dbl, p_dbl, p_dbl, p_int, p_dbl]),

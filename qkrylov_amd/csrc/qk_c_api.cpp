@@ -15,6 +15,7 @@
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -816,6 +817,57 @@ int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_optio
     o.graph = opt->graph_mode == 0 ? -1 : (opt->graph_mode == 1 ? 1 : 0);
     o.start = start;
     auto r = qk::lanczos_run(*H->engine(), o, qk::default_stream());
+    rep->energy = r.energy;
+    rep->iterations = r.iterations;
+    rep->converged = r.converged ? 1 : 0;
+    rep->steps_run = r.steps_run;
+    rep->n_alpha = (int)r.alphas.size();
+    rep->n_beta = (int)r.betas.size();
+    rep->loop_seconds = r.loop_seconds;
+    rep->ritz_seconds = r.ritz_seconds;
+    if (alphas) std::memcpy(alphas, r.alphas.data(), r.alphas.size() * sizeof(double));
+    if (betas) std::memcpy(betas, r.betas.data(), r.betas.size() * sizeof(double));
+    if (o.want_vector) std::memcpy(vector, r.vector.data(), r.vector.size() * sizeof(double));
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
+int qkrylov_lanczos_run_noalloc_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_options_t* opt, const double* start,
+                                    double* alphas, double* betas, double* vector, qkrylov_lanczos_report_t* rep)
+{
+    const auto* H = H_of(h);
+    if (!H || !opt || !rep) return QKRYLOV_ERROR_INVALID_ARG;
+    if (opt->graph_mode < 0 || opt->graph_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    if (!H->is_real()) throw std::invalid_argument("the no-allocation Lanczos driver needs a real Hamiltonian");
+    qk::LanczosOptions o;
+    o.maxiter = opt->maxiter;
+    o.tol = opt->tol;
+    o.reorth = opt->reorthogonalize != 0;
+    o.fixed_steps = opt->fixed_steps != 0;
+    o.want_vector = (opt->want_vector != 0) && vector != nullptr;
+    o.seed = opt->seed;
+    o.check_every = opt->check_every;
+    o.graph = 0;   // graph instantiation may allocate
+    o.start = start;
+    int dev = 0;
+    qk::hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= 64) throw std::runtime_error("device id out of range");
+    // one caller at a time per device: every call shares the device's array
+    static std::mutex mu[64];
+    std::lock_guard<std::mutex> lk(mu[dev]);
+    void* base = nullptr;
+    size_t cap = 0;
+    if (qkl_scratch(&base, &cap) != 0) throw std::runtime_error("static workspace lookup failed");
+    const size_t need = qk::lanczos_arena_bytes(*H->engine(), o);
+    if (need > cap)
+        throw std::length_error("no-allocation Lanczos: needs " + std::to_string(need) + " bytes of workspace, the static "
+                                "workspace holds " + std::to_string(cap) + " (dimension " +
+                                std::to_string(H->dimension()) + ")");
+    qk::Arena arena;
+    arena.base = static_cast<char*>(base);
+    arena.cap = cap;
+    auto r = qk::lanczos_run(*H->engine(), o, qk::default_stream(), &arena);
     rep->energy = r.energy;
     rep->iterations = r.iterations;
     rep->converged = r.converged ? 1 : 0;

@@ -48,12 +48,24 @@ DevBuf::~DevBuf() { reset(); }
 
 void DevBuf::reset()
 {
-    if (p_) {
+    if (p_ && owned_) {
         if (async_) (void)hipFreeAsync(p_, stream_);
         else (void)hipFree(p_);
     }
     p_ = nullptr;
     bytes_ = 0;
+    owned_ = true;
+}
+
+DevBuf Arena::take(size_t bytes)
+{
+    const size_t start = (used + 255) & ~(size_t)255;
+    const size_t len = std::max<size_t>(bytes, 8);
+    if (start > cap || len > cap - start)
+        throw std::length_error("workspace of " + std::to_string(cap) + " bytes exhausted (needs " +
+                                std::to_string(start + len) + ")");
+    used = start + len;
+    return DevBuf::view(base + start, bytes);
 }
 
 DevBuf& DevBuf::operator=(DevBuf&& o) noexcept
@@ -64,6 +76,7 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept
         bytes_ = o.bytes_;
         stream_ = o.stream_;
         async_ = o.async_;
+        owned_ = o.owned_;
         o.p_ = nullptr;
         o.bytes_ = 0;
     }
@@ -548,10 +561,10 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool so)
-    : E(e), s(st), D((size_t)e.dim()), kcap(kcap_)
+RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool so, Arena* ar)
+    : E(e), s(st), D((size_t)e.dim()), kcap(kcap_), arena(ar)
 {
-    auto mk = [&](size_t bytes) { return so ? DevBuf(bytes, st) : DevBuf(bytes); };
+    auto mk = [&](size_t bytes) { return ar ? ar->take(bytes) : (so ? DevBuf(bytes, st) : DevBuf(bytes)); };
     for (auto& q : Q) q = mk(D * sizeof(double));
     alpha = mk((kcap + 2) * sizeof(double));
     nrm = mk((kcap + 3) * sizeof(double));
@@ -584,7 +597,7 @@ void RealCore::steps_graph(int k, int n)
         graph = nullptr;
     }
     if (!gexec) {
-        if (kdev.bytes() == 0) kdev = DevBuf(2 * sizeof(int));
+        if (kdev.bytes() == 0) kdev = arena ? arena->take(2 * sizeof(int)) : DevBuf(2 * sizeof(int));
         cap_ic = ic;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
         int rc = 0;
@@ -668,24 +681,62 @@ void RealCore::step(int k, const double* basis)
 
 static LanczosOutput lanczos_complex(const Engine& E, const LanczosOptions& o, hipStream_t s);
 
-LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t s)
+size_t lanczos_arena_bytes(const Engine& E, const LanczosOptions& o)
+{
+    const size_t D = (size_t)E.dim();
+    const int kmax = (int)std::min<int64_t>(o.maxiter, (int64_t)D);
+    if (D == 0 || kmax <= 0) return 0;
+    auto al = [](size_t b) { return (std::max<size_t>(b, 8) + 255) & ~(size_t)255; };
+    const size_t npA = (size_t)std::max(qkl_apply_partials(E.tables(), E.wide() ? 1 : 0), 1);
+    const size_t NB = (size_t)qkl_blas_partials();
+    size_t core = 3 * al(D * sizeof(double)) + al((kmax + 2) * sizeof(double)) + al((kmax + 3) * sizeof(double)) +
+                  al(npA * sizeof(double)) + al(NB * sizeof(double));
+    size_t basis = 0;
+    if (o.reorth) {
+        basis = al((size_t)(kmax + 1) * D * sizeof(double));
+        core += al((kmax + 2) * sizeof(double)) + al(NB * (kmax + 2) * sizeof(double));
+    }
+    size_t ritz = 0;
+    if (o.want_vector) {
+        ritz = al(D * sizeof(double)) + al(2 * sizeof(double)) + al(NB * 2 * sizeof(double));
+        if (o.reorth) ritz += al((size_t)(kmax + 1) * sizeof(double));
+        else ritz += 3 * al(D * sizeof(double)) + al((kmax + 2) * sizeof(double)) + al((kmax + 3) * sizeof(double)) +
+                     al(npA * sizeof(double)) + al(NB * sizeof(double));
+    }
+    return basis + std::max(core, ritz);
+}
+
+LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t s, Arena* arena)
 {
     LanczosOutput out;
     const int64_t D = E.dim();
     if (D == 0) return out;
-    if (!E.is_real()) return lanczos_complex(E, o, s);
+    if (!E.is_real()) {
+        if (arena) throw std::invalid_argument("the workspace Lanczos driver needs a real Hamiltonian");
+        return lanczos_complex(E, o, s);
+    }
     const int kmax = (int)std::min<int64_t>(o.maxiter, D);
     if (kmax <= 0) return out;
     const bool reorth = o.reorth;
-    size_t free_b = 0, total_b = 0;
-    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     const size_t basis_bytes = (size_t)(kmax + 1) * (size_t)D * sizeof(double);
-    const bool store = reorth || (o.want_vector && basis_bytes < free_b / 3);
-    if (reorth && basis_bytes > (free_b / 10) * 9)
-        throw std::runtime_error("full reorthogonalisation needs more device memory than is free");
-    RealCore C(E, kmax, s, reorth);
+    bool store = reorth;
+    if (arena) {
+        if (lanczos_arena_bytes(E, o) > arena->cap - std::min(arena->cap, arena->used))
+            throw std::length_error("Lanczos needs " + std::to_string(lanczos_arena_bytes(E, o)) +
+                                    " bytes of workspace, " + std::to_string(arena->cap - arena->used) + " available");
+    } else {
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        store = reorth || (o.want_vector && basis_bytes < free_b / 3);
+        if (reorth && basis_bytes > (free_b / 10) * 9)
+            throw std::runtime_error("full reorthogonalisation needs more device memory than is free");
+    }
+    auto mk = [&](size_t bytes) { return arena ? arena->take(bytes) : DevBuf(bytes); };
     DevBuf basis;
-    if (store) basis = DevBuf(basis_bytes);
+    if (store) basis = mk(basis_bytes);
+    const size_t core_mark = arena ? arena->mark() : 0;
+    auto Cp = std::make_unique<RealCore>(E, kmax, s, reorth, false, arena);
+    RealCore& C = *Cp;
 
     auto load_start = [&](double* dst) {
         if (o.start_dev) hip_check(hipMemcpyAsync(dst, o.start_dev, D * sizeof(double), hipMemcpyDeviceToDevice, s), "start");
@@ -698,7 +749,7 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
 
     LanczosReplay rp{kmax, o.tol, o.fixed_steps, {}, {}};
     // launch-bound sizes replay a captured six-step graph between host checks
-    const bool graph = !store && !reorth && kmax >= 3 && (o.graph > 0 || (o.graph < 0 && D < (1 << 22)));
+    const bool graph = !arena && !store && !reorth && kmax >= 3 && (o.graph > 0 || (o.graph < 0 && D < (1 << 22)));
     // host convergence checks: every 8-12 steps while launches dominate,
     // every 4 up to 2^25 rows (a read-back costs a tenth of a step there),
     // every step beyond; the replay fixes the iteration count either way
@@ -738,20 +789,24 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
     out.betas = rp.betas;
     std::vector<double> z;
     out.energy = tridiag_min(rp.alphas.data(), rp.betas.data(), n, o.want_vector ? &z : nullptr);
+    // the recurrence's buffers are done with: the Ritz pass reuses their room
+    hip_check(hipStreamSynchronize(s), "sync");
+    Cp.reset();
+    if (arena) arena->rewind(core_mark);
 
     if (o.want_vector) {
         const double t1 = now_s();
-        DevBuf ritz(D * sizeof(double));
+        DevBuf ritz = mk(D * sizeof(double));
         lcheck(qkl_fill_zero((size_t)D, ritz.d(), s), "zero");
         if (store) {
-            DevBuf zc(n * sizeof(double));
+            DevBuf zc = mk(n * sizeof(double));
             hip_check(hipMemcpyAsync(zc.get(), z.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
             lcheck(qkl_multiaxpy((size_t)D, n, basis.d(), (size_t)D, zc.d(), 1.0, ritz.d(), s), "ritz");
             hip_check(hipStreamSynchronize(s), "sync");
         } else {
             // second pass: rerun the same deterministic recurrence and
             // accumulate sum_k z_k v_k (v_k = q_k / nrm_k)
-            RealCore R(E, n, s, false);
+            RealCore R(E, n, s, false, false, arena);
             load_start(R.cur());
             R.init_norm();
             for (int kk = 0; kk < n; ++kk) {
@@ -759,9 +814,10 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
                 if (kk == n - 1) break;
                 R.step(kk, nullptr);
             }
+            hip_check(hipStreamSynchronize(s), "sync");
         }
-        DevBuf tmp(2 * sizeof(double));
-        DevBuf part(qkl_blas_partials() * 2 * sizeof(double));
+        DevBuf tmp = mk(2 * sizeof(double));
+        DevBuf part = mk(qkl_blas_partials() * 2 * sizeof(double));
         const double nr = std::sqrt(dot_host(false, (size_t)D, ritz.d(), ritz.d(), part.d(), tmp.d(), s).real());
         if (nr != 0.0) lcheck(qkl_scal(0, (size_t)D, 1.0 / nr, 0.0, ritz.d(), s), "scal");
         out.vector.resize(D);

@@ -25,12 +25,22 @@ public:
     ~DevBuf();
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_), stream_(o.stream_), async_(o.async_)
+    DevBuf(DevBuf&& o) noexcept
+        : p_(o.p_), bytes_(o.bytes_), stream_(o.stream_), async_(o.async_), owned_(o.owned_)
     {
         o.p_ = nullptr;
         o.bytes_ = 0;
     }
     DevBuf& operator=(DevBuf&& o) noexcept;
+    // non-owning view of memory that someone else keeps (never freed here)
+    static DevBuf view(void* p, size_t bytes)
+    {
+        DevBuf b;
+        b.p_ = p;
+        b.bytes_ = bytes;
+        b.owned_ = false;
+        return b;
+    }
     void* get() const { return p_; }
     double* d() const { return static_cast<double*>(p_); }
     size_t bytes() const { return bytes_; }
@@ -41,6 +51,19 @@ private:
     size_t bytes_ = 0;
     hipStream_t stream_ = nullptr;
     bool async_ = false;
+    bool owned_ = true;
+};
+
+// Bump allocator over device memory the caller keeps (a fixed workspace):
+// take() hands out 256 B aligned non-owning views and throws when the
+// workspace is exhausted; mark()/rewind() reuse a region after its users are
+// gone. Used by the entry points that must not allocate.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+    DevBuf take(size_t bytes);
+    size_t mark() const { return used; }
+    void rewind(size_t m) { used = m; }
 };
 
 // one library stream per host thread (default priority)
@@ -149,7 +172,10 @@ struct RealCore {
     int cap_ic = -1;   // rotation state the graph was captured in
     int pending = -1;  // nrm index whose partial sums are not yet totalled
 
-    RealCore(const Engine& e, int kcap, hipStream_t st, bool reorth, bool stream_ordered = false);
+    Arena* arena = nullptr;   // set: every device buffer comes from it
+
+    RealCore(const Engine& e, int kcap, hipStream_t st, bool reorth, bool stream_ordered = false,
+             Arena* arena = nullptr);
     ~RealCore();
     RealCore(const RealCore&) = delete;
     RealCore& operator=(const RealCore&) = delete;
@@ -200,7 +226,12 @@ struct LanczosOutput {
     int steps_run = 0;               // device steps computed (>= iterations)
 };
 
-LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t s);
+// arena: when given, every device buffer (Krylov vectors, scalars, partial
+// sums, the stored basis) is carved from it and nothing is allocated; real
+// Hamiltonians only, no graph capture.
+LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t s, Arena* arena = nullptr);
+// device bytes lanczos_run needs with an arena for these options
+size_t lanczos_arena_bytes(const Engine& E, const LanczosOptions& o);
 
 // three-term recurrence from a start vector without reorthogonalisation;
 // alphas/betas truncated at the first beta < 1e-15 as the reference's

@@ -1288,6 +1288,14 @@ __global__ __launch_bounds__(QK_TPB) void k_precond(size_t n, const double* __re
     }
 }
 
+// ------------------------------------------------------------------ static workspace
+// The workspace of the entry points that must not allocate (for callers
+// under graph capture or in loops that forbid hipMalloc): one array per GPU,
+// alive until the process ends. Host code never uses the symbol's name as
+// an address; qkl_scratch looks up the device address once per device.
+#define QK_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) double scratch[QK_SCRATCH_BYTES / sizeof(double)];
+
 // ------------------------------------------------------------------ launch helpers
 // LDS bytes of the lo-rank table, rounded to 16 so what follows it is aligned
 inline size_t table_bytes(const QkTables* T)
@@ -1741,6 +1749,28 @@ int qkl_davidson_precond(int cplx, size_t n, const double* r, const double* diag
     if (cplx) hipLaunchKernelGGL(k_precond<true>, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, r, diag, lambda, t);
     else hipLaunchKernelGGL(k_precond<false>, dim3(grid_for(n)), dim3(QK_TPB), 0, (hipStream_t)sv, n, r, diag, lambda, t);
     return rc(hipGetLastError());
+}
+
+// device address and size of the static workspace on the current device:
+// hipGetSymbolAddress once per device (with that device current), kept
+int qkl_scratch(void** ptr, size_t* bytes)
+{
+    static std::mutex mu;
+    static void* addr[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return -(int)e;
+    if (dev < 0 || dev >= 64) return -1;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return e != hipSuccess ? -(int)e : -1;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    *bytes = QK_SCRATCH_BYTES;
+    return 0;
 }
 
 } // extern "C"

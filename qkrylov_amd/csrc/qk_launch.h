@@ -108,6 +108,11 @@ int qkl_states(const qk::QkTables* T, int wide, uint64_t r0, uint64_t n, uint64_
  * (used to build S^z_q seeds): cplx x/y interleaved. */
 int qkl_apply_diag_terms(const qk::QkTables* T, int wide, int xcplx, const double* x, double* y, qkl_stream s);
 
+/* Static per-device workspace of the no-allocation entry points: its
+ * device address on the current device (looked up once per device) and its
+ * size in bytes. */
+int qkl_scratch(void** ptr, size_t* bytes);
+
 /* Davidson preconditioner t = r / (diag - lambda) with the reference clamp */
 int qkl_davidson_precond(int cplx, size_t n, const double* r, const double* diag, double lambda, double* t,
                          qkl_stream s);
