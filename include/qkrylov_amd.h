@@ -226,6 +226,9 @@ typedef struct {
     size_t hbm_budget_bytes;   /* admission budget (0: 80% of free memory) */
     int max_active;            /* concurrent requests on the device */
     int realtime;              /* 1: honour arrival times; 0: all arrive at t=0 */
+    double* const* gs_vectors; /* optional, nreq host pointers (each NULL or dim
+                                * doubles): the normalised ground state a kind-1
+                                * request's continued fraction started from */
 } qkrylov_serve_options_t;
 
 /* Runs the trace. cf_alphas/cf_betas: nreq * cf_cap host arrays (optional).

@@ -8,7 +8,7 @@ from .api import (  # noqa: F401
     FermionSite, FTLMResult, HubbardBasis, HubbardSite, LanczosResult, MatrixFreeHamiltonian, N, Ndn, Nup, Nupdn,
     Op, OpSum, Site, SpinHalfBasis, SpinHalfSite, Sm, Sp, Sx, Sy, Sz, TJBasis, TJSite, continued_fraction_batched,
     continued_fraction_coeffs, davidson_lowest, evaluate_spectral_function, ftlm, ftlm_batched, lanczos_ground_state,
-    OpProduct, OpTermSum, StructureFactorResult, structure_factor,
+    OpProduct, OpTermSum, StructureFactorResult, structure_factor, tridiag_eigen, tridiag_lowest,
 )
 from . import models  # noqa: F401
 from .storage import load_result, save_result  # noqa: F401
@@ -36,6 +36,6 @@ __all__ = [
     "LanczosResult", "lanczos_ground_state", "DavidsonResult", "davidson_lowest", "DynamicsResult",
     "continued_fraction_coeffs", "evaluate_spectral_function", "FTLMResult", "ftlm",
     "ftlm_batched", "continued_fraction_batched", "DeviceArray", "structure_factor", "StructureFactorResult",
-    "OpProduct", "OpTermSum",
+    "OpProduct", "OpTermSum", "tridiag_eigen", "tridiag_lowest",
     "find_gpu", "gpu_count", "save_result", "load_result",
 ]

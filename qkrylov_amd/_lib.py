@@ -66,7 +66,8 @@ class ServeResult(ctypes.Structure):
 
 
 class ServeOptions(ctypes.Structure):
-    _fields_ = [("hbm_budget_bytes", c_size), ("max_active", c_int), ("realtime", c_int)]
+    _fields_ = [("hbm_budget_bytes", c_size), ("max_active", c_int), ("realtime", c_int),
+                ("gs_vectors", P(c_vp))]
 
 
 # name -> (restype, argtypes)

@@ -669,6 +669,35 @@ def continued_fraction_batched(H, seeds, n_iter=100):
     return out, sec.value
 
 
+def tridiag_lowest(alphas, betas):
+    """Lowest eigenvalue of the symmetric tridiagonal (alphas, betas) by
+    Sturm bisection (host; the Lanczos drivers' convergence test)."""
+    a = np.ascontiguousarray(alphas, dtype=np.float64)
+    b = np.ascontiguousarray(betas if len(betas) else [0.0], dtype=np.float64)
+    if len(a) == 0:
+        raise ValueError("empty tridiagonal")
+    out = ctypes.c_double()
+    check(_lib.load().qkrylov_amd_tridiag_lowest(len(a), _dp(a), _dp(b), ctypes.byref(out)), "tridiag_lowest")
+    return out.value
+
+
+def tridiag_eigen(alphas, betas, vectors="first"):
+    """Eigenvalues (ascending) of the symmetric tridiagonal (alphas, betas)
+    and, for vectors="first", the first eigenvector components (the FTLM
+    weights), for vectors="all" the eigenvector matrix (columns), for
+    vectors=None nothing (host implicit QL)."""
+    a = np.ascontiguousarray(alphas, dtype=np.float64)
+    b = np.ascontiguousarray(betas if len(betas) else [0.0], dtype=np.float64)
+    n = len(a)
+    want = {None: 0, "first": 1, "all": 2}[vectors]
+    w = np.zeros(max(n, 1))
+    z = np.zeros(max(n * n if want == 2 else n, 1))
+    check(_lib.load().qkrylov_amd_tridiag_eigen(n, _dp(a), _dp(b), _dp(w), _dp(z), want), "tridiag_eigen")
+    if want == 0:
+        return w[:n]
+    return w[:n], (z[:n * n].reshape(n, n) if want == 2 else z[:n])
+
+
 class StructureFactorResult:
     def __init__(self, omega, spectrum, cf, seconds):
         self.omega, self.spectrum, self.cf, self.seconds = omega, spectrum, cf, seconds

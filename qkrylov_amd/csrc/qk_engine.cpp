@@ -562,7 +562,7 @@ double now_s()
 }
 
 RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool so, Arena* ar)
-    : E(e), s(st), D((size_t)e.dim()), kcap(kcap_), arena(ar)
+    : E(e), s(st), D((size_t)e.dim()), kcap(kcap_), arena(ar), stream_ordered(so)
 {
     auto mk = [&](size_t bytes) { return ar ? ar->take(bytes) : (so ? DevBuf(bytes, st) : DevBuf(bytes)); };
     for (auto& q : Q) q = mk(D * sizeof(double));
@@ -597,7 +597,9 @@ void RealCore::steps_graph(int k, int n)
         graph = nullptr;
     }
     if (!gexec) {
-        if (kdev.bytes() == 0) kdev = arena ? arena->take(2 * sizeof(int)) : DevBuf(2 * sizeof(int));
+        if (kdev.bytes() == 0)
+            kdev = arena ? arena->take(2 * sizeof(int))
+                         : (stream_ordered ? DevBuf(2 * sizeof(int), s) : DevBuf(2 * sizeof(int)));
         cap_ic = ic;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
         int rc = 0;

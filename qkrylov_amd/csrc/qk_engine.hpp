@@ -173,6 +173,7 @@ struct RealCore {
     int pending = -1;  // nrm index whose partial sums are not yet totalled
 
     Arena* arena = nullptr;   // set: every device buffer comes from it
+    bool stream_ordered = false;   // buffers from hipMallocAsync on s
 
     RealCore(const Engine& e, int kcap, hipStream_t st, bool reorth, bool stream_ordered = false,
              Arena* arena = nullptr);

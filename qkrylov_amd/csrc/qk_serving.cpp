@@ -49,6 +49,9 @@ OpSum szpi(int L)
     return o;
 }
 
+// pinned host buffers for the alpha / norm read-backs, one pair per stream
+// slot, allocated once per run (freeing pinned memory synchronises the
+// device, so nothing is freed while requests are in flight)
 struct Pinned {
     double* p = nullptr;
     explicit Pinned(size_t n)
@@ -60,6 +63,8 @@ struct Pinned {
     {
         if (p) (void)hipHostFree(p);
     }
+    Pinned(const Pinned&) = delete;
+    Pinned& operator=(const Pinned&) = delete;
 };
 
 enum Phase { kGS, kRitz, kCF, kDone };
@@ -73,7 +78,8 @@ struct Job {
     std::unique_ptr<Engine> H, Op;
     std::unique_ptr<RealCore> C;
     std::unique_ptr<LanczosReplay> rp;
-    std::unique_ptr<Pinned> ha, hn;
+    Pinned *ha = nullptr, *hn = nullptr;   // the slot's read-back buffers
+    double* gs_out = nullptr;              // caller's host buffer for the ground state (or null)
     DevBuf ritz, psi2;
     Phase phase = kGS;
     int kcap = 0, k = 0, n_gs = 0;
@@ -136,8 +142,6 @@ void start_job(Job& j)
     j.kcap = (int)std::min<int64_t>(j.rq.maxiter, j.basis->dim());
     const int cap = std::max(j.kcap, j.rq.cf_steps) + 2;
     j.C = std::make_unique<RealCore>(*j.H, cap, j.s, false, true);
-    j.ha = std::make_unique<Pinned>(cap + 3);
-    j.hn = std::make_unique<Pinned>(cap + 3);
     j.rp = std::make_unique<LanczosReplay>(LanczosReplay{j.kcap, j.rq.tol, false, {}, {}});
     j.seed = 1234;
     if (qkl_fill_random(0, (size_t)j.basis->dim(), j.seed, 0, 1, 1, 0, j.C->cur(), j.s) != 0)
@@ -227,8 +231,23 @@ void advance(Job& j, qkrylov_serve_result_t& res, double* cf_a, double* cf_b, in
             }
         }
         res.n_cf = na;
+        if (j.gs_out) {
+            // the normalised Ritz vector the continued fraction started from
+            hip_check(hipMemcpyAsync(j.gs_out, j.ritz.get(), D * sizeof(double), hipMemcpyDeviceToHost, j.s), "d2h");
+            hip_check(hipStreamSynchronize(j.s), "sync");
+            const double inv = (psi2 > 0) ? 1.0 / std::sqrt(psi2) : 0.0;
+            for (size_t i = 0; i < D; ++i) j.gs_out[i] *= inv;
+        }
         j.phase = kDone;
     }
+}
+
+// pinned read-back room a request needs (alpha and norm arrays)
+size_t readback_doubles(const qkrylov_serve_request_t& r)
+{
+    const size_t D = dim_of(r);
+    const int kcap = (int)std::min<size_t>((size_t)r.maxiter, D);
+    return (size_t)std::max(kcap, r.cf_steps) + 2 + 3;
 }
 
 } // namespace
@@ -279,6 +298,9 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         // streams, events and jobs are released on every exit; on an error
         // the streams are drained first, so no job's memory is freed while
         // its kernels may still run
+        // the read-back buffers outlive `release` below, which drains the
+        // streams before anything else is freed
+        std::vector<std::unique_ptr<Pinned>> pin_a, pin_n;
         std::vector<hipStream_t> streams;
         std::vector<hipEvent_t> events;
         std::vector<std::unique_ptr<Job>> active;
@@ -294,6 +316,8 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                 for (auto e : ev) (void)hipEventDestroy(e);
             }
         } release{streams, events, active};
+        size_t rb = 1;
+        for (int i = 0; i < nreq; ++i) rb = std::max(rb, readback_doubles(reqs[i]));
         for (int i = 0; i < max_active; ++i) {
             hipStream_t st = nullptr;
             hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
@@ -301,6 +325,8 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
             hipEvent_t ev = nullptr;
             hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
             events.push_back(ev);
+            pin_a.push_back(std::make_unique<Pinned>(rb));
+            pin_n.push_back(std::make_unique<Pinned>(rb));
         }
         std::vector<int> order(nreq);
         for (int i = 0; i < nreq; ++i) order[i] = i;
@@ -331,6 +357,9 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                 free_slots.pop_back();
                 j->s = streams[slot];
                 j->ev = events[slot];
+                j->ha = pin_a[slot].get();
+                j->hn = pin_n[slot].get();
+                j->gs_out = (opt && opt->gs_vectors && reqs[id].kind == 1) ? opt->gs_vectors[id] : nullptr;
                 j->footprint = fp;
                 j->start = now_s() - t0;
                 results[id].start = j->start;

@@ -33,7 +33,10 @@ def make_trace(n, rate=50.0, L_range=(12, 28), delta_range=(0.5, 1.5), max_abs_s
     return reqs
 
 
-def run(reqs, realtime=True, max_active=32, hbm_budget_bytes=0, cf_cap=None):
+def run(reqs, realtime=True, max_active=32, hbm_budget_bytes=0, cf_cap=None, want_ground_states=False):
+    """Runs the trace; with want_ground_states every kind-1 result also
+    carries "ground_state", the normalised vector its continued fraction
+    started from."""
     n = len(reqs)
     cf_cap = cf_cap or max([r["cf_steps"] for r in reqs] + [1])
     arr = (_lib.ServeRequest * n)()
@@ -44,7 +47,10 @@ def run(reqs, realtime=True, max_active=32, hbm_budget_bytes=0, cf_cap=None):
     cfa = np.zeros(n * cf_cap)
     cfb = np.zeros(n * cf_cap)
     tot = np.zeros(4)
-    opt = _lib.ServeOptions(int(hbm_budget_bytes), int(max_active), 1 if realtime else 0)
+    gs = [np.zeros(xxz_dim(r["L"], r["sz2"])) if (want_ground_states and r["kind"] == 1) else None for r in reqs]
+    ptrs = (ctypes.c_void_p * n)(*[g.ctypes.data if g is not None else None for g in gs])
+    opt = _lib.ServeOptions(int(hbm_budget_bytes), int(max_active), 1 if realtime else 0,
+                            ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)) if want_ground_states else None)
     p = _lib.p_dbl
     _lib.check(_lib.load().qkrylov_serve_run(arr, n, ctypes.byref(opt), res, cfa.ctypes.data_as(p),
                                              cfb.ctypes.data_as(p), cf_cap, tot.ctypes.data_as(p)), "serve_run")
@@ -54,5 +60,7 @@ def run(reqs, realtime=True, max_active=32, hbm_budget_bytes=0, cf_cap=None):
         d = {f: getattr(r, f) for f, _ in r._fields_}
         d["cf_alphas"] = cfa[i * cf_cap:i * cf_cap + r.n_cf].copy()
         d["cf_betas"] = cfb[i * cf_cap:i * cf_cap + max(r.n_cf - 1, 0)].copy()
+        if gs[i] is not None:
+            d["ground_state"] = gs[i]
         out.append(d)
     return out, dict(wall_s=tot[0], solves_per_s=tot[1], p50_s=tot[2], p99_s=tot[3])

@@ -152,17 +152,20 @@ def test_full_size_apply_is_symmetric(name):
 
 
 def test_full_size_ftlm_batched_equals_sequential():
-    """C4 at full size (t-J 4x5, 2 holes, D = 9,237,800): FTLM over R = 8
-    Rademacher columns (M = 100, 64 temperatures) as one multi-vector
-    recurrence and as 8 sequential single-vector runs over the same vectors:
-    the tridiagonals and Z, E, C_v agree within the 1e-8 observable
-    tolerance; Z is positive, E lies above the ground-state energy."""
+    """C4 exactly as configured (t-J 4x5, 2 holes, D = 9,237,800; R = 64
+    Rademacher columns, M = 100, 64 temperatures T = 0.05..5) as one
+    64-column recurrence and as 64 sequential single-vector runs over the
+    same vectors: the tridiagonals and Z, E, C_v agree within the 1e-8
+    observable tolerance; Z is positive, E lies above the ground-state
+    energy."""
     H = qa.MatrixFreeHamiltonian(qa.TJBasis(20, nup=9, ndn=9), qa.TJSite(), qa.OpSum(models.tj_model(4, 5, 1.0, 0.4)))
     assert H.dimension == 9237800
     betas = 1.0 / np.linspace(0.05, 5.0, 64)
-    b = qa.ftlm_batched(H, 8, 100, betas, seed=42)
-    s = qa.ftlm_batched(H, 8, 100, betas, seed=42, sequential=True)
-    for r in range(8):
+    R = 64
+    b = qa.ftlm_batched(H, R, 100, betas, seed=42)
+    s = qa.ftlm_batched(H, R, 100, betas, seed=42, sequential=True)
+    print(f"C4 FTLM R=64: batched {b['seconds']:.3f}s, sequential {s['seconds']:.3f}s", flush=True)
+    for r in range(R):
         assert len(b["alphas"][r]) == len(s["alphas"][r])
         assert np.max(np.abs(b["alphas"][r] - s["alphas"][r])) < 1e-8 * np.max(np.abs(s["alphas"][r]))
     for key in ("Z", "E", "Cv"):
@@ -205,3 +208,38 @@ def test_full_size_davidson_equals_lanczos():
           flush=True)
     assert abs(d.eigenvalues[0] - e_l) < 1e-10 * abs(e_l)
     assert r < 1e-4
+
+
+def test_full_size_hubbard16_lanczos_vs_lieb_wu():
+    """C3 exactly as configured (Hubbard ring, 16 sites, t = 1, U = 4,
+    8 up 8 down, D = 165,636,900; seeded uniform start, 250 plain
+    iterations): the energy against the Lieb-Wu Bethe-ansatz ground state
+    (tests/textbook.py, pinned against dense and oracle spectra on the CPU)
+    within 1e-10 relative."""
+    H = qa.MatrixFreeHamiltonian(qa.HubbardBasis(16, nup=8, ndn=8), qa.HubbardSite(),
+                                 qa.OpSum(models.hubbard_chain(16, 1.0, 4.0)))
+    assert H.dimension == 165636900
+    r = qa.lanczos_ground_state(H, maxiter=250, reorthogonalize=False, fixed_steps=True, return_state=False)
+    e = TB.lieb_wu_ring(16, 8, 8, 4.0)
+    print(f"C3: E(250) = {r.energy!r}, Lieb-Wu {e!r}", flush=True)
+    assert abs(r.energy - e) < 1e-10 * abs(e)
+
+
+def test_full_size_j1j2_energy_independent_of_split():
+    """C2 exactly as configured (J1-J2 4x8 periodic, D = 601,080,390, 200
+    plain iterations, no reorthogonalisation): the energy after 200 steps
+    does not depend on the lo-bit split (which changes the order in which a
+    row's entries are added) within 1e-10 relative, and lies below the
+    energy after 100 steps (variational)."""
+    terms = models.j1j2_square(8, 4, 1.0, 0.5)
+    energies = []
+    for nl in (-1, 12):
+        H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(32, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms), split_bits=nl)
+        assert H.dimension == 601080390
+        r = qa.lanczos_ground_state(H, maxiter=200, reorthogonalize=False, fixed_steps=True, return_state=False)
+        energies.append(r.energy)
+        e100 = qa.tridiag_lowest(r.alphas[:100], r.betas[:99])
+        assert r.energy <= e100
+        del H
+    print(f"C2: E(200) = {energies[0]!r} (default split), {energies[1]!r} (NL 12)", flush=True)
+    assert abs(energies[0] - energies[1]) < 1e-10 * abs(energies[0])

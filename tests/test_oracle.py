@@ -190,6 +190,45 @@ def test_bethe_helper_small_rings():
         assert abs(np.linalg.eigvalsh(H)[0] - TB.bethe_xxx_ring(L)) < 1e-12
 
 
+@pytest.mark.parametrize("L,sz2", [(6, 0), (8, 0), (8, 2), (10, 0), (10, 4)])
+def test_xxz_bethe_helper_vs_textbook_dense(L, sz2):
+    """The XXZ Bethe-ansatz solver the serving tests use, against dense
+    matrices built from the definitions (no oracle involved)."""
+    st = TB.sector_states(TB.SPIN, L, use_sz=1, sz2=sz2)
+    for delta in (0.5, 0.73, 0.95):
+        H = TB.stored_matrix(TB.SPIN, models.xxz_chain(L, delta), L, st)
+        assert abs(np.linalg.eigvalsh(H)[0] - TB.bethe_xxz_ring(L, delta, sz2)) < 1e-12 * L
+    # sector symmetry Sz <-> -Sz
+    assert TB.bethe_xxz_ring(L, 0.6, sz2) == TB.bethe_xxz_ring(L, 0.6, -sz2)
+
+
+@pytest.mark.parametrize("L", [4])
+def test_lieb_wu_helper_vs_textbook_dense(L):
+    """The Lieb-Wu solver the full-size Hubbard test uses (half filling),
+    against dense Jordan-Wigner matrices built from the definitions, for
+    two values of U."""
+    h = L // 2
+    st = TB.sector_states(TB.HUBBARD, L, use_nup=1, nup=h, use_ndn=1, ndn=h)
+    for U in (4.0, 2.5):
+        H = TB.stored_matrix(TB.HUBBARD, models.hubbard_chain(L, 1.0, U), 2 * L, st)
+        assert abs(np.linalg.eigvalsh(H)[0] - TB.lieb_wu_ring(L, h, h, U)) < 1e-12 * L
+
+
+def test_lieb_wu_helper_vs_oracle_L6_to_L10():
+    """Beyond the textbook matrices' reach: the oracle's sector matrices at
+    L = 6 and 8 (D = 400, 4,900, dense) and its reorthogonalised Lanczos at L = 10
+    (D = 63,504) against Lieb-Wu."""
+    for L in (6, 8):
+        m = O.Model(O.HUBBARD, L, O.sector(use_nup=1, nup=L // 2, use_ndn=1, ndn=L // 2), O.HUBBARD,
+                    models.hubbard_chain(L, 1.0, 4.0), enumerate=False)
+        assert abs(np.linalg.eigvalsh(m.dense())[0] - TB.lieb_wu_ring(L, L // 2, L // 2, 4.0)) < 1e-12 * L
+    m = O.Model(O.HUBBARD, 10, O.sector(use_nup=1, nup=5, use_ndn=1, ndn=5), O.HUBBARD,
+                models.hubbard_chain(10, 1.0, 4.0), enumerate=False)
+    r = m.lanczos(300, 1e-13)
+    e = TB.lieb_wu_ring(10, 5, 5, 4.0)
+    assert abs(r["energy"] - e) < 1e-11 * abs(e)
+
+
 def test_lanczos_hubbard_tj_vs_dense():
     for bk, n, sec, terms, st_kw in [
         (O.HUBBARD, 4, dict(use_nup=1, nup=2, use_ndn=1, ndn=2), models.hubbard_chain(4, 1.0, 4.0),

@@ -1,8 +1,9 @@
 """Independent textbook constructions used to pin the oracle and the engine:
 dense operators from the definitions (Kronecker / Jordan-Wigner on the bit
 conventions of the reference: spin up = bit 1; Hubbard up at bit 2i, down
-at 2i+1), the Bethe-ansatz ground state of the XXX ring, and exact thermal
-and resolvent quantities by dense diagonalisation."""
+at 2i+1), the Bethe-ansatz ground states of the XXX and XXZ rings, the
+Lieb-Wu ground state of the Hubbard ring, and exact thermal and resolvent
+quantities by dense diagonalisation."""
 import math
 
 import numpy as np
@@ -130,6 +131,64 @@ def bethe_xxx_ring(N):
             break
         lam = 0.5 * lam + 0.5 * new
     return N / 4.0 - np.sum(2.0 / (1.0 + 4.0 * lam ** 2))
+
+
+def bethe_xxz_ring(L, delta, sz2=0):
+    """Ground-state energy of the periodic XXZ ring
+    H = sum_i [Sx_i Sx_{i+1} + Sy_i Sy_{i+1} + delta Sz_i Sz_{i+1}] for
+    0 < delta < 1 (delta = cos gamma), L even, in the sector 2 Sz = sz2:
+    M = L/2 - |Sz| down spins with rapidities lam_j solving
+    L theta1(lam_j) = 2 pi I_j + sum_k theta2(lam_j - lam_k),
+    theta_n(x) = 2 atan(cot(n gamma / 2) tanh x), I_j = -(M-1)/2 + j;
+    E = L delta / 4 - sum_j sin^2 gamma / (cosh 2 lam_j - cos gamma)."""
+    if not 0.0 < delta < 1.0 or L % 2:
+        raise ValueError("critical regime 0 < delta < 1 and even L only")
+    M = L // 2 - abs(sz2) // 2
+    g = math.acos(delta)
+    I = np.array([-(M - 1) / 2 + j for j in range(M)])
+    t1, cg = math.tan(g / 2), 1.0 / math.tan(g)
+    lam = np.arctanh(np.tan(np.pi * I / L) * t1)
+    for _ in range(200000):
+        d = lam[:, None] - lam[None, :]
+        x = (2 * np.pi * I + 2 * np.arctan(cg * np.tanh(d)).sum(axis=1)) / L
+        new = np.arctanh(np.tan(x / 2) * t1)
+        if np.max(np.abs(new - lam)) < 1e-15:
+            lam = new
+            break
+        lam = 0.5 * lam + 0.5 * new
+    return L * delta / 4 - np.sum(math.sin(g) ** 2 / (np.cosh(2 * lam) - math.cos(g)))
+
+
+def lieb_wu_ring(L, nup, ndn, U, t=1.0):
+    """Ground-state energy of the periodic Hubbard ring
+    H = -t sum (c+_i c_j + h.c.) + U sum n_up n_dn with N = nup + ndn
+    electrons, M = min(nup, ndn), from the Lieb-Wu equations (u = U / 4t)
+    L k_j = 2 pi I_j - sum_a 2 atan((sin k_j - L_a) / u),
+    sum_j 2 atan((L_a - sin k_j) / u) = 2 pi J_a + sum_b 2 atan((L_a - L_b) / 2u),
+    with consecutive I_j (integer iff M even) and J_a (integer iff N - M
+    odd) centred on zero; E = -2 t sum_j cos k_j. Used at half filling,
+    where this is the ground state."""
+    from scipy.optimize import root
+    N, M = nup + ndn, min(nup, ndn)
+    u = U / (4.0 * t)
+    I = np.array([-(N - 1) / 2 + j for j in range(N)])
+    if (I[0] % 1 == 0) != (M % 2 == 0):
+        I = I + 0.5
+    J = np.array([-(M - 1) / 2 + a for a in range(M)])
+    if (J[0] % 1 == 0) != ((N - M) % 2 == 1):
+        J = J + 0.5
+
+    def F(x):
+        k, lam = x[:N], x[N:]
+        f1 = L * k - 2 * np.pi * I + 2 * np.arctan((np.sin(k)[:, None] - lam[None, :]) / u).sum(axis=1)
+        f2 = (2 * np.arctan((lam[:, None] - np.sin(k)[None, :]) / u).sum(axis=1) - 2 * np.pi * J
+              - 2 * np.arctan((lam[:, None] - lam[None, :]) / (2 * u)).sum(axis=1))
+        return np.concatenate([f1, f2])
+
+    sol = root(F, np.concatenate([2 * np.pi * I / L, 0.3 * J]), method="hybr", tol=1e-15)
+    if np.max(np.abs(F(sol.x))) > 1e-11:
+        raise RuntimeError("Lieb-Wu equations did not converge")
+    return -2.0 * t * np.sum(np.cos(sol.x[:N]))
 
 
 def thermal(evals, betas):
