@@ -330,7 +330,7 @@ def stream_copy_gbs(qa, L, gib=2.0, reps=5):
 
 def config_flags(args):
     """Command-line flags that reproduce this run's split and launch tuning."""
-    f = ["--split-bits", str(args.split_bits), "--nvec", str(args.nvec)]
+    f = ["--split-bits", str(args.split_bits), "--nvec", str(args.nvec), "--apply", args.apply]
     if args.sched is not None:
         f += ["--sched", str(args.sched)]
     if args.block_waves is not None:
@@ -365,7 +365,8 @@ def measure_traffic(args, flags):
             cmd = ["timeout", "-s", "KILL", "300", prof, "--pmc", ctr, "-d", out, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--workload", args.workload] + flags + [
                    "--steps", "2", "--warmup", "0", "--no-lanczos",
-                   "--no-cpu", "--no-traffic", "--no-stream", "--no-extras", "--copy-calib-gib", str(calib_gib)]
+                   "--no-cpu", "--no-traffic", "--no-stream", "--no-extras", "--no-autotune",
+                   "--copy-calib-gib", str(calib_gib)]
             log(f"[bench] traffic pass {ctr}: {' '.join(cmd[3:])}")
             r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
                                text=True)
@@ -373,8 +374,17 @@ def measure_traffic(args, flags):
             if r.returncode != 0 or not os.path.exists(db):
                 return None, f"rocprofv3 --pmc {ctr} exited {r.returncode}: {r.stderr.strip()[-300:]}"
             dbs[ctr] = db
-        res = profiling.calibrated_traffic(dbs["FETCH_SIZE"], dbs["WRITE_SIZE"], calib_gib * (1 << 30),
-                                             "k_apply_block<" if args.nvec > 1 else "k_apply<")
+        mode = flags[flags.index("--apply") + 1] if "--apply" in flags else "auto"
+        res = None
+        if mode != "public" and args.nvec == 1:
+            try:
+                res = profiling.calibrated_traffic_split(dbs["FETCH_SIZE"], dbs["WRITE_SIZE"], calib_gib * (1 << 30))
+            except RuntimeError:
+                if mode == "split":
+                    raise
+        if res is None:
+            res = profiling.calibrated_traffic(dbs["FETCH_SIZE"], dbs["WRITE_SIZE"], calib_gib * (1 << 30),
+                                                 "k_apply_block<" if args.nvec > 1 else "k_apply<")
         return res["bytes_per_dispatch"], res
     except Exception as e:          # a counter run that cannot be read leaves traffic unmeasured
         return None, f"{type(e).__name__}: {e}"
@@ -397,6 +407,10 @@ def main():
     ap.add_argument("--lanczos-iters", type=int, default=None)
     ap.add_argument("--no-lanczos", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--apply", choices=["auto", "split", "public"], default="auto",
+                    help="timed apply: the single-pass apply in the public row order (what the Lanczos loop runs; "
+                         "'auto') or the two-pass split apply in its relabelled row order ('split', real spin "
+                         "models that admit it)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the C1 Lanczos, C4 batched FTLM / S(q, omega) and C5 serving legs of the default run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -472,6 +486,14 @@ def main():
         tune["block_rows"] = args.block_rows
     H.set_tuning(**tune)
     tune_report = None
+    want_split = args.nvec == 1 and (args.apply == "split" or (args.apply == "auto" and not args.no_extras))
+    split = H.split_info(any_size=True) if want_split else None
+    if args.apply == "split" and split is None:
+        raise SystemExit(f"--apply split: {args.workload} admits no split")
+    if args.apply == "auto":
+        # the single pass: on this image the two passes measured slower on
+        # the J1-J2 4x8 (62 vs 53 ms) and the 32-site ring (25 vs 16 ms)
+        args.apply = "public"
     if args.autotune:
         t0 = time.perf_counter()
         tune_report = H.autotune(nvec=args.nvec, allow_split=args.split_bits < 0)
@@ -480,19 +502,31 @@ def main():
         if want_traffic:
             # counter runs as child processes with the configuration chosen here
             t = H.tuning()
-            flags = ["--split-bits", str(H.info["NL"]), "--nvec", str(args.nvec),
+            flags = ["--split-bits", str(H.info["NL"]), "--nvec", str(args.nvec), "--apply", args.apply,
                      "--apply-waves", str(t["apply_waves"]), "--block-waves", str(t["block_waves"]),
                      "--block-rows", str(t["block_rows"])]
             sched = t["block_schedule"] if args.nvec > 1 else t["schedule"]
             flags += ["--sched", str(sched)]
             traffic, traffic_info = measure_traffic(args, flags)
             log(f"[bench] traffic: {traffic_info if traffic is None else f'{traffic / 1e9:.3f} GB per apply'}")
-    ms = H.time_apply(args.nvec, args.warmup, args.steps)
+    ms_public = ms_split = None
+    if args.apply == "public" and split is not None:
+        ms_split = H.time_split_apply(args.warmup, args.steps)
+        log(f"[bench] two-pass split apply (relabelled row order): {ms_split:.3f} ms/apply")
+    if args.apply == "split":
+        if not args.no_extras:
+            ms_public = H.time_apply(1, args.warmup, args.steps)
+            log(f"[bench] single-pass apply (public row order): {ms_public:.3f} ms/apply")
+        t0 = time.perf_counter()
+        ms = H.time_split_apply(args.warmup, args.steps)
+    else:
+        ms = H.time_apply(args.nvec, args.warmup, args.steps)
     L.qkrylov_amd_synchronize()
     wall = time.perf_counter() - t0
     alg_bytes = 16.0 * D * args.nvec
     gbs = alg_bytes / (ms * 1e-3) / 1e9
-    log(f"[bench] apply: {ms:.3f} ms/apply -> {gbs:.1f} GB/s algorithmic ({wall:.2f}s wall incl. warmup)")
+    log(f"[bench] apply ({args.apply}): {ms:.3f} ms/apply -> {gbs:.1f} GB/s algorithmic "
+        f"({wall:.2f}s wall incl. warmup)")
 
     out = {
         "metric": "H.x apply HBM throughput (algorithmic bytes, FP64)",
@@ -509,13 +543,27 @@ def main():
         "data": "synthetic (x uniform[-1,1) from the counter RNG)",
         "config": {"workload": w["desc"], "D": D, "terms": len(terms), "columns": args.nvec, "NL": H.info["NL"],
                    "tuning": H.tuning(), "autotune": tune_report,
-                   "offdiag_branches": H.info["n_offdiag_branches"], "setup_s": round(t_setup, 3)},
+                   "offdiag_branches": H.info["n_offdiag_branches"], "setup_s": round(t_setup, 3),
+                   "apply": ("two-pass split apply in the relabelled basis' row order (the Lanczos loop's apply), "
+                             f"site groups NL/nB/nC = {split}") if args.apply == "split"
+                            else "single-pass apply in the public row order"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 5),
                      "traffic": None if traffic is None else int(traffic),
                      "algorithmic_bytes_per_launch": int(alg_bytes)},
     }
 
+    if ms_split is not None:
+        out["apply_split_two_pass"] = {"ms": round(ms_split, 4),
+                                       "GBs_alg": round(alg_bytes / (ms_split * 1e-3) / 1e9, 2),
+                                       "site_groups_NL_nB_nC": list(split),
+                                       "note": "the same apply as two passes over the relabelled basis "
+                                               "(qkrylov_hamiltonian_time_split_apply); not the default, slower here"}
+    if ms_public is not None:
+        out["apply_public_order"] = {"ms": round(ms_public, 4),
+                                     "GBs_alg": round(alg_bytes / (ms_public * 1e-3) / 1e9, 2),
+                                     "note": "single-pass apply on vectors in the public (reference) row order, "
+                                             "the kernel of qkrylov_hamiltonian_apply_device_f64"}
     if not args.no_stream:
         copy_gbs = stream_copy_gbs(qa, L)
         out["roofline"]["stream_copy_GBs"] = round(copy_gbs, 1)
@@ -530,6 +578,10 @@ def main():
             "fetch_bytes": int(traffic_info["FETCH_SIZE"]["bytes_per_dispatch"]),
             "write_bytes": int(traffic_info["WRITE_SIZE"]["bytes_per_dispatch"]),
             "traffic_over_algorithmic": round(traffic / alg_bytes, 2)}
+        if "pass_bytes" in traffic_info["FETCH_SIZE"]:
+            out["roofline"]["traffic_detail"]["per_pass_bytes"] = [
+                int(traffic_info["FETCH_SIZE"]["pass_bytes"][i] + traffic_info["WRITE_SIZE"]["pass_bytes"][i])
+                for i in range(2)]
     else:
         out["roofline"]["traffic_detail"] = {"source": traffic_info}
 

@@ -82,7 +82,9 @@ enum {
     QKRYLOV_TUNE_BLOCK_SCHEDULE = 1,
     QKRYLOV_TUNE_APPLY_WAVES = 2,
     QKRYLOV_TUNE_BLOCK_WAVES = 3,
-    QKRYLOV_TUNE_BLOCK_ROWS = 4
+    QKRYLOV_TUNE_BLOCK_ROWS = 4,
+    QKRYLOV_TUNE_SPLIT_SCHEDULE = 5,   /* tile schedule of both passes of the split apply (default 0) */
+    QKRYLOV_TUNE_SPLIT_WAVES = 6       /* waves per workgroup of both passes (default 16) */
 };
 int qkrylov_hamiltonian_set_tuning(qkrylov_hamiltonian_h h, int knob, int value);
 
@@ -124,6 +126,24 @@ int qkrylov_hamiltonian_states(qkrylov_hamiltonian_h h, uint64_t row0, uint64_t 
  * mean per-apply time of the events bracket (ms) */
 int qkrylov_hamiltonian_time_apply(qkrylov_hamiltonian_h h, int nvec, int warmup, int reps, double* mean_ms);
 
+/* ---- two-pass apply (real spin-1/2 models, 32 sites at most): the sites
+ * are relabelled into groups [B | C | A] (A the low bits) such that every
+ * off-diagonal term acts inside A u B or inside A u C; pass 1 applies the
+ * A u B terms and the diagonal with the C bits fixed, pass 2 the rest with
+ * the B bits fixed, so each pass walks units of at most 21 sites that stay
+ * in one XCD's L2. Vectors of the relabelled basis are a permutation of the
+ * public ones; the Lanczos driver keeps its vectors in that order. ---- */
+/* NL, nB, nC of the split (0, 0, 0 when the model admits none; below 2^24
+ * rows only when any_size is set) */
+int qkrylov_hamiltonian_split_info(qkrylov_hamiltonian_h h, int any_size, int* NL, int* nB, int* nC);
+/* y = H x through the two passes, public row order in and out (permutes
+ * both; allocates three vectors) */
+int qkrylov_hamiltonian_apply_split_device_f64(qkrylov_hamiltonian_h h, const double* x_dev, double* y_dev,
+                                               void* stream);
+/* mean time of the two passes on vectors in the internal order (HIP events
+ * on the library stream, as qkrylov_hamiltonian_time_apply) */
+int qkrylov_hamiltonian_time_split_apply(qkrylov_hamiltonian_h h, int warmup, int reps, double* mean_ms);
+
 /* ---- Lanczos driver ---- */
 typedef struct {
     int maxiter;          /* reference default 200 */
@@ -135,6 +155,10 @@ typedef struct {
     int check_every;      /* 0: automatic */
     int graph_mode;       /* 0: automatic (replay a captured six-step graph below
                            * 2^22 rows when no basis is stored), 1: always, 2: never */
+    int split_mode;       /* two-pass apply over a relabelled basis (real spin-1/2
+                           * models whose terms admit it): 1 use it when admitted;
+                           * 0 (automatic) and 2 the single-pass apply, which
+                           * measured faster on the J1-J2 4x8 and 32-site ring */
 } qkrylov_lanczos_options_t;
 
 typedef struct {
@@ -145,6 +169,7 @@ typedef struct {
     int n_alpha, n_beta;
     double loop_seconds;
     double ritz_seconds;
+    int split;            /* 1: the two-pass apply ran */
 } qkrylov_lanczos_report_t;
 
 /* start: optional host start vector (real for real H, interleaved complex

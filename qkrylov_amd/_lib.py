@@ -47,12 +47,14 @@ class HamInfo(ctypes.Structure):
 
 class LanczosOptions(ctypes.Structure):
     _fields_ = [("maxiter", c_int), ("tol", c_double), ("reorthogonalize", c_int), ("fixed_steps", c_int),
-                ("want_vector", c_int), ("seed", c_u64), ("check_every", c_int), ("graph_mode", c_int)]
+                ("want_vector", c_int), ("seed", c_u64), ("check_every", c_int), ("graph_mode", c_int),
+                ("split_mode", c_int)]
 
 
 class LanczosReport(ctypes.Structure):
     _fields_ = [("energy", c_double), ("iterations", c_int), ("converged", c_int), ("steps_run", c_int),
-                ("n_alpha", c_int), ("n_beta", c_int), ("loop_seconds", c_double), ("ritz_seconds", c_double)]
+                ("n_alpha", c_int), ("n_beta", c_int), ("loop_seconds", c_double), ("ritz_seconds", c_double),
+                ("split", c_int)]
 
 
 class ServeRequest(ctypes.Structure):
@@ -173,6 +175,9 @@ SIGNATURES = {
                                   p_dbl]),
     "qkrylov_structure_factor_f64": (c_int, [c_vp, P(c_vp), c_int, p_dbl, c_int, c_double, c_int, p_dbl, c_int,
                                              c_double, p_dbl, p_dbl, p_dbl, p_dbl, p_int, p_dbl]),
+    "qkrylov_hamiltonian_split_info": (c_int, [c_vp, c_int, p_int, p_int, p_int]),
+    "qkrylov_hamiltonian_apply_split_device_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "qkrylov_hamiltonian_time_split_apply": (c_int, [c_vp, c_int, c_int, p_dbl]),
     # host eigensolvers
     "qkrylov_amd_tridiag_eigen": (c_int, [c_int, p_dbl, p_dbl, p_dbl, p_dbl, c_int]),
     "qkrylov_amd_tridiag_lowest": (c_int, [c_int, p_dbl, p_dbl, p_dbl]),

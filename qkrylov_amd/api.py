@@ -505,14 +505,16 @@ class MatrixFreeHamiltonian:
         check(_lib.load().qkrylov_hamiltonian_states(self._h, row0, n, out.ctypes.data_as(_lib.p_u64)), "states")
         return out
 
-    TUNING_KNOBS = {"schedule": 0, "block_schedule": 1, "apply_waves": 2, "block_waves": 3, "block_rows": 4}
+    TUNING_KNOBS = {"schedule": 0, "block_schedule": 1, "apply_waves": 2, "block_waves": 3, "block_rows": 4,
+                    "split_schedule": 5, "split_waves": 6}
 
     def set_tuning(self, **knobs):
         """Launch tuning of the apply kernels (same apply output, different
         speed): schedule / block_schedule (0 one tile segment per XCD, 1 one
         front over all XCDs, 2 one front of XCD-contiguous chunks),
         apply_waves (4, 8, 16), block_waves (16, 8, 4), block_rows (64, 32,
-        16 rows per work item of the multi-vector apply)."""
+        16 rows per work item of the multi-vector apply); split_schedule and
+        split_waves set both passes of the split apply."""
         for k, v in knobs.items():
             check(_lib.load().qkrylov_hamiltonian_set_tuning(self._h, self.TUNING_KNOBS[k], int(v)), f"set_tuning {k}")
 
@@ -528,10 +530,30 @@ class MatrixFreeHamiltonian:
     def tuning(self):
         out = {}
         for k, i in self.TUNING_KNOBS.items():
+            if i >= 5:   # the split apply's knobs: see split_tuning()
+                continue
             v = ctypes.c_int()
             check(_lib.load().qkrylov_hamiltonian_get_tuning(self._h, i, ctypes.byref(v)), "get_tuning")
             out[k] = v.value
         return out
+
+    def split_info(self, any_size=True):
+        """(NL, nB, nC) of the two-pass apply's site groups, or None when the
+        model admits no split (below 2^24 rows only with any_size)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(_lib.load().qkrylov_hamiltonian_split_info(self._h, 1 if any_size else 0, ctypes.byref(a),
+                                                         ctypes.byref(b), ctypes.byref(c)), "split_info")
+        return (a.value, b.value, c.value) if a.value else None
+
+    def apply_split_device(self, x_ptr, y_ptr, stream=None):
+        """y = H x through the two-pass apply (public row order in and out)."""
+        check(_lib.load().qkrylov_hamiltonian_apply_split_device_f64(self._h, x_ptr, y_ptr, stream), "apply_split")
+
+    def time_split_apply(self, warmup=2, reps=10):
+        ms = ctypes.c_double()
+        check(_lib.load().qkrylov_hamiltonian_time_split_apply(self._h, warmup, reps, ctypes.byref(ms)),
+              "time_split_apply")
+        return ms.value
 
     def time_apply(self, nvec=1, warmup=2, reps=10):
         ms = ctypes.c_double()
@@ -557,15 +579,19 @@ class LanczosResult:
 
 
 def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return_state=True, seed=1234,
-                         fixed_steps=False, start=None, check_every=0, graph=None):
+                         fixed_steps=False, start=None, check_every=0, graph=None, split=None):
     """Ground state energy and, as in bindings/python/qkrylov/solvers.py:24-51,
     the eigenvector (complex, `H.dimension` entries; return_state=False
     skips it). graph: None automatic (replay a captured step graph below
-    2^22 rows when no Krylov basis is stored), True always, False never."""
+    2^22 rows when no Krylov basis is stored), True always, False never.
+    split: True runs the two-pass apply over a relabelled basis (real
+    spin-1/2 models that admit it); None (automatic) and False the
+    single-pass apply; report["split"] tells which ran."""
     L = _lib.load()
     gm = 0 if graph is None else (1 if graph else 2)
+    sm = 0 if split is None else (1 if split else 2)
     o = _lib.LanczosOptions(int(maxiter), float(tol), 1 if reorthogonalize else 0, 1 if fixed_steps else 0,
-                            1 if return_state else 0, int(seed), int(check_every), gm)
+                            1 if return_state else 0, int(seed), int(check_every), gm, sm)
     al = np.zeros(max(maxiter, 1))
     be = np.zeros(max(maxiter, 1))
     n = H.dimension if H.is_real else 2 * H.dimension

@@ -24,6 +24,7 @@ SOURCES = [
     "qk_c_api.cpp",
     "qk_serving.cpp",
     "qk_eigen.cpp",
+    "qk_split.cpp",
 ]
 
 

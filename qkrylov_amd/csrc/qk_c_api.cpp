@@ -8,6 +8,7 @@
 
 #include "qk_eigen.hpp"
 #include "qk_engine.hpp"
+#include "qk_split.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -799,6 +800,65 @@ int qkrylov_hamiltonian_time_apply(qkrylov_hamiltonian_h h, int nvec, int warmup
     QK_CATCH
 }
 
+int qkrylov_hamiltonian_split_info(qkrylov_hamiltonian_h h, int any_size, int* NL, int* nB, int* nC)
+{
+    const auto* H = H_of(h);
+    if (!H || !NL || !nB || !nC) return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    const auto* sp = H->engine()->split_engines(any_size ? 0 : qk::Engine::kSplitMinRows);
+    *NL = sp ? sp->plan().NL : 0;
+    *nB = sp ? sp->plan().nB : 0;
+    *nC = sp ? sp->plan().nC : 0;
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
+int qkrylov_hamiltonian_apply_split_device_f64(qkrylov_hamiltonian_h h, const double* x, double* y, void* stream)
+{
+    const auto* H = H_of(h);
+    if (!H || !x || !y) return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    const auto* sp = H->engine()->split_engines(0);
+    if (!sp) throw std::invalid_argument("this model admits no split apply");
+    const size_t D = H->dimension();
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : qk::default_stream();
+    qk::DevBuf xi(D * sizeof(double)), yi(D * sizeof(double)), t(D * sizeof(double));
+    sp->to_internal(x, xi.d(), s);
+    sp->apply(xi.d(), yi.d(), t.d(), s);
+    sp->to_public(yi.d(), y, s);
+    qk::hip_check(hipStreamSynchronize(s), "sync");
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
+int qkrylov_hamiltonian_time_split_apply(qkrylov_hamiltonian_h h, int warmup, int reps, double* mean_ms)
+{
+    const auto* H = H_of(h);
+    if (!H || reps <= 0 || !mean_ms) return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    const auto* sp = H->engine()->split_engines(0);
+    if (!sp) throw std::invalid_argument("this model admits no split apply");
+    const size_t D = H->dimension();
+    hipStream_t s = qk::default_stream();
+    qk::DevBuf x(D * sizeof(double)), y(D * sizeof(double)), t(D * sizeof(double));
+    if (qkl_fill_random(0, D, 7, 0, 1, 1, 0, x.d(), s) != 0) throw std::runtime_error("fill failed");
+    for (int i = 0; i < warmup; ++i) sp->apply(x.d(), y.d(), t.d(), s);
+    hipEvent_t e0, e1;
+    qk::hip_check(hipEventCreate(&e0), "event");
+    qk::hip_check(hipEventCreate(&e1), "event");
+    qk::hip_check(hipEventRecord(e0, s), "record");
+    for (int i = 0; i < reps; ++i) sp->apply(x.d(), y.d(), t.d(), s);
+    qk::hip_check(hipEventRecord(e1, s), "record");
+    qk::hip_check(hipEventSynchronize(e1), "sync");
+    float ms = 0.0f;
+    qk::hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *mean_ms = (double)ms / reps;
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
 int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_options_t* opt, const double* start,
                             double* alphas, double* betas, double* vector, qkrylov_lanczos_report_t* rep)
 {
@@ -815,8 +875,11 @@ int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_optio
     o.check_every = opt->check_every;
     if (opt->graph_mode < 0 || opt->graph_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
     o.graph = opt->graph_mode == 0 ? -1 : (opt->graph_mode == 1 ? 1 : 0);
+    if (opt->split_mode < 0 || opt->split_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    o.split = opt->split_mode == 0 ? -1 : (opt->split_mode == 1 ? 1 : 0);
     o.start = start;
     auto r = qk::lanczos_run(*H->engine(), o, qk::default_stream());
+    rep->split = r.split ? 1 : 0;
     rep->energy = r.energy;
     rep->iterations = r.iterations;
     rep->converged = r.converged ? 1 : 0;

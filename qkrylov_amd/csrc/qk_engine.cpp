@@ -3,6 +3,7 @@
 // the block recurrence for batched FTLM / CF, and Davidson.
 #include "qk_engine.hpp"
 #include "qk_eigen.hpp"
+#include "qk_split.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -169,7 +170,7 @@ std::shared_ptr<const BasisTables> basis_tables(const BasisSpec& b, int NL)
 // ---------------------------------------------------------------- Engine
 
 Engine::Engine(std::shared_ptr<const BasisSpec> basis, Kind site_kind, const OpSum& ops, EngineOptions opt)
-    : basis_(std::move(basis))
+    : basis_(std::move(basis)), kind_(site_kind), ops_(ops)
 {
     if (basis_->dim() > 0) comp_ = compile_opsum(*basis_, site_kind, ops);
     const int NL = opt.NL >= 0 ? opt.NL : choose_NL_model(*basis_, comp_);
@@ -431,6 +432,11 @@ std::shared_ptr<const Engine> autotune(const std::shared_ptr<const BasisSpec>& b
 
 int Engine::tuning(int knob) const
 {
+    if (knob == 5 || knob == 6) {
+        const SplitEngines* sp = split_engines(0);
+        if (!sp) throw std::invalid_argument("this model admits no split apply");
+        return sp->tuning(knob == 5 ? 0 : 2);
+    }
     switch (knob) {
     case 0: return T_.sched;
     case 1: return T_.sched_block;
@@ -443,6 +449,12 @@ int Engine::tuning(int knob) const
 
 void Engine::set_tuning(int knob, int value) const
 {
+    if (knob == 5 || knob == 6) {
+        const SplitEngines* sp = split_engines(0);
+        if (!sp) throw std::invalid_argument("this model admits no split apply");
+        sp->set_tuning(knob == 5 ? 0 : 2, value);
+        return;
+    }
     const bool sched_ok = value >= 0 && value <= 2;
     const bool waves_ok = value == 4 || value == 8 || value == 16;
     const bool rows_ok = value == 16 || value == 32 || value == 64;
@@ -476,6 +488,19 @@ void Engine::apply_block(const double* X, double* Y, int ld, int ncols, hipStrea
 void Engine::diagonal(double* d, bool cplx, hipStream_t s) const
 {
     lcheck(qkl_diagonal(&T_, cplx ? 1 : 0, wide() ? 1 : 0, d, s), "diagonal");
+}
+
+const SplitEngines* Engine::split_engines(int64_t min_rows) const
+{
+    if (dim() < min_rows || !is_real() || kind_ != Kind::SpinHalf || basis_->kind() != Kind::SpinHalf) return nullptr;
+    std::lock_guard<std::mutex> lk(split_mu_);
+    if (!split_tried_) {
+        split_tried_ = true;
+        SplitPlan plan;
+        // units of at most 21 sites: C(21, 10) rows = 2.8 MB of a vector
+        if (find_split(*basis_, ops_, 21, plan)) split_ = std::make_shared<SplitEngines>(*this, std::move(plan));
+    }
+    return split_.get();
 }
 
 void Engine::apply_diag_operator(const double* x, bool xcplx, double* y, hipStream_t s) const
@@ -561,14 +586,16 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool so, Arena* ar)
-    : E(e), s(st), D((size_t)e.dim()), kcap(kcap_), arena(ar), stream_ordered(so)
+RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool so, Arena* ar,
+                   const SplitEngines* sp)
+    : E(e), s(st), D((size_t)e.dim()), kcap(kcap_), arena(ar), stream_ordered(so), split(sp)
 {
     auto mk = [&](size_t bytes) { return ar ? ar->take(bytes) : (so ? DevBuf(bytes, st) : DevBuf(bytes)); };
     for (auto& q : Q) q = mk(D * sizeof(double));
+    if (split) tmp = mk(D * sizeof(double));
     alpha = mk((kcap + 2) * sizeof(double));
     nrm = mk((kcap + 3) * sizeof(double));
-    npA = qkl_apply_partials(E.tables(), E.wide() ? 1 : 0);
+    npA = split ? qkl_apply_partials_acc(split->tables2()) : qkl_apply_partials(E.tables(), E.wide() ? 1 : 0);
     if (npA <= 0) throw std::runtime_error("apply: the operator tables do not fit in LDS");
     NB = qkl_blas_partials();
     partA = mk(npA * sizeof(double));
@@ -588,7 +615,8 @@ RealCore::~RealCore()
 void RealCore::steps_graph(int k, int n)
 {
     if (n % 3 != 0 || k + n > kcap) throw std::logic_error("graph steps: n must be a multiple of 3 within the cap");
-    if (qkl_apply_partials(E.tables(), E.wide() ? 1 : 0) != npA) throw std::logic_error("apply launch shape changed during a run");
+    if (!split && qkl_apply_partials(E.tables(), E.wide() ? 1 : 0) != npA)
+        throw std::logic_error("apply launch shape changed during a run");
     if (gexec && ic != cap_ic) {
         // eager steps moved the rotation: capture again for the current roles
         (void)hipGraphExecDestroy(gexec);
@@ -611,8 +639,7 @@ void RealCore::steps_graph(int k, int n)
         for (int i = 0; i < 6 && rc == 0; ++i) {
             int* kc = kd + i / 3;
             int* kn = (i % 3 == 2) ? kd + (1 - i / 3) : nullptr;
-            rc |= qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), 0, kc, i % 3, prv(), nrm.d(), partB.d(),
-                                    NB, partA.d(), npA, s);
+            rc |= lanczos_apply(0, kc, i % 3);
             rc |= qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), 0, kc, i % 3, partA.d(), npA, partB.d(), kn,
                                       s);
             const int t = ip;
@@ -655,14 +682,25 @@ void RealCore::flush()
 // one step k: nrm[k] from the previous partials; w = H v_k - beta v_{k-1};
 // alpha_k; q_{k+1} and the partial sums of |q_{k+1}|^2 (nrm[k+1] is totalled
 // by the next step's apply, or by flush())
+int RealCore::lanczos_apply(int k, const int* kdev, int koff)
+{
+    if (split) {
+        const int rc = qkl_apply(split->tables1(), 0, 0, cur(), tmp.d(), s);
+        if (rc != 0) return rc;
+        return qkl_lanczos_apply_acc(split->tables2(), cur(), wrk(), k, kdev, koff, prv(), nrm.d(), partB.d(), NB,
+                                     partA.d(), npA, tmp.d(), s);
+    }
+    return qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), k, kdev, koff, prv(), nrm.d(), partB.d(), NB,
+                             partA.d(), npA, s);
+}
+
 void RealCore::step(int k, const double* basis)
 {
     // the partial buffer was sized for the launch shape of the tuning at
     // construction; a retuned engine would write past it
-    if (qkl_apply_partials(E.tables(), E.wide() ? 1 : 0) != npA) throw std::logic_error("apply launch shape changed during a run");
-    lcheck(qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), k, nullptr, 0, prv(), nrm.d(), partB.d(), NB,
-                             partA.d(), npA, s),
-           "apply");
+    if (!split && qkl_apply_partials(E.tables(), E.wide() ? 1 : 0) != npA)
+        throw std::logic_error("apply launch shape changed during a run");
+    lcheck(lanczos_apply(k, nullptr, 0), "apply");
     lcheck(qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), k, nullptr, 0, partA.d(), npA, partB.d(), nullptr, s),
            "update");
     if (basis) {
@@ -734,18 +772,29 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
             throw std::runtime_error("full reorthogonalisation needs more device memory than is free");
     }
     auto mk = [&](size_t bytes) { return arena ? arena->take(bytes) : DevBuf(bytes); };
+    // the two-pass apply keeps every Krylov vector in its internal row
+    // order: a given start vector is permuted in, the Ritz vector out; a
+    // random start is drawn directly in that order
+    // (automatic = the single pass: on the J1-J2 4x8 and 32-site-ring
+    // workloads the two passes measured slower, 62 vs 53 ms and 25 vs 16 ms)
+    const SplitEngines* sp = nullptr;
+    if (!arena && o.split > 0) sp = E.split_engines(0);
     DevBuf basis;
     if (store) basis = mk(basis_bytes);
     const size_t core_mark = arena ? arena->mark() : 0;
-    auto Cp = std::make_unique<RealCore>(E, kmax, s, reorth, false, arena);
+    auto Cp = std::make_unique<RealCore>(E, kmax, s, reorth, false, arena, sp);
     RealCore& C = *Cp;
+    out.split = sp != nullptr;
 
-    auto load_start = [&](double* dst) {
-        if (o.start_dev) hip_check(hipMemcpyAsync(dst, o.start_dev, D * sizeof(double), hipMemcpyDeviceToDevice, s), "start");
-        else if (o.start) hip_check(hipMemcpyAsync(dst, o.start, D * sizeof(double), hipMemcpyHostToDevice, s), "start");
+    // dst: the recurrence's start; stage: a scratch vector for the permutation
+    auto load_start = [&](double* dst, double* stage) {
+        double* tgt = (sp && (o.start_dev || o.start)) ? stage : dst;
+        if (o.start_dev) hip_check(hipMemcpyAsync(tgt, o.start_dev, D * sizeof(double), hipMemcpyDeviceToDevice, s), "start");
+        else if (o.start) hip_check(hipMemcpyAsync(tgt, o.start, D * sizeof(double), hipMemcpyHostToDevice, s), "start");
         else lcheck(qkl_fill_random(0, (size_t)D, o.seed, 0, 1, 1, 0, dst, s), "random start");
+        if (tgt != dst) sp->to_internal(tgt, dst, s);
     };
-    load_start(C.cur());
+    load_start(C.cur(), C.wrk());
     C.init_norm();
     if (store) lcheck(qkl_copy_scaled((size_t)D, C.cur(), basis.d(), C.nrm.d(), 0, s), "store v0");
 
@@ -808,8 +857,8 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
         } else {
             // second pass: rerun the same deterministic recurrence and
             // accumulate sum_k z_k v_k (v_k = q_k / nrm_k)
-            RealCore R(E, n, s, false, false, arena);
-            load_start(R.cur());
+            RealCore R(E, n, s, false, false, arena, sp);
+            load_start(R.cur(), R.wrk());
             R.init_norm();
             for (int kk = 0; kk < n; ++kk) {
                 lcheck(qkl_axpy(0, (size_t)D, z[kk] / hn[kk], 0.0, R.cur(), ritz.d(), s), "axpy");
@@ -822,8 +871,15 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
         DevBuf part = mk(qkl_blas_partials() * 2 * sizeof(double));
         const double nr = std::sqrt(dot_host(false, (size_t)D, ritz.d(), ritz.d(), part.d(), tmp.d(), s).real());
         if (nr != 0.0) lcheck(qkl_scal(0, (size_t)D, 1.0 / nr, 0.0, ritz.d(), s), "scal");
+        const double* vout = ritz.d();
+        DevBuf pubv;
+        if (sp) {
+            pubv = mk(D * sizeof(double));
+            sp->to_public(ritz.d(), pubv.d(), s);
+            vout = pubv.d();
+        }
         out.vector.resize(D);
-        hip_check(hipMemcpyAsync(out.vector.data(), ritz.get(), D * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        hip_check(hipMemcpyAsync(out.vector.data(), vout, D * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
         hip_check(hipStreamSynchronize(s), "sync");
         out.ritz_seconds = now_s() - t1;
     }

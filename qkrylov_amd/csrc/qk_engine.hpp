@@ -7,10 +7,13 @@
 #include <hip/hip_runtime_api.h>
 
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 namespace qk {
+
+class SplitEngines;
 
 void hip_check(hipError_t e, const char* what);
 
@@ -110,8 +113,19 @@ public:
     // y = O x for a diagonal operator compiled into this engine
     void apply_diag_operator(const double* x, bool xcplx, double* y, hipStream_t s) const;
 
+    // The two-pass apply over a relabelled basis (qk_split.hpp) when this
+    // model admits one and has at least min_rows rows, else null; built on
+    // the first call and kept with the engine.
+    static constexpr int64_t kSplitMinRows = int64_t(1) << 24;
+    const SplitEngines* split_engines(int64_t min_rows = kSplitMinRows) const;
+
 private:
     std::shared_ptr<const BasisSpec> basis_;
+    Kind kind_;
+    OpSum ops_;
+    mutable std::mutex split_mu_;
+    mutable std::shared_ptr<const SplitEngines> split_;
+    mutable bool split_tried_ = false;
     Compiled comp_;
     std::shared_ptr<const BasisTables> tables_;
     DevBuf br_, brk_, dg_;
@@ -174,9 +188,13 @@ struct RealCore {
 
     Arena* arena = nullptr;   // set: every device buffer comes from it
     bool stream_ordered = false;   // buffers from hipMallocAsync on s
+    // set: the apply runs as the two passes of the split engines, in their
+    // internal row order (tmp holds the first pass)
+    const SplitEngines* split = nullptr;
+    DevBuf tmp;
 
     RealCore(const Engine& e, int kcap, hipStream_t st, bool reorth, bool stream_ordered = false,
-             Arena* arena = nullptr);
+             Arena* arena = nullptr, const SplitEngines* split = nullptr);
     ~RealCore();
     RealCore(const RealCore&) = delete;
     RealCore& operator=(const RealCore&) = delete;
@@ -188,6 +206,8 @@ struct RealCore {
     // partB: the next step's apply totals them; flush() does it now (before
     // nrm[k+1] is read on the host or by another kernel)
     void step(int k, const double* basis);   // basis: normalised v_0..v_k for reorthogonalisation
+    // the Lanczos-form apply of step k (k from *kdev + koff when kdev is set)
+    int lanczos_apply(int k, const int* kdev, int koff);
     void flush();
     // steps k .. k + n - 1 (n a multiple of 3, k + n <= kcap) as replays of
     // the captured six-step graph (a remaining three steps eagerly); the
@@ -212,6 +232,8 @@ struct LanczosOptions {
     uint64_t seed = 1234;
     int check_every = 0;         // 0: automatic
     int graph = -1;              // replay captured step graphs: -1 below 2^22 rows, 0 never, 1 always
+    int split = -1;              // two-pass apply (real spin models that admit it, qk_split.hpp): 1 use it,
+                                 // -1 (automatic) and 0 the single pass
     const double* start = nullptr;   // host start vector (real, or interleaved complex for complex H)
     const double* start_dev = nullptr;   // device start vector (same layout)
 };
@@ -225,6 +247,7 @@ struct LanczosOutput {
     double loop_seconds = 0.0;       // wall time of the iteration loop
     double ritz_seconds = 0.0;
     int steps_run = 0;               // device steps computed (>= iterations)
+    bool split = false;              // the two-pass apply ran (qk_split.hpp)
 };
 
 // arena: when given, every device buffer (Krylov vectors, scalars, partial

@@ -334,7 +334,9 @@ __device__ __forceinline__ void lane_branches(int gbeg, int gend, const unsigned
 // alone, its epilogue's partial dot raises the register count past the
 // plain apply's and costs waves per CU in the 8- and 4-wave shapes. Other
 // instances keep the compiler's choice.
-template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
+// ACC: y = acc + (this pass's sum), the second pass of the split apply
+// (qk_split.hpp); real 32-bit spin tables only.
+template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS, bool ACC>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CPLX) ? 6 : 1))) void k_apply(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
                                                    const uint32_t* __restrict__ lrank, const uint16_t* __restrict__ tl,
                                                    const uint32_t* __restrict__ th, int nth,
@@ -345,7 +347,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
                                                    const double* __restrict__ prev, double* __restrict__ nrm,
                                                    double* __restrict__ partials, int table_bytes,
                                                    const int* __restrict__ kdev, int koff,
-                                                   const double* __restrict__ nrm_part, int n_nrm_part)
+                                                   const double* __restrict__ nrm_part, int n_nrm_part,
+                                                   const double* __restrict__ acc)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double wsum[16];
@@ -402,6 +405,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
         // the Lanczos epilogue's q_{k-1} entry, loaded now so that its
         // latency hides behind the tile's gathers
         const double pv = (LZ && lzk > 0) ? __builtin_nontemporal_load(prev + row) : 0.0;
+        const double av = ACC ? __builtin_nontemporal_load(acc + row) : 0.0;
         const ST s = (static_cast<ST>(hi) << NL) | static_cast<ST>(lo);
 
         // (1) the uniform part of every branch, one branch per lane
@@ -483,6 +487,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
 
         // y is written once and not read again by this kernel: stream it
         // past the caches so L2 keeps x
+        if (ACC) ar += av;
         if (active) {
             if (LZ) {
                 double w = ar * inv_n;
@@ -1288,6 +1293,38 @@ __global__ __launch_bounds__(QK_TPB) void k_precond(size_t n, const double* __re
     }
 }
 
+// ------------------------------------------------------------------ basis permutation
+// Rows of a relabelled spin basis <-> rows of the public basis. The internal
+// state (hi << NL) | lo of row `row` maps to the public state bit by bit
+// through four byte tables (bytemap[256 k + v]: the public bits of the
+// internal bits 8k..8k+7 holding v); its public row is
+// hoff_p[s >> NLp] + lrank_p[s & (2^NLp - 1)]. TO_PUB: dst[public row] =
+// src[row], else dst[row] = src[public row]. One wave per tile.
+template <bool TO_PUB>
+__global__ __launch_bounds__(QK_TPB) void k_permute(const QkTile* __restrict__ tiles, uint32_t ntiles,
+                                                     const uint32_t* __restrict__ lolist, int NL,
+                                                     const uint32_t* __restrict__ bytemap,
+                                                     const uint32_t* __restrict__ hoff_p,
+                                                     const uint32_t* __restrict__ lrank_p, int NLp,
+                                                     const double* __restrict__ src, double* __restrict__ dst)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t lmask = (1u << NLp) - 1u;
+    for (uint32_t t = wave; t < ntiles; t += nw) {
+        const QkTile tl = tiles[t];
+        if (lane >= tl.count) continue;
+        const uint32_t row = tl.row_start + lane;
+        const uint32_t si = (tl.hi << NL) | lolist[tl.lo_base + lane];
+        const uint32_t sp = bytemap[si & 255u] | bytemap[256u + ((si >> 8) & 255u)] |
+                            bytemap[512u + ((si >> 16) & 255u)] | bytemap[768u + (si >> 24)];
+        const uint32_t r = hoff_p[sp >> NLp] + lrank_p[sp & lmask];
+        if (TO_PUB) dst[r] = src[row];
+        else dst[row] = src[r];
+    }
+}
+
 // ------------------------------------------------------------------ static workspace
 // The workspace of the entry points that must not allocate (for callers
 // under graph capture or in loops that forbid hipMalloc): one array per GPU,
@@ -1429,24 +1466,38 @@ struct LzArgs {
     int n_part = 0;           // partial sums the caller holds room for
 };
 
-template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
+template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS, bool ACC = false>
 const void* apply_fn()
 {
-    return reinterpret_cast<const void*>(&k_apply<ST, CPLX, RM, LZ, SIGNS>);
+    return reinterpret_cast<const void*>(&k_apply<ST, CPLX, RM, LZ, SIGNS, ACC>);
 }
 
-template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS>
+template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS, bool ACC = false>
 int launch_apply(const QkTables* T, const double* x, double* y, int k, const double* prev, double* nrm,
-                 double* partials, hipStream_t s, const LzArgs& lz)
+                 double* partials, hipStream_t s, const LzArgs& lz, const double* acc = nullptr)
 {
     ApplyCfg c;
-    if (!apply_config(T, CPLX, apply_fn<ST, CPLX, RM, LZ, SIGNS>(), c)) return -2;
+    if (!apply_config(T, CPLX, apply_fn<ST, CPLX, RM, LZ, SIGNS, ACC>(), c)) return -2;
     if (LZ && lz.n_part != c.nb) return -1;   // one partial sum per workgroup
-    hipLaunchKernelGGL((k_apply<ST, CPLX, RM, LZ, SIGNS>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff,
-                       T->lrank, T->tl, T->th, T->nth,
+    hipLaunchKernelGGL((k_apply<ST, CPLX, RM, LZ, SIGNS, ACC>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles,
+                       T->hoff, T->lrank, T->tl, T->th, T->nth,
                        T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T),
-                       lz.kdev, lz.koff, lz.nrm_part, lz.n_nrm_part);
+                       lz.kdev, lz.koff, lz.nrm_part, lz.n_nrm_part, acc);
     return rc(hipGetLastError());
+}
+
+// the accumulating (second-pass) instances: real vectors, 32-bit states,
+// no parity masks, one-level lo rank
+template <bool LZ>
+int launch_apply_acc(const QkTables* T, const double* x, double* y, int k, const double* prev, double* nrm,
+                     double* partials, hipStream_t s, const LzArgs& lz, const double* acc)
+{
+    if (T->signs != 0) return -1;
+    if (T->rank_mode == 1)
+        return launch_apply<uint32_t, false, 1, LZ, false, true>(T, x, y, k, prev, nrm, partials, s, lz, acc);
+    if (T->rank_mode == 0)
+        return launch_apply<uint32_t, false, 0, LZ, false, true>(T, x, y, k, prev, nrm, partials, s, lz, acc);
+    return -1;
 }
 
 template <typename ST, int RM, bool LZ, int NW>
@@ -1527,6 +1578,13 @@ int qkl_apply_partials(const QkTables* T, int wide)
     const void* f = wide ? lz_apply_fn<uint64_t>(T) : lz_apply_fn<uint32_t>(T);
     return apply_config(T, false, f, c) ? c.nb : -2;
 }
+int qkl_apply_partials_acc(const QkTables* T)
+{
+    ApplyCfg c;
+    const void* f = T->rank_mode == 1 ? apply_fn<uint32_t, false, 1, true, false, true>()
+                                      : apply_fn<uint32_t, false, 0, true, false, true>();
+    return apply_config(T, false, f, c) ? c.nb : -2;
+}
 int qkl_block_partials(const QkTables* T)
 {
     BlockCfg c;
@@ -1563,6 +1621,28 @@ int qkl_lanczos_apply(const QkTables* T, int wide, const double* x, double* y, i
     lz.n_nrm_part = n_nrm_part;
     return wide ? apply_any<uint64_t, false>(T, true, x, y, k, prev, nrm, partials, s, lz)
                 : apply_any<uint32_t, false>(T, true, x, y, k, prev, nrm, partials, s, lz);
+}
+
+int qkl_apply_acc(const QkTables* T, const double* x, double* y, const double* acc, qkl_stream sv)
+{
+    if (T->ntiles == 0) return 0;
+    if (!acc) return -1;
+    const LzArgs none;
+    return launch_apply_acc<false>(T, x, y, -1, nullptr, nullptr, nullptr, (hipStream_t)sv, none, acc);
+}
+
+int qkl_lanczos_apply_acc(const QkTables* T, const double* x, double* y, int k, const int* kdev, int koff,
+                          const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
+                          int n_part, const double* acc, qkl_stream sv)
+{
+    if (T->ntiles == 0 || !nrm_part || n_nrm_part <= 0 || !acc) return -1;
+    LzArgs lz;
+    lz.n_part = n_part;
+    lz.kdev = kdev;
+    lz.koff = koff;
+    lz.nrm_part = nrm_part;
+    lz.n_nrm_part = n_nrm_part;
+    return launch_apply_acc<true>(T, x, y, k, prev, nrm, partials, (hipStream_t)sv, lz, acc);
 }
 
 int qkl_lanczos_update1(size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
@@ -1771,6 +1851,22 @@ int qkl_scratch(void** ptr, size_t* bytes)
     *ptr = addr[dev];
     *bytes = QK_SCRATCH_BYTES;
     return 0;
+}
+
+int qkl_permute(const QkTables* Tint, const uint32_t* bytemap, const QkTables* Tpub, int to_pub, const double* src,
+                double* dst, qkl_stream sv)
+{
+    if (Tint->dim != Tpub->dim || Tint->NL <= 0 || Tint->NL >= 32 || Tpub->NL <= 0 || Tpub->NL >= 32) return -1;
+    if (Tint->ntiles == 0) return 0;
+    uint32_t nb = (Tint->ntiles + 3u) / 4u;
+    if (nb > 16384u) nb = 16384u;
+    if (to_pub)
+        hipLaunchKernelGGL(k_permute<true>, dim3(nb), dim3(QK_TPB), 0, (hipStream_t)sv, Tint->tiles, Tint->ntiles,
+                           Tint->lolist, Tint->NL, bytemap, Tpub->hoff, Tpub->lrank, Tpub->NL, src, dst);
+    else
+        hipLaunchKernelGGL(k_permute<false>, dim3(nb), dim3(QK_TPB), 0, (hipStream_t)sv, Tint->tiles, Tint->ntiles,
+                           Tint->lolist, Tint->NL, bytemap, Tpub->hoff, Tpub->lrank, Tpub->NL, src, dst);
+    return rc(hipGetLastError());
 }
 
 } // extern "C"

@@ -108,6 +108,21 @@ int qkl_states(const qk::QkTables* T, int wide, uint64_t r0, uint64_t n, uint64_
  * (used to build S^z_q seeds): cplx x/y interleaved. */
 int qkl_apply_diag_terms(const qk::QkTables* T, int wide, int xcplx, const double* x, double* y, qkl_stream s);
 
+/* Second pass of the split apply (qk_split.hpp): y = acc + H x over these
+ * tables, plain or in the Lanczos form of qkl_lanczos_apply (w formed from
+ * acc + H x). Real vectors, 32-bit states, no parity masks, one-level lo
+ * rank; otherwise -1. qkl_apply_partials_acc: partial sums of the Lanczos
+ * form. */
+int qkl_apply_acc(const qk::QkTables* T, const double* x, double* y, const double* acc, qkl_stream s);
+int qkl_lanczos_apply_acc(const qk::QkTables* T, const double* x, double* y, int k, const int* kdev, int koff,
+                          const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
+                          int n_part, const double* acc, qkl_stream s);
+int qkl_apply_partials_acc(const qk::QkTables* T);
+/* rows of the relabelled basis of Tint <-> rows of the public basis of Tpub
+ * (32-bit states; bytemap: 4 x 256 public-bit tables of the internal bytes) */
+int qkl_permute(const qk::QkTables* Tint, const uint32_t* bytemap, const qk::QkTables* Tpub, int to_pub,
+                const double* src, double* dst, qkl_stream s);
+
 /* Static per-device workspace of the no-allocation entry points: its
  * device address on the current device (looked up once per device) and its
  * size in bytes. */

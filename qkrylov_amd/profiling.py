@@ -95,6 +95,45 @@ def calibrated_traffic(fetch_db, write_db, copy_bytes, match):
     return res
 
 
+def apply_acc_flag(name):
+    """For a k_apply kernel name, its last template argument (the
+    accumulating second pass of the split apply): 'true' / 'false', else
+    None."""
+    i = name.find("k_apply<")
+    if i < 0:
+        return None
+    depth, j0 = 0, i + len("k_apply<")
+    for j in range(j0, len(name)):
+        if name[j] == "<":
+            depth += 1
+        elif name[j] == ">":
+            if depth == 0:
+                return name[j0:j].split(",")[-1].strip()
+            depth -= 1
+    return None
+
+
+def calibrated_traffic_split(fetch_db, write_db, copy_bytes):
+    """HBM-side bytes of one split apply: the mean per dispatch of its first
+    pass (k_apply<..., false>) plus that of its second (k_apply<..., true>),
+    each counter scaled by its k_copy factor as in calibrated_traffic."""
+    res = {}
+    for db, counter in ((fetch_db, "FETCH_SIZE"), (write_db, "WRITE_SIZE")):
+        vals = counter_values(db, counter)
+        copy = [v for _, n, v in vals if "k_copy(" in n]
+        p1 = [v for _, n, v in vals if apply_acc_flag(n) == "false"]
+        p2 = [v for _, n, v in vals if apply_acc_flag(n) == "true"]
+        if not copy or not p1 or not p2:
+            raise RuntimeError(f"{counter}: no k_copy or no split-apply dispatches in {db}")
+        factor = (copy_bytes / 1024.0) / copy[-1]
+        kib = statistics.fmean(p1) + statistics.fmean(p2)
+        res[counter] = dict(kib_per_dispatch=kib, dispatches=len(p1) + len(p2), copy_kib=copy[-1], factor=factor,
+                            bytes_per_dispatch=kib * factor * 1024.0,
+                            pass_bytes=[statistics.fmean(p1) * factor * 1024.0, statistics.fmean(p2) * factor * 1024.0])
+    res["bytes_per_dispatch"] = res["FETCH_SIZE"]["bytes_per_dispatch"] + res["WRITE_SIZE"]["bytes_per_dispatch"]
+    return res
+
+
 def _main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m qkrylov_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)

@@ -225,21 +225,26 @@ def test_full_size_hubbard16_lanczos_vs_lieb_wu():
     assert abs(r.energy - e) < 1e-10 * abs(e)
 
 
-def test_full_size_j1j2_energy_independent_of_split():
+def test_full_size_j1j2_energy_independent_of_apply():
     """C2 exactly as configured (J1-J2 4x8 periodic, D = 601,080,390, 200
     plain iterations, no reorthogonalisation): the energy after 200 steps
-    does not depend on the lo-bit split (which changes the order in which a
-    row's entries are added) within 1e-10 relative, and lies below the
-    energy after 100 steps (variational)."""
+    is the same within 1e-10 relative through the single-pass apply at two
+    lo-bit splits (which change the order in which a row's entries are
+    added) and through the two-pass apply over the relabelled basis (its
+    own row order and start vector), and lies below the energy after 100
+    steps (variational)."""
     terms = models.j1j2_square(8, 4, 1.0, 0.5)
     energies = []
-    for nl in (-1, 12):
+    for nl, split in ((-1, False), (12, False), (-1, True)):
         H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(32, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms), split_bits=nl)
         assert H.dimension == 601080390
-        r = qa.lanczos_ground_state(H, maxiter=200, reorthogonalize=False, fixed_steps=True, return_state=False)
+        r = qa.lanczos_ground_state(H, maxiter=200, reorthogonalize=False, fixed_steps=True, return_state=False,
+                                    split=split)
+        assert r.report["split"] == (1 if split else 0)
         energies.append(r.energy)
         e100 = qa.tridiag_lowest(r.alphas[:100], r.betas[:99])
         assert r.energy <= e100
+        print(f"C2 NL={nl} split={split}: E(200) = {r.energy!r}, loop {r.report['loop_seconds']:.2f} s", flush=True)
         del H
-    print(f"C2: E(200) = {energies[0]!r} (default split), {energies[1]!r} (NL 12)", flush=True)
-    assert abs(energies[0] - energies[1]) < 1e-10 * abs(energies[0])
+    for e in energies[1:]:
+        assert abs(e - energies[0]) < 1e-10 * abs(energies[0])
