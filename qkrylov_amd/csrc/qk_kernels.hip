@@ -351,6 +351,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
                                                    const double* __restrict__ acc)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    // static LDS: QK_APPLY_STATIC_LDS bytes, counted by apply_lds_plan
     __shared__ double wsum[16];
     __shared__ double tot_sh;
     const int nA = sc.nA, nB = sc.nB, nbk = sc.nA + sc.nB;
@@ -1381,28 +1382,42 @@ struct ApplyCfg {
     size_t lds;
 };
 
+// LDS of a gfx950 workgroup, and the static LDS k_apply declares beside its
+// dynamic tables (wsum[16] and tot_sh)
+#define QK_LDS_LIMIT ((size_t)160 * 1024)
+#define QK_APPLY_STATIC_LDS ((size_t)17 * sizeof(double))
+
+extern "C" int qkl_apply_lds_plan(size_t tables, int nB, int nA, int cplx, int pref_waves, int* waves,
+                                  size_t* dyn_lds)
+{
+    for (int wpb = pref_waves; wpb >= 1; wpb /= 2) {
+        const size_t lds = tables + apply_branch_bytes(nB) + (size_t)wpb * apply_wave_bytes(nB, nA, cplx != 0);
+        if (lds + QK_APPLY_STATIC_LDS > QK_LDS_LIMIT) continue;
+        *waves = wpb;
+        *dyn_lds = lds;
+        return 0;
+    }
+    return -1;
+}
+
 inline bool apply_config(const QkTables* T, bool cplx, const void* f, ApplyCfg& c)
 {
     int wpb = (T->apply_waves == 8 || T->apply_waves == 16) ? T->apply_waves : 4;
-    for (; wpb >= 1; wpb /= 2) {
-        const size_t lds = table_bytes(T) + apply_branch_bytes(T->nB) +
-                           (size_t)wpb * apply_wave_bytes(T->nB, T->nA, cplx);
-        if (lds > 160 * 1024) continue;
-        int per_cu = 1, ncu = 256;
-        residency(f, wpb, lds, per_cu, ncu);
-        const uint32_t resident = (uint32_t)per_cu * (uint32_t)ncu;
-        const uint32_t rounds = std::max(1u, (8192u / (uint32_t)wpb) / resident);
-        uint32_t nb = (T->ntiles + wpb - 1) / wpb;
-        const uint32_t maxb = resident * rounds;
-        if (nb > maxb) nb = maxb;
-        if (nb >= 8) nb = (nb + 7u) & ~7u;
-        if (nb == 0) nb = 1;
-        c.wpb = wpb;
-        c.nb = (int)nb;
-        c.lds = lds;
-        return true;
-    }
-    return false;
+    size_t lds = 0;
+    if (qkl_apply_lds_plan(table_bytes(T), T->nB, T->nA, cplx ? 1 : 0, wpb, &wpb, &lds) != 0) return false;
+    int per_cu = 1, ncu = 256;
+    residency(f, wpb, lds, per_cu, ncu);
+    const uint32_t resident = (uint32_t)per_cu * (uint32_t)ncu;
+    const uint32_t rounds = std::max(1u, (8192u / (uint32_t)wpb) / resident);
+    uint32_t nb = (T->ntiles + wpb - 1) / wpb;
+    const uint32_t maxb = resident * rounds;
+    if (nb > maxb) nb = maxb;
+    if (nb >= 8) nb = (nb + 7u) & ~7u;
+    if (nb == 0) nb = 1;
+    c.wpb = wpb;
+    c.nb = (int)nb;
+    c.lds = lds;
+    return true;
 }
 
 // Block-apply launch shape: one workgroup per CU (up to 150 KB of LDS) of

@@ -20,6 +20,13 @@ typedef void* qkl_stream;
 int qkl_apply_partials(const qk::QkTables* T, int wide);
 int qkl_blas_partials(void);
 
+/* LDS plan of the single-vector apply (host only, no device call): the
+ * widest workgroup of at most pref_waves waves (16, 8, 4, 2, 1) whose dynamic
+ * LDS (tables bytes of lo-rank table, branch tables, per-wave tables) plus
+ * the kernel's static LDS fits the 160 KB of a gfx950 workgroup; -1 when
+ * not even one wave fits. */
+int qkl_apply_lds_plan(size_t tables, int nB, int nA, int cplx, int pref_waves, int* waves, size_t* dyn_lds);
+
 /* y = H x (pull form). cplx: interleaved complex x/y. wide: 64-bit states.
  * The Lanczos form is qkl_lanczos_apply below. */
 int qkl_apply(const qk::QkTables* T, int cplx, int wide, const double* x, double* y, qkl_stream s);

@@ -135,3 +135,33 @@ def test_large_basis_rank_without_enumeration():
         assert s == OL.qo_count_unrank(O.SPIN, 32, sec, int(r))
         assert L.qkrylov_basis_index(b, s) == r
     L.qkrylov_basis_destroy(b)
+
+
+def test_apply_lds_plan_counts_the_static_lds():
+    """The single-vector apply's launch plan (host arithmetic only) keeps the
+    dynamic tables plus the kernel's 136 B of static LDS within the 160 KB of
+    a workgroup: a table set that fits 16 waves only without the static part
+    falls back to 8 waves instead of failing at launch."""
+    from qkrylov_amd import _lib
+    L = _lib.load()
+    f = L.qkl_apply_lds_plan
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)]
+    nB, nA = 40, 30
+    branch = (nB * 36 + 15) & ~15              # lo fields, lM, re, im of the per-lane branches
+    wave = (nB * 16 + nA * 12 + 15) & ~15      # real vectors
+    limit, static = 160 * 1024, 17 * 8
+    w, lds = ctypes.c_int(), ctypes.c_size_t()
+
+    def plan(tables, pref=16):
+        rc = f(tables, nB, nA, 0, pref, ctypes.byref(w), ctypes.byref(lds))
+        return rc, w.value, lds.value
+
+    exact = limit - static - branch - 16 * wave            # 16 waves fit with nothing to spare
+    assert plan(exact) == (0, 16, exact + branch + 16 * wave)
+    for over in (8, 64, static):                           # within the static bytes of the limit
+        rc, waves, dyn = plan(exact + over)
+        assert (rc, waves) == (0, 8) and dyn + static <= limit
+    assert plan(1024, pref=4)[:2] == (0, 4)
+    assert plan(limit)[0] == -1                            # nothing fits
