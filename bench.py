@@ -215,13 +215,19 @@ def serve_leg(args):
     """Serving stream of XXZ-ring solves: Poisson arrivals at 50 req/s,
     L in 12..28, Delta in [0.5, 1.5], |Sz| <= 2, 70 % ground states
     (tol 1e-10), 30 % ground state + S^zz(pi, omega) (100 steps); then the
-    same requests all present at t = 0 (capacity)."""
+    same requests all present at t = 0 (capacity), batched by sector and,
+    for comparison, one request per batch."""
     from qkrylov_amd import serving
     reqs = serving.make_trace(args.serve_n, rate=50.0, seed=2024)
     out, tot = serving.run(reqs, realtime=True, max_active=args.serve_active)
     log(f"[bench-serve] {len(reqs)} requests: {tot}")
-    _, sat = serving.run(reqs, realtime=False, max_active=args.serve_active)
+    sat_out, sat = serving.run(reqs, realtime=False, max_active=args.serve_active)
     log(f"[bench-serve] saturated: {sat}")
+    _, sat1 = serving.run(reqs, realtime=False, max_active=args.serve_active, max_batch=1)
+    log(f"[bench-serve] saturated, unbatched: {sat1}")
+
+    def mean_batch(res):
+        return round(sum(r["batch"] for r in res) / max(len(res), 1), 3)
     return {
         "workload": "XXZ ring serving stream, 50 req/s Poisson, L 12..28, Delta 0.5..1.5, |Sz| <= 2, "
                     "70% ground state / 30% + S^zz CF (100 steps)",
@@ -230,8 +236,13 @@ def serve_leg(args):
         "p50_latency_s": round(tot["p50_s"], 5), "p99_latency_s": round(tot["p99_s"], 5),
         "wall_s": round(tot["wall_s"], 3),
         "converged": int(sum(o["converged"] for o in out)),
+        "mean_batch": mean_batch(out),
         "saturated": {"solves_per_s": round(float(sat["solves_per_s"]), 3), "p50_latency_s": round(float(sat["p50_s"]), 5),
-                      "p99_latency_s": round(float(sat["p99_s"]), 5), "wall_s": round(float(sat["wall_s"]), 3)},
+                      "p99_latency_s": round(float(sat["p99_s"]), 5), "wall_s": round(float(sat["wall_s"]), 3),
+                      "mean_batch": mean_batch(sat_out)},
+        "saturated_unbatched": {"solves_per_s": round(float(sat1["solves_per_s"]), 3),
+                                "p50_latency_s": round(float(sat1["p50_s"]), 5),
+                                "p99_latency_s": round(float(sat1["p99_s"]), 5), "wall_s": round(float(sat1["wall_s"]), 3)},
     }
 
 

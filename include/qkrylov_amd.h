@@ -107,6 +107,16 @@ int qkrylov_hamiltonian_get_tuning(qkrylov_hamiltonian_h h, int knob, int* value
 /* real vectors; nvec > 1: row-major block x[row * nvec + c] */
 int qkrylov_hamiltonian_apply_device_f64(qkrylov_hamiltonian_h h, const double* x_dev, double* y_dev, int nvec,
                                          void* stream);
+/* nv in {1, 2, 4, 8} real row-major columns x[row * nv + c]; column c is
+ * multiplied by H with its diagonal scaled by diag_scale[c] (host array of
+ * nv), every decoded matrix element serving all columns: the batched apply
+ * of requests that share the off-diagonal part of H and differ in the weight
+ * of its diagonal (the serving loop's XXZ anisotropies). Column c's result
+ * does not depend on nv or on the other columns. Real Hamiltonians without
+ * fermion signs on at most 32 bits; others are refused (-1). No counterpart
+ * in the reference. */
+int qkrylov_hamiltonian_apply_columns_device_f64(qkrylov_hamiltonian_h h, const double* x_dev, double* y_dev, int nv,
+                                                 const double* diag_scale, void* stream);
 /* interleaved complex vectors */
 int qkrylov_hamiltonian_apply_device_complex_f64(qkrylov_hamiltonian_h h, const double* x_dev, double* y_dev,
                                                  void* stream);
@@ -245,15 +255,19 @@ typedef struct {
     double cf_norm;
     double start, finish;   /* seconds from trace start */
     double latency;         /* finish - arrival */
+    int batch;              /* requests in the batch this one ran in */
 } qkrylov_serve_result_t;
 
 typedef struct {
     size_t hbm_budget_bytes;   /* admission budget (0: 80% of free memory) */
-    int max_active;            /* concurrent requests on the device */
+    int max_active;            /* concurrent batches on the device */
     int realtime;              /* 1: honour arrival times; 0: all arrive at t=0 */
     double* const* gs_vectors; /* optional, nreq host pointers (each NULL or dim
                                 * doubles): the normalised ground state a kind-1
                                 * request's continued fraction started from */
+    int max_batch;             /* requests of one (L, Sz) sector run as the columns
+                                * of one batch of 1, 2, 4 or 8, up to this many
+                                * (0: 4; at most 8) */
 } qkrylov_serve_options_t;
 
 /* Runs the trace. cf_alphas/cf_betas: nreq * cf_cap host arrays (optional).

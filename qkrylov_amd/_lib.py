@@ -64,12 +64,13 @@ class ServeRequest(ctypes.Structure):
 
 class ServeResult(ctypes.Structure):
     _fields_ = [("energy", c_double), ("iterations", c_int), ("converged", c_int), ("n_cf", c_int),
-                ("cf_norm", c_double), ("start", c_double), ("finish", c_double), ("latency", c_double)]
+                ("cf_norm", c_double), ("start", c_double), ("finish", c_double), ("latency", c_double),
+                ("batch", c_int)]
 
 
 class ServeOptions(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", c_size), ("max_active", c_int), ("realtime", c_int),
-                ("gs_vectors", P(c_vp))]
+                ("gs_vectors", P(c_vp)), ("max_batch", c_int)]
 
 
 # name -> (restype, argtypes)
@@ -160,6 +161,7 @@ SIGNATURES = {
     "qkrylov_hamiltonian_get_tuning": (c_int, [c_vp, c_int, P(c_int)]),
     "qkrylov_hamiltonian_autotune": (c_int, [c_vp, c_int, c_int, P(TuneReport)]),
     "qkrylov_hamiltonian_apply_device_f64": (c_int, [c_vp, c_vp, c_vp, c_int, c_vp]),
+    "qkrylov_hamiltonian_apply_columns_device_f64": (c_int, [c_vp, c_vp, c_vp, c_int, p_dbl, c_vp]),
     "qkrylov_hamiltonian_apply_device_complex_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "qkrylov_hamiltonian_diagonal_device_f64": (c_int, [c_vp, c_vp, c_int, c_vp]),
     "qkrylov_hamiltonian_row_entries": (c_int, [c_vp, c_u64, c_u64, c_int, p_i64, p_dbl, p_int]),

@@ -485,6 +485,14 @@ class MatrixFreeHamiltonian:
     def apply_device(self, x_ptr, y_ptr, nvec=1, stream=None):
         check(_lib.load().qkrylov_hamiltonian_apply_device_f64(self._h, x_ptr, y_ptr, nvec, stream), "apply_device")
 
+    def apply_columns_device(self, x_ptr, y_ptr, diag_scale, stream=None):
+        """nv = len(diag_scale) in (1, 2, 4, 8) real row-major columns
+        (x[row * nv + c]); column c is multiplied by H with its diagonal
+        scaled by diag_scale[c]."""
+        sc = np.ascontiguousarray(diag_scale, dtype=np.float64)
+        check(_lib.load().qkrylov_hamiltonian_apply_columns_device_f64(self._h, x_ptr, y_ptr, len(sc), _dp(sc),
+                                                                        stream), "apply_columns_device")
+
     def apply_device_complex(self, x_ptr, y_ptr, stream=None):
         check(_lib.load().qkrylov_hamiltonian_apply_device_complex_f64(self._h, x_ptr, y_ptr, stream),
               "apply_device_complex")

@@ -714,6 +714,25 @@ int qkrylov_hamiltonian_apply_device_f64(qkrylov_hamiltonian_h h, const double* 
     return QKRYLOV_SUCCESS;
     QK_CATCH
 }
+int qkrylov_hamiltonian_apply_columns_device_f64(qkrylov_hamiltonian_h h, const double* x, double* y, int nv,
+                                                 const double* diag_scale, void* stream)
+{
+    const auto* H = H_of(h);
+    if (!H || !x || !y || !diag_scale || qk::cols_width(nv) != nv) return QKRYLOV_ERROR_INVALID_ARG;
+    const auto& E = *H->engine();
+    if (!E.is_real() || E.wide() || E.tables()->signs != 0) {
+        g_last_error = "apply_columns: needs a real Hamiltonian without fermion signs on at most 32 bits";
+        return QKRYLOV_ERROR_INVALID_ARG;
+    }
+    QK_TRY
+    hipStream_t s = stream ? (hipStream_t)stream : qk::default_stream();
+    qk::DevBuf sc(nv * sizeof(double));
+    qk::hip_check(hipMemcpyAsync(sc.get(), diag_scale, nv * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    E.apply_cols(x, y, nv, sc.d(), s);
+    qk::hip_check(hipStreamSynchronize(s), "sync");   // before the scale buffer goes
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
 int qkrylov_hamiltonian_apply_device_complex_f64(qkrylov_hamiltonian_h h, const double* x, double* y, void* stream)
 {
     const auto* H = H_of(h);

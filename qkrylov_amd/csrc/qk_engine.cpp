@@ -475,6 +475,14 @@ void Engine::apply(const double* x, double* y, bool cplx, hipStream_t s) const
     lcheck(qkl_apply(&T_, cplx ? 1 : 0, wide() ? 1 : 0, x, y, s), "apply");
 }
 
+void Engine::apply_cols(const double* X, double* Y, int nv, const double* dscale, hipStream_t s) const
+{
+    if (cols_width(nv) != nv) throw std::invalid_argument("column block width must be 1, 2, 4 or 8");
+    if (!is_real() || wide()) throw std::invalid_argument("column blocks need a real Hamiltonian on <= 32 bits");
+    if (T_.signs != 0) throw std::invalid_argument("column blocks: operators with parity masks are not supported");
+    lcheck(qkl_apply_cols(&T_, 0, nv, X, Y, dscale, -1, nullptr, nullptr, nullptr, 0, s), "apply_cols");
+}
+
 void Engine::apply_block(const double* X, double* Y, int ld, int ncols, hipStream_t s) const
 {
     if (!is_real()) throw std::invalid_argument("block apply needs a real Hamiltonian");
@@ -604,6 +612,63 @@ RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool
         hcoef = mk((kcap + 2) * sizeof(double));
         partM = mk((size_t)NB * (kcap + 2) * sizeof(double));
     }
+}
+
+int cols_width(int n)
+{
+    return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : n <= 8 ? 8 : 0;
+}
+
+ColsCore::ColsCore(const Engine& e, int nv_, int kcap_, const double* dscale, hipStream_t st, bool so)
+    : E(e), s(st), D((size_t)e.dim()), nv(nv_), kcap(kcap_)
+{
+    if (cols_width(nv) != nv) throw std::invalid_argument("column block width must be 1, 2, 4 or 8");
+    if (!E.is_real() || E.wide()) throw std::invalid_argument("column blocks need a real Hamiltonian on <= 32 bits");
+    if (E.tables()->signs != 0)
+        throw std::invalid_argument("column blocks: operators with parity masks are not supported");
+    auto mk = [&](size_t bytes) { return so ? DevBuf(bytes, st) : DevBuf(bytes); };
+    for (auto& q : Q) q = mk(D * nv * sizeof(double));
+    alpha = mk((size_t)(kcap + 2) * nv * sizeof(double));
+    nrm = mk((size_t)(kcap + 3) * nv * sizeof(double));
+    npA = qkl_cols_partials(E.tables());
+    if (npA <= 0) throw std::runtime_error("column apply: the operator tables do not fit in LDS");
+    NB = qkl_blas_partials();
+    partA = mk((size_t)npA * nv * sizeof(double));
+    partB = mk((size_t)NB * nv * sizeof(double));
+    dsc = mk(nv * sizeof(double));
+    zero = mk(nv * sizeof(double));
+    one = mk(nv * sizeof(double));
+    hsc.assign(3 * (size_t)nv, 0.0);
+    for (int c = 0; c < nv; ++c) {
+        hsc[c] = dscale[c];
+        hsc[2 * (size_t)nv + c] = 1.0;
+    }
+    // stream-ordered copies from host memory that lives as long as the core
+    hip_check(hipMemcpyAsync(dsc.get(), hsc.data(), nv * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    hip_check(hipMemcpyAsync(zero.get(), hsc.data() + nv, nv * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    hip_check(hipMemcpyAsync(one.get(), hsc.data() + 2 * nv, nv * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+}
+
+void ColsCore::init_norm()
+{
+    // an update with coefficient 0 / 1 leaves cur as it is and yields its
+    // column sums of squares
+    lcheck(qkl_update_cols(nv, D, cur(), cur(), zero.d(), one.d(), 0, partB.d(), s), "norm0");
+    lcheck(qkl_reduce(partB.d(), NB, nv, 1, 1, 0, nrm.d(), 0, nv, s), "reduce norm0");
+}
+
+void ColsCore::step(int k)
+{
+    if (k < 0 || k > kcap) throw std::logic_error("column recurrence: step beyond the capacity");
+    double* w = Q[iw].d();
+    lcheck(qkl_apply_cols(E.tables(), 0, nv, cur(), w, dsc.d(), k, prv(), nrm.d(), partA.d(), npA, s), "apply_cols");
+    lcheck(qkl_reduce(partA.d(), npA, nv, 1, 0, 0, alpha.d(), k, nv, s), "reduce alpha");
+    lcheck(qkl_update_cols(nv, D, w, cur(), alpha.d(), nrm.d(), k, partB.d(), s), "update_cols");
+    lcheck(qkl_reduce(partB.d(), NB, nv, 1, 1, 0, nrm.d(), k + 1, nv, s), "reduce beta");
+    const int t = ip;
+    ip = ic;
+    ic = iw;
+    iw = t;
 }
 
 RealCore::~RealCore()

@@ -109,6 +109,9 @@ public:
     // y = H^dagger-convention apply of the reference (y[a] = sum conj(amp) x[b])
     void apply(const double* x, double* y, bool cplx, hipStream_t s) const;
     void apply_block(const double* X, double* Y, int ld, int ncols, hipStream_t s) const;
+    // Y[r * nv + c] = (H_c X_c)[r] for nv in {1, 2, 4, 8} row-major real
+    // columns, H_c = H with its diagonal scaled by dscale[c] (device array)
+    void apply_cols(const double* X, double* Y, int nv, const double* dscale, hipStream_t s) const;
     void diagonal(double* d, bool cplx, hipStream_t s) const;
     // y = O x for a diagonal operator compiled into this engine
     void apply_diag_operator(const double* x, bool xcplx, double* y, hipStream_t s) const;
@@ -214,6 +217,36 @@ struct RealCore {
     // same kernels in the same order as step(), so the same numbers
     void steps_graph(int k, int n);
 };
+
+// Device state of nv independent real three-term recurrences run as the
+// columns of row-major blocks (nv in 1, 2, 4, 8): column c uses H with its
+// diagonal scaled by dscale[c]. One step = the multi-column Lanczos-form
+// apply, the per-column alpha totals, the update and the per-column norm
+// totals, on stream s without host synchronisation. Column c computes the
+// same bits whatever nv is and whatever the other columns hold; a column
+// whose recurrence broke down (beta = 0) holds non-finite numbers from then
+// on without touching the others. Vectors stored unnormalised with their
+// norms nrm[k * nv + c], as in RealCore.
+struct ColsCore {
+    const Engine& E;
+    hipStream_t s;
+    size_t D;
+    int nv, kcap;
+    DevBuf Q[3], alpha, nrm, partA, partB, dsc, zero, one;
+    std::vector<double> hsc;   // host source of dsc, zero, one (kept until the core goes)
+    int npA, NB;
+    int ip = 0, ic = 1, iw = 2;
+    // dscale: nv host values; stream_ordered: buffers from hipMallocAsync on st
+    ColsCore(const Engine& e, int nv, int kcap, const double* dscale, hipStream_t st, bool stream_ordered);
+    double* cur() { return Q[ic].d(); }
+    double* prv() { return Q[ip].d(); }
+    // nrm[0][c] = |cur_c| (cur unchanged): a new recurrence from cur (step 0
+    // reads no previous vector)
+    void init_norm();
+    void step(int k);    // alpha[k][c], nrm[k + 1][c]; cur becomes q_{k+1}
+};
+// widths ColsCore and qkl_apply_cols take: the smallest of 1, 2, 4, 8 holding n columns (0 if n > 8)
+int cols_width(int n);
 
 // ---------------------------------------------------------------- tridiagonal
 // Lowest eigenvalue of the Lanczos tridiagonal (alpha, beta) and, when evec

@@ -925,6 +925,325 @@ __global__ __launch_bounds__(QK_TPB) void k_update_block(size_t nrows, int ncol,
     }
 }
 
+// ------------------------------------------------------------------ multi-column apply
+// Y = H_c X for NV real columns stored row-major (X[row * NV + c]), where H_c
+// is H with its diagonal scaled by dscale[c] (for instance the anisotropy of
+// an XXZ chain whose exchange part the columns share). One lane per row as
+// in k_apply: every decoded entry (target, value) serves all NV columns, and
+// every gather reads the NV consecutive doubles of one target row. In the
+// Lanczos form (LZ) each column c runs its own recurrence: scalars
+// nrm[k * NV + c], previous block P, partial <q_k, w> of column c at
+// partials[c * npart + workgroup]. A column's arithmetic (entries in branch
+// order, the FMA order, the reduction tree) depends neither on NV nor on the
+// other columns, and the launch grid depends only on the tables, so a column
+// gets the same bits in a block of any width. Real spin tables (no parity
+// masks), 32-bit states.
+template <int NV>
+__device__ __forceinline__ void ld_cols(const double* __restrict__ X, size_t r, double* v)
+{
+    if (NV == 1) {
+        v[0] = X[r];
+    } else {
+        const qk_double2* p = reinterpret_cast<const qk_double2*>(X + r * NV);
+#pragma unroll
+        for (int q = 0; q < NV / 2; ++q) {
+            const qk_double2 t = p[q];
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void st_cols(double* __restrict__ Y, size_t r, const double* v, bool nt)
+{
+    if (NV == 1) {
+        if (nt) __builtin_nontemporal_store(v[0], Y + r);
+        else Y[r] = v[0];
+    } else {
+        qk_double2* p = reinterpret_cast<qk_double2*>(Y + r * NV);
+#pragma unroll
+        for (int q = 0; q < NV / 2; ++q) {
+            qk_double2 t;
+            t.x = v[2 * q];
+            t.y = v[2 * q + 1];
+            if (nt) __builtin_nontemporal_store(t, p + q);
+            else p[q] = t;
+        }
+    }
+}
+
+// acc[c] += sum_u cr[u] * X[rt[u]][c]: all U row loads first, then the FMAs
+// in u order for every column
+template <int NV, int U>
+__device__ __forceinline__ void accumulate_cols(const double* __restrict__ X, const uint32_t* rt, const double* cr,
+                                                double* acc)
+{
+    double t[U][NV];
+#pragma unroll
+    for (int u = 0; u < U; ++u) ld_cols<NV>(X, rt[u], t[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) acc[c] = fma(cr[u], t[u][c], acc[c]);
+}
+
+template <int RM, int KIND, int NV>
+__device__ __forceinline__ void lane_branches_cols(int gbeg, int gend, const unsigned char* smem, const uint4* lb4,
+                                                   const double* lbre, const uint4* tb, uint32_t lo, uint32_t row,
+                                                   const OwnRank& orank, const QkScal& sc,
+                                                   const double* __restrict__ X, double* acc)
+{
+    constexpr int U = NV >= 4 ? 4 : 8;
+    for (int g = gbeg; g < gend; g += U) {
+        uint32_t rt[U];
+        double cr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint4 L = lb4[g + u];   // lcm, lcv, lalt, lF
+            const uint4 tv = tb[g + u];   // block offset, va, vb, hpar
+            const uint32_t lf = lo & L.x;
+            const uint32_t m = 0u - (uint32_t)((lf == tv.y) | (lf == tv.z));
+            const uint32_t lr = target_rank<RM, KIND>(smem, lo, L.w, orank, sc);
+            const uint64_t m64 = ((uint64_t)m << 32) | m;
+            rt[u] = row ^ ((row ^ (tv.x + lr)) & m);
+            cr[u] = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, lbre[g + u]) & m64);
+        }
+        accumulate_cols<NV, U>(X, rt, cr, acc);
+    }
+}
+
+template <int RM, bool LZ, int NV>
+__global__ __launch_bounds__(512) void k_apply_cols(const QkTile* __restrict__ tiles, const uint32_t* __restrict__ hoff,
+                                                     const uint32_t* __restrict__ lrank,
+                                                     const uint16_t* __restrict__ tl,
+                                                     const uint32_t* __restrict__ th, int nth,
+                                                     const uint32_t* __restrict__ lolist,
+                                                     const QkBranchK* __restrict__ brk,
+                                                     const QkDiagTerm* __restrict__ dg, QkScal sc,
+                                                     const double* __restrict__ X, double* __restrict__ Y,
+                                                     const double* __restrict__ dscale, int lz_k,
+                                                     const double* __restrict__ P, const double* __restrict__ nrm,
+                                                     double* __restrict__ partials, int npart, int table_bytes)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    // static LDS: QK_COLS_STATIC_LDS bytes
+    __shared__ double wsum[8][NV];
+    const int nA = sc.nA, nB = sc.nB, nbk = sc.nA + sc.nB;
+    uint4* lb4 = reinterpret_cast<uint4*>(smem + table_bytes);
+    double* lbre = reinterpret_cast<double*>(smem + table_bytes + ((size_t)nB * 20 + 7) / 8 * 8);
+    for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+        const QkBranchK B = brk[nA + i];
+        lb4[i] = make_uint4(B.lcm, B.lcv, B.lalt, B.lF);
+        lbre[i] = B.re;
+    }
+    load_lrank<RM>(smem, lrank, tl, th, sc.nlo, sc.ntl, nth);
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const int NL = sc.NL;
+    unsigned char* wb = smem + table_bytes + apply_branch_bytes(nB) + (threadIdx.x >> 6) * apply_wave_bytes(nB, nA, false);
+    uint4* tb = reinterpret_cast<uint4*>(wb);                          // [nB] {block offset, va, vb, 0}
+    double* a_cr = reinterpret_cast<double*>(wb + (size_t)nB * 16);    // [nA] taken hi-only values
+    uint32_t* a_base = reinterpret_cast<uint32_t*>(a_cr + nA);         // [nA] their block offsets
+    const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+
+    double dsc[NV], inv_n[NV], cprev[NV], part[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        dsc[c] = dscale[c];
+        inv_n[c] = 1.0;
+        cprev[c] = 0.0;
+        part[c] = 0.0;
+        if (LZ) {
+            const double nk = nrm[(size_t)lz_k * NV + c];
+            inv_n[c] = 1.0 / nk;
+            if (lz_k > 0) cprev[c] = nk / nrm[(size_t)(lz_k - 1) * NV + c];
+        }
+    }
+
+    uint32_t t0, tend, step;
+    tile_schedule(sc.ntiles, sc.sched, t0, tend, step);
+    for (uint32_t t = t0; t < tend; t += step) {
+        const QkTile tl = tiles[t];
+        const uint32_t hi = tl.hi;
+        const bool active = lane < tl.count;
+        const uint32_t base_self = hoff[hi];
+        const uint32_t lidx = active ? lane : 0u;
+        const uint32_t my_lr = tl.row_start - base_self + lidx;
+        const uint32_t row = tl.row_start + lidx;
+        const uint32_t lo = lolist[tl.lo_base + lidx];
+        double pv[NV];
+        if (LZ && lz_k > 0) ld_cols<NV>(P, row, pv);
+        const uint32_t s = (hi << NL) | lo;
+
+        // (1) the uniform part of every branch, one branch per lane
+        uint32_t ntaken = 0;
+        for (int b0 = 0; b0 < nbk; b0 += 64) {
+            const int b = b0 + (int)lane;
+            bool takeA = false;
+            uint32_t base = 0;
+            double vr = 0.0;
+            if (b < nbk) {
+                const QkBranchK B = brk[b];
+                const uint32_t hm = hi & B.hcm;
+                const uint32_t u0 = (hm == B.hcv) ? 1u : 0u;
+                const uint32_t u1 = (((B.halt | B.lalt) != 0) && (hm == (B.hcv ^ B.halt))) ? 2u : 0u;
+                const uint32_t never = ~B.lcm;
+                base = hoff[hi ^ B.hF];
+                if (b >= nA) tb[b - nA] = make_uint4(base, u0 ? B.lcv : never, u1 ? (B.lcv ^ B.lalt) : never, 0u);
+                takeA = (b < nA) && (u0 | u1);
+                vr = B.re;
+            }
+            // (2) compact the taken hi-only branches
+            const uint64_t m = __ballot(takeA);
+            if (takeA) {
+                const uint32_t pos = ntaken + (uint32_t)__popcll(m & lt_mask);
+                a_base[pos] = base;
+                a_cr[pos] = vr;
+            }
+            ntaken += (uint32_t)__popcll(m);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
+        double dre, dim;
+        eval_diag<uint32_t, false>(dg, sc.ndg, sc.dcr, sc.dci, s, dre, dim);
+        double xv[NV], acc[NV];
+        ld_cols<NV>(X, row, xv);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) acc[c] = (dre * dsc[c]) * xv[c];
+
+        // (3) taken hi-only branches, four in flight
+        for (uint32_t i = 0; i < ntaken; i += 4) {
+            uint32_t rt[4];
+            double cr[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool in = (i + u) < ntaken;
+                const uint32_t k = in ? i + u : 0u;
+                rt[u] = in ? a_base[k] + my_lr : row;
+                cr[u] = in ? a_cr[k] : 0.0;
+            }
+            accumulate_cols<NV, 4>(X, rt, cr, acc);
+        }
+
+        // (4) per-lane branches
+        const OwnRank orank = own_rank<RM>(smem, lo, sc);
+        if (RM == 2) {
+            lane_branches_cols<RM, 1, NV>(0, sc.nB1, smem, lb4, lbre, tb, lo, row, orank, sc, X, acc);
+            lane_branches_cols<RM, 2, NV>(sc.nB1, sc.nB1 + sc.nB2, smem, lb4, lbre, tb, lo, row, orank, sc, X, acc);
+            lane_branches_cols<RM, 0, NV>(sc.nB1 + sc.nB2, nB, smem, lb4, lbre, tb, lo, row, orank, sc, X, acc);
+        } else {
+            lane_branches_cols<RM, 0, NV>(0, nB, smem, lb4, lbre, tb, lo, row, orank, sc, X, acc);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
+        if (active) {
+            if (LZ) {
+#pragma unroll
+                for (int c = 0; c < NV; ++c) {
+                    double w = acc[c] * inv_n[c];
+                    if (lz_k > 0) w = fma(-cprev[c], pv[c], w);
+                    acc[c] = w;
+                    part[c] = fma(xv[c] * inv_n[c], w, part[c]);
+                }
+            }
+            st_cols<NV>(Y, row, acc, true);
+        }
+    }
+    if (LZ) {
+        const uint32_t wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            const double v = wave_sum(part[c]);
+            if (lane == 0) wsum[wv][c] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < (unsigned)NV) {
+            double s = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += wsum[i][threadIdx.x];
+            partials[(size_t)threadIdx.x * npart + blockIdx.x] = s;
+        }
+    }
+}
+
+// Lanczos update of NV row-major columns, each with its own coefficient:
+// w[r][c] -= (alpha[k][c] / nrm[k][c]) q[r][c]; partials[c * grid + b] of
+// |w_c|^2. Rows are thread-strided over a fixed grid, so a column's partial
+// sums do not depend on NV.
+template <int NV>
+__global__ __launch_bounds__(QK_TPB) void k_update_cols(size_t nrows, double* __restrict__ w,
+                                                         const double* __restrict__ q,
+                                                         const double* __restrict__ alpha,
+                                                         const double* __restrict__ nrm, int k,
+                                                         double* __restrict__ partials)
+{
+    __shared__ double wsum[QK_WPB][NV];
+    double coef[NV], part[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        coef[c] = alpha[(size_t)k * NV + c] / nrm[(size_t)k * NV + c];
+        part[c] = 0.0;
+    }
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += stride) {
+        double wv[NV], qv[NV];
+        ld_cols<NV>(w, r, wv);
+        ld_cols<NV>(q, r, qv);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            wv[c] = fma(-coef[c], qv[c], wv[c]);
+            part[c] = fma(wv[c], wv[c], part[c]);
+        }
+        st_cols<NV>(w, r, wv, false);
+    }
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        const double v = wave_sum(part[c]);
+        if (lane == 0) wsum[wv][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)NV) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += wsum[i][threadIdx.x];
+        partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// Y[r][c] += coef[c] X[r][c] for the columns whose coefficient is not zero
+// (a zero coefficient leaves the column untouched, whatever X holds)
+template <int NV>
+__global__ __launch_bounds__(QK_TPB) void k_axpy_cols(size_t nrows, const double* __restrict__ coef,
+                                                       const double* __restrict__ X, double* __restrict__ Y)
+{
+    double a[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) a[c] = coef[c];
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (size_t)gridDim.x * blockDim.x) {
+        double xv[NV], yv[NV];
+        ld_cols<NV>(X, r, xv);
+        ld_cols<NV>(Y, r, yv);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) yv[c] = (a[c] != 0.0) ? fma(a[c], xv[c], yv[c]) : yv[c];
+        st_cols<NV>(Y, r, yv, false);
+    }
+}
+
+// Y[r][c] = d[r] X[r][c] (a diagonal operator applied to every column)
+template <int NV>
+__global__ __launch_bounds__(QK_TPB) void k_scale_rows_cols(size_t nrows, const double* __restrict__ d,
+                                                             const double* __restrict__ X, double* __restrict__ Y)
+{
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (size_t)gridDim.x * blockDim.x) {
+        double xv[NV];
+        ld_cols<NV>(X, r, xv);
+        const double dr = d[r];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) xv[c] = dr * xv[c];
+        st_cols<NV>(Y, r, xv, false);
+    }
+}
+
 // ------------------------------------------------------------------ reductions
 // One workgroup per result group. Fixed order: thread-strided sums, then a
 // fixed tree, then columns of the group added in index order.
@@ -1387,17 +1706,47 @@ struct ApplyCfg {
 #define QK_LDS_LIMIT ((size_t)160 * 1024)
 #define QK_APPLY_STATIC_LDS ((size_t)17 * sizeof(double))
 
-extern "C" int qkl_apply_lds_plan(size_t tables, int nB, int nA, int cplx, int pref_waves, int* waves,
-                                  size_t* dyn_lds)
+// static LDS of k_apply_cols (wsum[8][NV] at the widest NV = 8)
+#define QK_COLS_STATIC_LDS ((size_t)8 * 8 * sizeof(double))
+
+inline int apply_lds_plan(size_t tables, int nB, int nA, bool cplx, int pref_waves, size_t static_lds, int* waves,
+                          size_t* dyn_lds)
 {
     for (int wpb = pref_waves; wpb >= 1; wpb /= 2) {
-        const size_t lds = tables + apply_branch_bytes(nB) + (size_t)wpb * apply_wave_bytes(nB, nA, cplx != 0);
-        if (lds + QK_APPLY_STATIC_LDS > QK_LDS_LIMIT) continue;
+        const size_t lds = tables + apply_branch_bytes(nB) + (size_t)wpb * apply_wave_bytes(nB, nA, cplx);
+        if (lds + static_lds > QK_LDS_LIMIT) continue;
         *waves = wpb;
         *dyn_lds = lds;
         return 0;
     }
     return -1;
+}
+
+extern "C" int qkl_apply_lds_plan(size_t tables, int nB, int nA, int cplx, int pref_waves, int* waves,
+                                  size_t* dyn_lds)
+{
+    return apply_lds_plan(tables, nB, nA, cplx != 0, pref_waves, QK_APPLY_STATIC_LDS, waves, dyn_lds);
+}
+
+// Multi-column apply launch shape: the single-vector apply's preferred
+// waves per workgroup up to 8 (the kernel's bound, which leaves the widest
+// instance room for its registers), within LDS, and a grid of at most 4096
+// waves; both depend only on the tables (never on the number of columns).
+// The grid is a multiple of 8 workgroups.
+inline bool cols_config(const QkTables* T, ApplyCfg& c)
+{
+    int wpb = (T->apply_waves == 8 || T->apply_waves == 16) ? 8 : 4;
+    size_t lds = 0;
+    if (apply_lds_plan(table_bytes(T), T->nB, T->nA, false, wpb, QK_COLS_STATIC_LDS, &wpb, &lds) != 0) return false;
+    uint32_t nb = (T->ntiles + wpb - 1) / wpb;
+    const uint32_t maxb = 4096u / (uint32_t)wpb;
+    if (nb > maxb) nb = maxb;
+    if (nb >= 8) nb = (nb + 7u) & ~7u;
+    if (nb == 0) nb = 1;
+    c.wpb = wpb;
+    c.nb = (int)nb;
+    c.lds = lds;
+    return true;
 }
 
 inline bool apply_config(const QkTables* T, bool cplx, const void* f, ApplyCfg& c)
@@ -1585,6 +1934,55 @@ const void* lz_apply_fn(const QkTables* T)
 
 } // namespace
 
+template <int RM, bool LZ, int NV>
+int launch_cols(const QkTables* T, const ApplyCfg& c, const double* X, double* Y, const double* dscale, int k,
+                const double* P, const double* nrm, double* partials, int npart, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_apply_cols<RM, LZ, NV>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff, T->lrank,
+                       T->tl, T->th, T->nth, T->lolist, T->brk, T->dg, make_scal(T), X, Y, dscale, k, P, nrm, partials,
+                       npart, (int)table_bytes(T));
+    return rc(hipGetLastError());
+}
+
+template <int RM, bool LZ>
+int cols_nv(int nv, const QkTables* T, const ApplyCfg& c, const double* X, double* Y, const double* dscale, int k,
+            const double* P, const double* nrm, double* partials, int npart, hipStream_t s)
+{
+    switch (nv) {
+    case 1: return launch_cols<RM, LZ, 1>(T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    case 2: return launch_cols<RM, LZ, 2>(T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    case 4: return launch_cols<RM, LZ, 4>(T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    case 8: return launch_cols<RM, LZ, 8>(T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    default: return -1;
+    }
+}
+
+template <int NV>
+int launch_cols_blas(int op, size_t nrows, double* w, const double* q, const double* a, const double* nrm, int k,
+                     double* partials, hipStream_t s)
+{
+    if (op == 0)
+        hipLaunchKernelGGL(k_update_cols<NV>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, a, nrm, k,
+                           partials);
+    else if (op == 1)
+        hipLaunchKernelGGL(k_axpy_cols<NV>, dim3(grid_for(nrows)), dim3(QK_TPB), 0, s, nrows, a, q, w);
+    else
+        hipLaunchKernelGGL(k_scale_rows_cols<NV>, dim3(grid_for(nrows)), dim3(QK_TPB), 0, s, nrows, a, q, w);
+    return rc(hipGetLastError());
+}
+
+int cols_blas(int nv, int op, size_t nrows, double* w, const double* q, const double* a, const double* nrm, int k,
+              double* partials, hipStream_t s)
+{
+    switch (nv) {
+    case 1: return launch_cols_blas<1>(op, nrows, w, q, a, nrm, k, partials, s);
+    case 2: return launch_cols_blas<2>(op, nrows, w, q, a, nrm, k, partials, s);
+    case 4: return launch_cols_blas<4>(op, nrows, w, q, a, nrm, k, partials, s);
+    case 8: return launch_cols_blas<8>(op, nrows, w, q, a, nrm, k, partials, s);
+    default: return -1;
+    }
+}
+
 extern "C" {
 
 int qkl_apply_partials(const QkTables* T, int wide)
@@ -1606,6 +2004,48 @@ int qkl_block_partials(const QkTables* T)
     return block_config(T, c) ? c.nb : -2;
 }
 int qkl_blas_partials(void) { return QK_BLAS_BLOCKS; }
+
+int qkl_cols_partials(const QkTables* T)
+{
+    ApplyCfg c;
+    return cols_config(T, c) ? c.nb : -2;
+}
+
+int qkl_apply_cols(const QkTables* T, int wide, int nv, const double* X, double* Y, const double* dscale, int k,
+                   const double* P, const double* nrm, double* partials, int npart, qkl_stream sv)
+{
+    hipStream_t s = (hipStream_t)sv;
+    if (wide || T->signs != 0 || !dscale) return -1;
+    if (T->ntiles == 0) return 0;
+    ApplyCfg c;
+    if (!cols_config(T, c)) return -2;
+    if (k >= 0 && (npart != c.nb || !nrm || !partials || (k > 0 && !P))) return -1;
+    const bool lz = k >= 0;
+    switch (T->rank_mode) {
+    case 2: return lz ? cols_nv<2, true>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s)
+                      : cols_nv<2, false>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    case 1: return lz ? cols_nv<1, true>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s)
+                      : cols_nv<1, false>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    default: return lz ? cols_nv<0, true>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s)
+                       : cols_nv<0, false>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    }
+}
+
+int qkl_update_cols(int nv, size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, int k,
+                    double* partials, qkl_stream sv)
+{
+    return cols_blas(nv, 0, nrows, w, q, alpha, nrm, k, partials, (hipStream_t)sv);
+}
+
+int qkl_axpy_cols(int nv, size_t nrows, const double* coef, const double* X, double* Y, qkl_stream sv)
+{
+    return cols_blas(nv, 1, nrows, Y, X, coef, nullptr, 0, nullptr, (hipStream_t)sv);
+}
+
+int qkl_scale_rows_cols(int nv, size_t nrows, const double* d, const double* X, double* Y, qkl_stream sv)
+{
+    return cols_blas(nv, 2, nrows, Y, X, d, nullptr, 0, nullptr, (hipStream_t)sv);
+}
 
 int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y, qkl_stream sv)
 {

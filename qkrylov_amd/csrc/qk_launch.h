@@ -79,6 +79,26 @@ int qkl_lanczos_update(int cplx, size_t nrows, int ncol, int ld, double* w_inout
 int qkl_reduce(const double* partials, int npart, int ncol, int group, int mode, int cplx,
                double* out, int idx, int ld_sc, qkl_stream s);
 
+/* Multi-column apply: nv in {1, 2, 4, 8} real columns stored row-major
+ * (X[row * nv + c]); column c uses H with its diagonal scaled by dscale[c]
+ * (device array of nv). Real spin tables without parity masks, 32-bit
+ * states (wide = 0), else -1. Plain form (Y = H_c X) for k < 0; Lanczos
+ * form for k >= 0, per column as qkl_apply_block's (nrm[k * nv + c], P the
+ * previous block, partials[c * npart + b] with npart = qkl_cols_partials).
+ * A column's bits do not depend on nv or on the other columns.
+ * qkl_update_cols: the Lanczos update of nv row-major columns, per-column
+ * coefficients alpha[k * nv + c] / nrm[k * nv + c], partial sums of the
+ * squared norms (npart = qkl_blas_partials). qkl_axpy_cols: Y[r][c] +=
+ * coef[c] X[r][c] for every column whose coefficient is not zero.
+ * qkl_scale_rows_cols: Y[r][c] = d[r] X[r][c]. */
+int qkl_cols_partials(const qk::QkTables* T);
+int qkl_apply_cols(const qk::QkTables* T, int wide, int nv, const double* X, double* Y, const double* dscale, int k,
+                   const double* P, const double* nrm, double* partials, int npart, qkl_stream s);
+int qkl_update_cols(int nv, size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, int k,
+                    double* partials, qkl_stream s);
+int qkl_axpy_cols(int nv, size_t nrows, const double* coef, const double* X, double* Y, qkl_stream s);
+int qkl_scale_rows_cols(int nv, size_t nrows, const double* d, const double* X, double* Y, qkl_stream s);
+
 /* BLAS-1 with deterministic partials (npart = qkl_blas_partials()) */
 int qkl_dot_partials(int cplx, size_t n, const double* x, const double* y, double* partials, qkl_stream s);
 int qkl_axpy(int cplx, size_t n, double are, double aim, const double* x, double* y, qkl_stream s);

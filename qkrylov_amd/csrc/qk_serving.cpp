@@ -1,14 +1,28 @@
 // qk_serving.cpp -- continuous-batching request loop for ground-state and
 // S^zz(pi, omega) continued-fraction solves of XXZ rings.
 //
-// Each admitted request owns a stream, its Krylov vectors (stream-ordered
-// allocations) and its alpha/beta arrays; all requests in flight enqueue
+// Requests are admitted in arrival order while the HBM footprint of what is
+// on the device stays under the budget and a stream slot is free. Waiting
+// requests of the same (L, Sz) sector are admitted together as one batch:
+// they share the exchange part of H and differ only in the anisotropy,
+// which scales the Ising diagonal, so they run as the columns of one block
+// recurrence (ColsCore) in which every decoded matrix element serves all
+// columns. Each batch owns a stream, its Krylov blocks (stream-ordered
+// allocations) and its alpha/norm arrays; all batches in flight enqueue
 // their steps without host synchronisation, so their kernels overlap on the
-// device, and the host only polls events. Requests are admitted first-fit
-// in arrival order while the HBM footprint of the active set stays under
-// the budget and a stream slot is free. Every kernel and reduction has a
-// fixed grid and order, so a request's result does not depend on what else
-// is running.
+// device, and the host only polls events. A batch runs three phases:
+//   GS    the Lanczos recurrence of every column until each column's replay
+//         of the reference's convergence test is done (columns that finish
+//         early keep being stepped, their numbers unused);
+//   Ritz  for the columns that also want S^zz(pi, omega): the recurrence
+//         again from the same start vectors (a narrower block of just those
+//         columns), accumulating each column's Ritz vector;
+//   CF    S^z_pi applied to the Ritz vectors, then the continued-fraction
+//         recurrence of those columns.
+// A ground-state request is finished when the GS phase ends. Every kernel
+// and reduction of a column has a fixed grid and order that depend neither
+// on the block width nor on the other columns, so a request's result is the
+// same bits alone (a batch of one) and in any batch.
 #include "qkrylov_amd.h"
 
 #include "qk_engine.hpp"
@@ -18,7 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <deque>
+#include <list>
 #include <map>
 #include <memory>
 #include <string>
@@ -27,6 +41,13 @@
 
 namespace qk {
 namespace {
+
+// Widest batch by default, and the widest column block (ColsCore). Batches
+// take power-of-two sizes, so no column of a block is padding. On an MI355X
+// the XXZ ring's column apply is bound by the bytes its gathers move, which
+// grow with the width: per column it costs 0.63x the single-vector apply at
+// two and four columns and 0.83x at eight (L = 28), so four is the default.
+constexpr int kDefaultBatch = 4, kMaxBatch = 8;
 
 OpSum xxz_ring(int L, double delta)
 {
@@ -67,187 +88,281 @@ struct Pinned {
     Pinned& operator=(const Pinned&) = delete;
 };
 
-enum Phase { kGS, kRitz, kCF, kDone };
-
-struct Job {
-    int id = 0;
-    qkrylov_serve_request_t rq{};
-    hipStream_t s = nullptr;
-    hipEvent_t ev = nullptr;
-    std::shared_ptr<BasisSpec> basis;
-    std::unique_ptr<Engine> H, Op;
-    std::unique_ptr<RealCore> C;
-    std::unique_ptr<LanczosReplay> rp;
-    Pinned *ha = nullptr, *hn = nullptr;   // the slot's read-back buffers
-    double* gs_out = nullptr;              // caller's host buffer for the ground state (or null)
-    DevBuf ritz, psi2;
-    Phase phase = kGS;
-    int kcap = 0, k = 0, n_gs = 0;
-    bool inflight = false;
-    std::vector<double> z, nrm_gs;
-    size_t footprint = 0;
-    double start = 0.0;
-    uint64_t seed = 0;
-};
-
-size_t dim_of(const qkrylov_serve_request_t& r)
+size_t dim_of(int L, int sz2)
 {
     Sector s;
     s.use_sz = true;
-    s.sz2 = r.sz2;
-    return (size_t)BasisSpec(Kind::SpinHalf, r.L, s).dim();
+    s.sz2 = sz2;
+    return (size_t)BasisSpec(Kind::SpinHalf, L, s).dim();
 }
 
-size_t footprint_of(const qkrylov_serve_request_t& r)
+int kcap_of(const qkrylov_serve_request_t& r)
 {
-    const size_t D = dim_of(r);
-    const size_t vecs = 3 + (r.kind == 1 ? 1 : 0);
-    return vecs * D * sizeof(double) + (size_t)(r.maxiter + r.cf_steps + 8) * 16 + (1 << 20);
+    return (int)std::min<size_t>((size_t)r.maxiter, dim_of(r.L, r.sz2));
 }
 
-constexpr int kChunk = 12;   // Lanczos steps enqueued per poll
-
-void copy_scalars(Job& j, int nsteps)
+// device bytes of a batch at its widest phase: the GS block recurrence
+// (three blocks of nv columns), then, for the S^zz columns (nvr of them,
+// padded), their recurrence plus the Ritz block; the GS blocks are released
+// before the second phase allocates
+size_t footprint_of(const qkrylov_serve_request_t* const* rq, int n)
 {
-    hip_check(hipMemcpyAsync(j.ha->p, j.C->alpha.d(), nsteps * sizeof(double), hipMemcpyDeviceToHost, j.s), "d2h");
-    hip_check(hipMemcpyAsync(j.hn->p, j.C->nrm.d(), (nsteps + 1) * sizeof(double), hipMemcpyDeviceToHost, j.s), "d2h");
-}
-
-void enqueue_gs_chunk(Job& j)
-{
-    const int kend = std::min(j.kcap, j.k + kChunk);
-    // whole turns of the vector rotation as graph replays, the rest eager
-    const int n3 = (kend - j.k) / 3 * 3;
-    if (n3 > 0) {
-        j.C->steps_graph(j.k, n3);
-        j.k += n3;
+    const size_t D = dim_of(rq[0]->L, rq[0]->sz2);
+    int ncf = 0, kmax = 0;
+    for (int i = 0; i < n; ++i) {
+        ncf += rq[i]->kind == 1 ? 1 : 0;
+        kmax = std::max(kmax, std::max(rq[i]->maxiter, rq[i]->cf_steps));
     }
-    for (; j.k < kend; ++j.k) j.C->step(j.k, nullptr);
-    j.C->flush();
-    copy_scalars(j, j.k);
-    hip_check(hipEventRecord(j.ev, j.s), "event");
-    j.inflight = true;
+    const size_t nv = (size_t)cols_width(n), nvr = ncf ? (size_t)cols_width(ncf) : 0;
+    const size_t vecs = std::max(3 * nv, 4 * nvr);
+    return vecs * D * sizeof(double) + (size_t)(kmax + 8) * 16 * nv + (2 << 20);
 }
 
-void start_job(Job& j)
+constexpr int kChunk = 12;   // recurrence steps enqueued per poll
+
+enum Phase { kGS, kRitz, kCF, kDone };
+
+struct Column {
+    int id = 0;
+    qkrylov_serve_request_t rq{};
+    LanczosReplay rp{0, 0.0, false, {}, {}};
+    int kcap = 0, n_gs = 0;
+    std::vector<double> z, nrm_gs;   // Ritz coefficients and the GS norms they divide by
+    double* gs_out = nullptr;        // caller's host buffer for the ground state (or null)
+};
+
+struct Batch {
+    std::vector<Column> cols;
+    std::vector<int> cf;             // indices into cols of the S^zz columns
+    int L = 0, sz2 = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    Pinned *ha = nullptr, *hn = nullptr;   // the slot's read-back buffers
+    std::shared_ptr<BasisSpec> basis;
+    std::unique_ptr<Engine> H, Op;
+    std::unique_ptr<ColsCore> C;
+    DevBuf ritz, coef, psi2;
+    Phase phase = kGS;
+    int k = 0, kend = 0;             // steps enqueued; steps the phase runs at most
+    bool inflight = false;
+    size_t footprint = 0;
+    uint64_t seed = 1234;
+};
+
+// column c of a width-nv block gets the start vector a single request uses
+// (uniform [-1, 1) from the counter RNG, counter = row)
+void fill_start(Batch& b, int nv)
+{
+    const size_t D = (size_t)b.basis->dim();
+    for (int c = 0; c < nv; ++c)
+        if (qkl_fill_random(0, D, b.seed, (uint64_t)0 - (uint64_t)c * D, 1, nv, c, b.C->cur(), b.s) != 0)
+            throw std::runtime_error("start vector");
+}
+
+void copy_scalars(Batch& b, int nsteps)
+{
+    const size_t nv = (size_t)b.C->nv;
+    hip_check(hipMemcpyAsync(b.ha->p, b.C->alpha.d(), nsteps * nv * sizeof(double), hipMemcpyDeviceToHost, b.s), "d2h");
+    hip_check(hipMemcpyAsync(b.hn->p, b.C->nrm.d(), (nsteps + 1) * nv * sizeof(double), hipMemcpyDeviceToHost, b.s),
+              "d2h");
+}
+
+void enqueue_gs_chunk(Batch& b)
+{
+    const int kend = std::min(b.kend, b.k + kChunk);
+    for (; b.k < kend; ++b.k) b.C->step(b.k);
+    copy_scalars(b, b.k);
+    hip_check(hipEventRecord(b.ev, b.s), "event");
+    b.inflight = true;
+}
+
+void start_batch(Batch& b)
 {
     Sector sec;
     sec.use_sz = true;
-    sec.sz2 = j.rq.sz2;
-    j.basis = std::make_shared<BasisSpec>(Kind::SpinHalf, j.rq.L, sec);
+    sec.sz2 = b.sz2;
+    b.basis = std::make_shared<BasisSpec>(Kind::SpinHalf, b.L, sec);
     EngineOptions eo;
-    eo.stream = j.s;
-    eo.NL = choose_NL(*j.basis);   // the split whose tables the run keeps resident
-    j.H = std::make_unique<Engine>(j.basis, Kind::SpinHalf, xxz_ring(j.rq.L, j.rq.delta), eo);
-    j.kcap = (int)std::min<int64_t>(j.rq.maxiter, j.basis->dim());
-    const int cap = std::max(j.kcap, j.rq.cf_steps) + 2;
-    j.C = std::make_unique<RealCore>(*j.H, cap, j.s, false, true);
-    j.rp = std::make_unique<LanczosReplay>(LanczosReplay{j.kcap, j.rq.tol, false, {}, {}});
-    j.seed = 1234;
-    if (qkl_fill_random(0, (size_t)j.basis->dim(), j.seed, 0, 1, 1, 0, j.C->cur(), j.s) != 0)
-        throw std::runtime_error("start vector");
-    j.C->init_norm();
-    j.phase = kGS;
-    j.k = 0;
-    enqueue_gs_chunk(j);
+    eo.stream = b.s;
+    eo.NL = choose_NL(*b.basis);   // the split whose tables the run keeps resident
+    // the exchange part and the Ising diagonal at Delta = 1; column c scales
+    // the diagonal by its own Delta
+    b.H = std::make_unique<Engine>(b.basis, Kind::SpinHalf, xxz_ring(b.L, 1.0), eo);
+    const int n = (int)b.cols.size(), nv = cols_width(n);
+    std::vector<double> dsc(nv);
+    b.kend = 0;
+    for (int c = 0; c < nv; ++c) dsc[c] = b.cols[c < n ? c : 0].rq.delta;   // padding repeats column 0
+    for (auto& col : b.cols) {
+        col.kcap = kcap_of(col.rq);
+        col.rp = LanczosReplay{col.kcap, col.rq.tol, false, {}, {}};
+        b.kend = std::max(b.kend, col.kcap);
+        if (col.rq.kind == 1) b.cf.push_back((int)(&col - b.cols.data()));
+    }
+    b.C = std::make_unique<ColsCore>(*b.H, nv, b.kend + 2, dsc.data(), b.s, true);
+    fill_start(b, nv);
+    b.C->init_norm();
+    b.phase = kGS;
+    b.k = 0;
+    enqueue_gs_chunk(b);
 }
 
-// called when the job's last event completed; enqueues the next phase
-void advance(Job& j, qkrylov_serve_result_t& res, double* cf_a, double* cf_b, int cf_cap)
+// the CF columns' Ritz vectors: the recurrence again from the same start
+// vectors, each column adding z_k q_k / |q_k| for its own k < n_gs
+void start_ritz(Batch& b)
 {
-    const size_t D = (size_t)j.basis->dim();
-    if (j.phase == kGS) {
-        while (!j.rp->done && j.rp->next < j.k) j.rp->feed(j.ha->p[j.rp->next], j.hn->p[j.rp->next + 1]);
-        if (!j.rp->done && j.k >= j.kcap) j.rp->done = true;
-        if (!j.rp->done) {
-            enqueue_gs_chunk(j);
-            return;
-        }
-        j.n_gs = (int)j.rp->alphas.size();
-        res.iterations = j.rp->next;
-        res.converged = j.rp->converged ? 1 : 0;
-        res.energy = tridiag_min(j.rp->alphas.data(), j.rp->betas.data(), j.n_gs, j.rq.kind == 1 ? &j.z : nullptr);
-        if (j.rq.kind != 1) {
-            j.phase = kDone;
-            return;
-        }
-        // Ritz vector by a second pass over the same deterministic recurrence
-        j.nrm_gs.assign(j.hn->p, j.hn->p + j.n_gs);
-        j.ritz = DevBuf(D * sizeof(double), j.s);
-        j.psi2 = DevBuf(2 * sizeof(double), j.s);
-        if (qkl_fill_zero(D, j.ritz.d(), j.s) != 0) throw std::runtime_error("zero");
-        if (qkl_fill_random(0, D, j.seed, 0, 1, 1, 0, j.C->cur(), j.s) != 0) throw std::runtime_error("start");
-        j.C->init_norm();
-        for (int kk = 0; kk < j.n_gs; ++kk) {
-            if (qkl_axpy(0, D, j.z[kk] / j.nrm_gs[kk], 0.0, j.C->cur(), j.ritz.d(), j.s) != 0)
-                throw std::runtime_error("axpy");
-            if (kk + 1 < j.n_gs) j.C->step(kk, nullptr);
-        }
-        hip_check(hipEventRecord(j.ev, j.s), "event");
-        j.inflight = true;
-        j.phase = kRitz;
-        return;
+    const size_t D = (size_t)b.basis->dim();
+    const int ncf = (int)b.cf.size(), nvr = cols_width(ncf);
+    int nmax = 0;
+    for (int j : b.cf) nmax = std::max(nmax, b.cols[j].n_gs);
+    // the coefficients go through the slot's pinned buffer, whose GS
+    // read-back has been consumed
+    double* cf = b.hn->p;
+    std::fill(cf, cf + (size_t)nmax * nvr, 0.0);
+    std::vector<double> dsc(nvr);
+    int cfmax = 0;
+    for (int j = 0; j < nvr; ++j) {
+        const Column& col = b.cols[b.cf[j < ncf ? j : 0]];
+        dsc[j] = col.rq.delta;
+        if (j >= ncf) continue;   // padding: coefficient 0 keeps its Ritz column at zero
+        cfmax = std::max(cfmax, col.rq.cf_steps);
+        for (int kk = 0; kk < col.n_gs; ++kk) cf[(size_t)kk * nvr + j] = col.z[kk] / col.nrm_gs[kk];
     }
-    if (j.phase == kRitz) {
-        // phi = S^z_pi psi, then the plain recurrence from phi
-        Sector sec;
-        sec.use_sz = true;
-        sec.sz2 = j.rq.sz2;
-        EngineOptions eo;
-        eo.stream = j.s;
-        eo.NL = choose_NL(*j.basis);
-        j.Op = std::make_unique<Engine>(j.basis, Kind::SpinHalf, szpi(j.rq.L), eo);
-        j.Op->apply(j.ritz.d(), j.C->cur(), false, j.s);
-        if (qkl_dot_partials(0, D, j.ritz.d(), j.ritz.d(), j.C->partB.d(), j.s) != 0) throw std::runtime_error("dot");
-        if (qkl_reduce(j.C->partB.d(), j.C->NB, 1, 1, 0, 0, j.psi2.d(), 0, 1, j.s) != 0)
-            throw std::runtime_error("reduce");
-        j.C->init_norm();
-        {
-            const int n3 = j.rq.cf_steps / 3 * 3;
-            if (n3 > 0) j.C->steps_graph(0, n3);
-            for (int kk = n3; kk < j.rq.cf_steps; ++kk) j.C->step(kk, nullptr);
-        }
-        j.C->flush();
-        copy_scalars(j, j.rq.cf_steps);
-        hip_check(hipMemcpyAsync(j.ha->p + j.rq.cf_steps + 1, j.psi2.d(), sizeof(double), hipMemcpyDeviceToHost, j.s),
-                  "d2h");
-        hip_check(hipEventRecord(j.ev, j.s), "event");
-        j.inflight = true;
-        j.phase = kCF;
-        return;
+    b.C.reset();   // stream-ordered release of the GS blocks
+    b.C = std::make_unique<ColsCore>(*b.H, nvr, std::max(nmax, cfmax) + 2, dsc.data(), b.s, true);
+    b.ritz = DevBuf(D * nvr * sizeof(double), b.s);
+    b.coef = DevBuf(std::max<size_t>((size_t)nmax * nvr, 1) * sizeof(double), b.s);
+    b.psi2 = DevBuf(nvr * sizeof(double), b.s);
+    hip_check(hipMemcpyAsync(b.coef.get(), cf, (size_t)nmax * nvr * sizeof(double), hipMemcpyHostToDevice, b.s),
+              "h2d");
+    if (qkl_fill_zero(D * nvr, b.ritz.d(), b.s) != 0) throw std::runtime_error("zero");
+    fill_start(b, nvr);
+    b.C->init_norm();
+    for (int kk = 0; kk < nmax; ++kk) {
+        if (qkl_axpy_cols(nvr, D, b.coef.d() + (size_t)kk * nvr, b.C->cur(), b.ritz.d(), b.s) != 0)
+            throw std::runtime_error("axpy");
+        if (kk + 1 < nmax) b.C->step(kk);
     }
-    if (j.phase == kCF) {
-        const double n0 = j.hn->p[0];
-        const double psi2 = j.ha->p[j.rq.cf_steps + 1];
-        res.cf_norm = (n0 < 1e-15 || psi2 <= 0) ? 0.0 : n0 / std::sqrt(psi2);
-        int na = 0;
-        if (n0 >= 1e-15) {
-            for (int kk = 0; kk < j.rq.cf_steps; ++kk) {
-                if (cf_a && na < cf_cap) cf_a[(size_t)j.id * cf_cap + na] = j.ha->p[kk];
-                ++na;
-                const double b = j.hn->p[kk + 1];
-                if (b < 1e-15) break;
-                if (cf_b && kk < cf_cap) cf_b[(size_t)j.id * cf_cap + kk] = b;
+    hip_check(hipEventRecord(b.ev, b.s), "event");
+    b.inflight = true;
+    b.kend = cfmax;
+    b.phase = kRitz;
+}
+
+// phi_c = S^z_pi psi_c, |psi_c|^2, then cf_steps steps of the plain recurrence from phi
+void start_cf(Batch& b)
+{
+    const size_t D = (size_t)b.basis->dim();
+    const int nvr = b.C->nv;
+    Sector sec;
+    sec.use_sz = true;
+    sec.sz2 = b.sz2;
+    EngineOptions eo;
+    eo.stream = b.s;
+    eo.NL = choose_NL(*b.basis);
+    b.Op = std::make_unique<Engine>(b.basis, Kind::SpinHalf, szpi(b.L), eo);
+    b.Op->apply_cols(b.ritz.d(), b.C->cur(), nvr, b.C->one.d(), b.s);
+    // an update with coefficient 0 / 1 leaves the Ritz block unchanged and
+    // yields its column sums of squares
+    if (qkl_update_cols(nvr, D, b.ritz.d(), b.ritz.d(), b.C->zero.d(), b.C->one.d(), 0, b.C->partB.d(), b.s) != 0 ||
+        qkl_reduce(b.C->partB.d(), b.C->NB, nvr, 1, 0, 0, b.psi2.d(), 0, nvr, b.s) != 0)
+        throw std::runtime_error("psi norms");
+    b.C->init_norm();
+    for (int kk = 0; kk < b.kend; ++kk) b.C->step(kk);
+    copy_scalars(b, b.kend);
+    hip_check(hipMemcpyAsync(b.ha->p + (size_t)(b.kend + 1) * nvr, b.psi2.d(), nvr * sizeof(double),
+                             hipMemcpyDeviceToHost, b.s),
+              "d2h");
+    hip_check(hipEventRecord(b.ev, b.s), "event");
+    b.inflight = true;
+    b.phase = kCF;
+}
+
+// called when the batch's last event completed: takes the results it
+// brings and enqueues the next phase; returns the requests it finished
+std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a, double* cf_b, int cf_cap)
+{
+    std::vector<int> finished;
+    const size_t D = (size_t)b.basis->dim();
+    if (b.phase == kGS) {
+        const int nv = b.C->nv;
+        bool all = true;
+        for (int c = 0; c < (int)b.cols.size(); ++c) {
+            Column& col = b.cols[c];
+            if (col.rp.done) continue;
+            while (!col.rp.done && col.rp.next < b.k)
+                col.rp.feed(b.ha->p[(size_t)col.rp.next * nv + c], b.hn->p[(size_t)(col.rp.next + 1) * nv + c]);
+            if (!col.rp.done && b.k >= col.kcap) col.rp.done = true;
+            if (!col.rp.done) {
+                all = false;
+                continue;
+            }
+            col.n_gs = (int)col.rp.alphas.size();
+            qkrylov_serve_result_t& r = results[col.id];
+            r.iterations = col.rp.next;
+            r.converged = col.rp.converged ? 1 : 0;
+            r.energy = tridiag_min(col.rp.alphas.data(), col.rp.betas.data(), col.n_gs,
+                                   col.rq.kind == 1 ? &col.z : nullptr);
+            if (col.rq.kind == 1) {
+                col.nrm_gs.resize(col.n_gs);
+                for (int kk = 0; kk < col.n_gs; ++kk) col.nrm_gs[kk] = b.hn->p[(size_t)kk * nv + c];
+            } else {
+                finished.push_back(col.id);
             }
         }
-        res.n_cf = na;
-        if (j.gs_out) {
-            // the normalised Ritz vector the continued fraction started from
-            hip_check(hipMemcpyAsync(j.gs_out, j.ritz.get(), D * sizeof(double), hipMemcpyDeviceToHost, j.s), "d2h");
-            hip_check(hipStreamSynchronize(j.s), "sync");
-            const double inv = (psi2 > 0) ? 1.0 / std::sqrt(psi2) : 0.0;
-            for (size_t i = 0; i < D; ++i) j.gs_out[i] *= inv;
+        if (!all) {
+            enqueue_gs_chunk(b);
+            return finished;
         }
-        j.phase = kDone;
+        if (b.cf.empty()) {
+            b.phase = kDone;
+            return finished;
+        }
+        start_ritz(b);
+        return finished;
     }
+    if (b.phase == kRitz) {
+        start_cf(b);
+        return finished;
+    }
+    if (b.phase == kCF) {
+        const int nvr = b.C->nv;
+        for (int j = 0; j < (int)b.cf.size(); ++j) {
+            Column& col = b.cols[b.cf[j]];
+            qkrylov_serve_result_t& res = results[col.id];
+            const double n0 = b.hn->p[j];
+            const double psi2 = b.ha->p[(size_t)(b.kend + 1) * nvr + j];
+            res.cf_norm = (n0 < 1e-15 || psi2 <= 0) ? 0.0 : n0 / std::sqrt(psi2);
+            int na = 0;
+            if (n0 >= 1e-15) {
+                for (int kk = 0; kk < col.rq.cf_steps; ++kk) {
+                    if (cf_a && na < cf_cap) cf_a[(size_t)col.id * cf_cap + na] = b.ha->p[(size_t)kk * nvr + j];
+                    ++na;
+                    const double be = b.hn->p[(size_t)(kk + 1) * nvr + j];
+                    if (be < 1e-15) break;
+                    if (cf_b && kk < cf_cap) cf_b[(size_t)col.id * cf_cap + kk] = be;
+                }
+            }
+            res.n_cf = na;
+            if (col.gs_out) {
+                // the normalised Ritz vector the continued fraction started from
+                hip_check(hipMemcpy2DAsync(col.gs_out, sizeof(double), b.ritz.d() + j, nvr * sizeof(double),
+                                           sizeof(double), D, hipMemcpyDeviceToHost, b.s),
+                          "d2h");
+                hip_check(hipStreamSynchronize(b.s), "sync");
+                const double inv = (psi2 > 0) ? 1.0 / std::sqrt(psi2) : 0.0;
+                for (size_t i = 0; i < D; ++i) col.gs_out[i] *= inv;
+            }
+            finished.push_back(col.id);
+        }
+        b.phase = kDone;
+    }
+    return finished;
 }
 
-// pinned read-back room a request needs (alpha and norm arrays)
+// pinned read-back room of one request's column (alpha and norm arrays, the psi norms)
 size_t readback_doubles(const qkrylov_serve_request_t& r)
 {
-    const size_t D = dim_of(r);
-    const int kcap = (int)std::min<size_t>((size_t)r.maxiter, D);
-    return (size_t)std::max(kcap, r.cf_steps) + 2 + 3;
+    return (size_t)std::max(kcap_of(r), r.cf_steps) + 2 + 3;
 }
 
 } // namespace
@@ -267,17 +382,19 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         const size_t budget = (opt && opt->hbm_budget_bytes) ? opt->hbm_budget_bytes : (size_t)(0.8 * free_b);
         const int max_active = (opt && opt->max_active > 0) ? opt->max_active : 32;
         const bool realtime = opt ? opt->realtime != 0 : true;
+        const int max_batch = (opt && opt->max_batch > 0) ? std::min(opt->max_batch, kMaxBatch) : kDefaultBatch;
         for (int i = 0; i < nreq; ++i) {
             if (reqs[i].L < 2 || reqs[i].L > 32 || reqs[i].maxiter <= 0 || reqs[i].cf_steps < 0) {
                 set_last_error("serve: request with L outside 2..32, maxiter <= 0 or cf_steps < 0");
                 return QKRYLOV_ERROR_INVALID_ARG;
             }
-            if (dim_of(reqs[i]) == 0) {
+            if (dim_of(reqs[i].L, reqs[i].sz2) == 0) {
                 set_last_error("serve: request whose Sz sector is empty (sz2 out of range or of the wrong parity "
                                "for L)");
                 return QKRYLOV_ERROR_INVALID_ARG;
             }
-            if (footprint_of(reqs[i]) > budget) {   // could never be admitted
+            const qkrylov_serve_request_t* one = &reqs[i];
+            if (footprint_of(&one, 1) > budget) {   // could never be admitted
                 set_last_error("serve: a request's device footprint exceeds the memory budget");
                 return QKRYLOV_ERROR_INVALID_ARG;
             }
@@ -295,19 +412,18 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
             BasisSpec b(Kind::SpinHalf, reqs[i].L, s);
             keep[key] = basis_tables(b, choose_NL(b));
         }
-        // streams, events and jobs are released on every exit; on an error
-        // the streams are drained first, so no job's memory is freed while
-        // its kernels may still run
-        // the read-back buffers outlive `release` below, which drains the
-        // streams before anything else is freed
+        // streams, events and batches are released on every exit; on an
+        // error the streams are drained first, so no batch's memory is freed
+        // while its kernels may still run; the read-back buffers outlive
+        // `release` below
         std::vector<std::unique_ptr<Pinned>> pin_a, pin_n;
         std::vector<hipStream_t> streams;
         std::vector<hipEvent_t> events;
-        std::vector<std::unique_ptr<Job>> active;
+        std::vector<std::unique_ptr<Batch>> active;
         struct Release {
             std::vector<hipStream_t>& st;
             std::vector<hipEvent_t>& ev;
-            std::vector<std::unique_ptr<Job>>& jobs;
+            std::vector<std::unique_ptr<Batch>>& jobs;
             ~Release()
             {
                 for (auto s : st) (void)hipStreamSynchronize(s);
@@ -318,6 +434,7 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         } release{streams, events, active};
         size_t rb = 1;
         for (int i = 0; i < nreq; ++i) rb = std::max(rb, readback_doubles(reqs[i]));
+        rb *= (size_t)cols_width(max_batch);
         for (int i = 0; i < max_active; ++i) {
             hipStream_t st = nullptr;
             hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
@@ -331,67 +448,99 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         std::vector<int> order(nreq);
         for (int i = 0; i < nreq; ++i) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival < reqs[b].arrival; });
-        std::deque<int> pending(order.begin(), order.end());
+        std::list<int> pending(order.begin(), order.end());
         std::vector<int> free_slots;
         for (int i = max_active - 1; i >= 0; --i) free_slots.push_back(i);
         std::vector<int> slot_of;
         size_t used = 0;
         int done = 0;
         const double t0 = now_s();
+        auto finish = [&](int id) {
+            auto& r = results[id];
+            r.finish = now_s() - t0;
+            r.latency = r.finish - (realtime ? reqs[id].arrival : 0.0);
+            ++done;
+        };
         while (done < nreq) {
             bool progress = false;
             const double now = now_s() - t0;
-            // admission: first fit in arrival order
+            // admission: first fit in arrival order; a request takes along
+            // the waiting requests of its sector that fit
             for (auto it = pending.begin(); it != pending.end() && !free_slots.empty();) {
                 const int id = *it;
                 if (realtime && reqs[id].arrival > now) break;
-                const size_t fp = footprint_of(reqs[id]);
-                if (used + fp > budget) {
+                std::vector<const qkrylov_serve_request_t*> mem{&reqs[id]};
+                if (used + footprint_of(mem.data(), 1) > budget) {
                     ++it;
                     continue;
                 }
-                auto j = std::make_unique<Job>();
-                j->id = id;
-                j->rq = reqs[id];
+                std::vector<std::list<int>::iterator> take{it};
+                for (auto jt = std::next(it); jt != pending.end() && (int)mem.size() < max_batch; ++jt) {
+                    const int o = *jt;
+                    if (realtime && reqs[o].arrival > now) break;
+                    if (reqs[o].L != reqs[id].L || reqs[o].sz2 != reqs[id].sz2) continue;
+                    mem.push_back(&reqs[o]);
+                    if (used + footprint_of(mem.data(), (int)mem.size()) > budget) {
+                        mem.pop_back();
+                        continue;
+                    }
+                    take.push_back(jt);
+                }
+                // a power-of-two batch (the rest stays waiting for the next one)
+                while (cols_width((int)take.size()) != (int)take.size()) {
+                    take.pop_back();
+                    mem.pop_back();
+                }
+                auto b = std::make_unique<Batch>();
+                b->L = reqs[id].L;
+                b->sz2 = reqs[id].sz2;
+                for (auto t : take) {
+                    Column col;
+                    col.id = *t;
+                    col.rq = reqs[*t];
+                    col.gs_out = (opt && opt->gs_vectors && reqs[*t].kind == 1) ? opt->gs_vectors[*t] : nullptr;
+                    b->cols.push_back(std::move(col));
+                }
                 const int slot = free_slots.back();
                 free_slots.pop_back();
-                j->s = streams[slot];
-                j->ev = events[slot];
-                j->ha = pin_a[slot].get();
-                j->hn = pin_n[slot].get();
-                j->gs_out = (opt && opt->gs_vectors && reqs[id].kind == 1) ? opt->gs_vectors[id] : nullptr;
-                j->footprint = fp;
-                j->start = now_s() - t0;
-                results[id].start = j->start;
-                start_job(*j);
-                used += fp;
+                b->s = streams[slot];
+                b->ev = events[slot];
+                b->ha = pin_a[slot].get();
+                b->hn = pin_n[slot].get();
+                b->footprint = footprint_of(mem.data(), (int)mem.size());
+                const double st = now_s() - t0;
+                for (const auto& col : b->cols) {
+                    results[col.id].start = st;
+                    results[col.id].batch = (int)b->cols.size();
+                }
+                // the iterator `it` is among those erased: continue after the last one
+                auto next = std::next(take.back());
+                for (auto t : take) pending.erase(t);
+                it = (take.size() == 1) ? next : pending.begin();
+                start_batch(*b);
+                used += b->footprint;
                 slot_of.push_back(slot);
-                active.push_back(std::move(j));
-                it = pending.erase(it);
+                active.push_back(std::move(b));
                 progress = true;
             }
             for (size_t a = 0; a < active.size();) {
-                Job& j = *active[a];
-                if (j.inflight) {
-                    const hipError_t q = hipEventQuery(j.ev);
+                Batch& b = *active[a];
+                if (b.inflight) {
+                    const hipError_t q = hipEventQuery(b.ev);
                     if (q == hipErrorNotReady) {
                         ++a;
                         continue;
                     }
                     hip_check(q, "event query");
-                    j.inflight = false;
+                    b.inflight = false;
                     progress = true;
-                    advance(j, results[j.id], cf_a, cf_b, cf_cap);
+                    for (int id : advance(b, results, cf_a, cf_b, cf_cap)) finish(id);
                 }
-                if (j.phase == kDone && !j.inflight) {
-                    auto& r = results[j.id];
-                    r.finish = now_s() - t0;
-                    r.latency = r.finish - (realtime ? reqs[j.id].arrival : 0.0);
-                    used -= j.footprint;
+                if (b.phase == kDone && !b.inflight) {
+                    used -= b.footprint;
                     free_slots.push_back(slot_of[a]);
                     active.erase(active.begin() + (long)a);
                     slot_of.erase(slot_of.begin() + (long)a);
-                    ++done;
                     continue;
                 }
                 ++a;

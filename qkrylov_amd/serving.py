@@ -33,10 +33,14 @@ def make_trace(n, rate=50.0, L_range=(12, 28), delta_range=(0.5, 1.5), max_abs_s
     return reqs
 
 
-def run(reqs, realtime=True, max_active=32, hbm_budget_bytes=0, cf_cap=None, want_ground_states=False):
+def run(reqs, realtime=True, max_active=32, hbm_budget_bytes=0, cf_cap=None, want_ground_states=False,
+        max_batch=0):
     """Runs the trace; with want_ground_states every kind-1 result also
     carries "ground_state", the normalised vector its continued fraction
-    started from."""
+    started from. Requests of one (L, Sz) sector that wait together run as
+    the columns of one batch of 1, 2, 4 or 8, at most max_batch (0: the
+    library's default of 4; 1: every request alone); each result's "batch"
+    says how many shared its batch."""
     n = len(reqs)
     cf_cap = cf_cap or max([r["cf_steps"] for r in reqs] + [1])
     arr = (_lib.ServeRequest * n)()
@@ -50,7 +54,8 @@ def run(reqs, realtime=True, max_active=32, hbm_budget_bytes=0, cf_cap=None, wan
     gs = [np.zeros(xxz_dim(r["L"], r["sz2"])) if (want_ground_states and r["kind"] == 1) else None for r in reqs]
     ptrs = (ctypes.c_void_p * n)(*[g.ctypes.data if g is not None else None for g in gs])
     opt = _lib.ServeOptions(int(hbm_budget_bytes), int(max_active), 1 if realtime else 0,
-                            ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)) if want_ground_states else None)
+                            ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)) if want_ground_states else None,
+                            int(max_batch))
     p = _lib.p_dbl
     _lib.check(_lib.load().qkrylov_serve_run(arr, n, ctypes.byref(opt), res, cfa.ctypes.data_as(p),
                                              cfb.ctypes.data_as(p), cf_cap, tot.ctypes.data_as(p)), "serve_run")

@@ -1,5 +1,6 @@
-"""Serving loop: every request gives the same result alone and in the mix
-(bitwise: fixed kernels and reduction order per request); energies agree
+"""Serving loop: every request gives the same result alone and in the mix,
+also as a column of a batch of its sector (bitwise: fixed kernels and
+reduction order per column, whatever the batch width); energies agree
 with dense diagonalisation and, up to L = 28, with the Bethe ansatz;
 S^zz(omega) from the loop's continued fractions agrees with the dense
 resolvent of the seed the loop used; admission respects the HBM budget."""
@@ -24,11 +25,40 @@ def test_alone_equals_mixed_bitwise():
     assert tot["solves_per_s"] > 0 and tot["p99_s"] >= tot["p50_s"] > 0
     for i, r in enumerate(reqs):
         alone, _ = serving.run([dict(r, arrival=0.0)], realtime=False, max_active=1)
-        a, m = alone[0], mixed[i]
-        assert a["energy"] == m["energy"] and a["iterations"] == m["iterations"]
-        assert a["n_cf"] == m["n_cf"]
-        assert np.array_equal(a["cf_alphas"], m["cf_alphas"]) and np.array_equal(a["cf_betas"], m["cf_betas"])
-        assert a["cf_norm"] == m["cf_norm"]
+        same_bits(alone[0], mixed[i])
+
+
+def same_bits(a, m):
+    assert a["energy"] == m["energy"] and a["iterations"] == m["iterations"]
+    assert a["converged"] == m["converged"] and a["n_cf"] == m["n_cf"]
+    assert np.array_equal(a["cf_alphas"], m["cf_alphas"]) and np.array_equal(a["cf_betas"], m["cf_betas"])
+    assert a["cf_norm"] == m["cf_norm"]
+
+
+def test_batched_columns_equal_alone_bitwise():
+    """Requests of three sectors, all waiting at t = 0, ground-state and
+    S^zz requests mixed, different anisotropies and tolerances: each
+    sector's requests run as the columns of power-of-two batches (9 = 4 + 4
+    + 1, 3 = 2 + 1, 2; with up to 8: 9 = 8 + 1), and every result is bit for
+    bit the one the request gets alone (a batch of one) and with batching
+    switched off."""
+    rng = np.random.default_rng(3)
+    reqs = []
+    for i, (L, sz2) in enumerate([(12, 0)] * 9 + [(13, 1)] * 3 + [(14, 2)] * 2):
+        reqs.append(dict(L=L, sz2=sz2, delta=float(rng.uniform(0.5, 1.5)), kind=i % 2, arrival=0.0,
+                         tol=1e-10 if i % 3 else 1e-12, cf_steps=40 + 5 * (i % 3), maxiter=200))
+    batched, _ = serving.run(reqs, realtime=False, max_active=4)
+    assert [b["batch"] for b in batched] == [4] * 8 + [1] + [2] * 2 + [1] + [2] * 2
+    wide, _ = serving.run(reqs, realtime=False, max_active=4, max_batch=8)
+    assert [b["batch"] for b in wide] == [8] * 8 + [1] + [2] * 2 + [1] + [2] * 2
+    unbatched, _ = serving.run(reqs, realtime=False, max_active=4, max_batch=1)
+    assert all(u["batch"] == 1 for u in unbatched)
+    for i, r in enumerate(reqs):
+        alone, _ = serving.run([r], realtime=False, max_active=1)
+        print(f"request {i}: L={r['L']} sz2={r['sz2']} kind={r['kind']} E={alone[0]['energy']!r}", flush=True)
+        for other in (batched, wide, unbatched):
+            same_bits(alone[0], other[i])
+        assert alone[0]["converged"] == 1
 
 
 def test_energies_and_cf_vs_oracle():
