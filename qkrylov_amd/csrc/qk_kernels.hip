@@ -233,6 +233,44 @@ __device__ __forceinline__ void accumulate(const double* __restrict__ x, const u
     }
 }
 
+// as accumulate, but only the lanes whose ok[u] is set load x[rt[u]]: the
+// others take 0.0 (their coefficient is 0 as well) and issue no memory access
+// for that entry, which spares the vector memory unit the cache lines of
+// the lanes a branch does not reach
+template <bool CPLX, int U>
+__device__ __forceinline__ void accumulate_masked(const double* __restrict__ x, const uint32_t* rt, const bool* ok,
+                                                  const double* cr, const double* ci, double& ar, double& ai)
+{
+    if (CPLX) {
+        double tr[U], ti[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            tr[u] = 0.0;
+            ti[u] = 0.0;
+            if (ok[u]) {
+                tr[u] = x[2 * (size_t)rt[u]];
+                ti[u] = x[2 * (size_t)rt[u] + 1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ar = fma(cr[u], tr[u], ar);
+            ar = fma(-ci[u], ti[u], ar);
+            ai = fma(cr[u], ti[u], ai);
+            ai = fma(ci[u], tr[u], ai);
+        }
+    } else {
+        double t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            t[u] = 0.0;
+            if (ok[u]) t[u] = x[rt[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) ar = fma(cr[u], t[u], ar);
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -294,18 +332,24 @@ __host__ __device__ __forceinline__ size_t apply_branch_bytes(int nB)
     return (((size_t)nB * (16 + 4 + 8 + 8)) + 15) & ~(size_t)15;
 }
 
-// Per-lane branches [gbeg, gend) of one tile, eight gathers in flight.
-// Branch-free: a branch that does not apply gathers the row's own x with
-// coefficient 0 (masked, not selected by control flow, so the rank and
-// value loads of the eight branches overlap).
+// Per-lane branches [gbeg, gend) of one tile, eight gathers in flight. A
+// branch that does not apply to a lane (or a lane past the tile's rows)
+// contributes with coefficient 0. Spin models (no parity masks) then issue
+// no load for that lane: the vector memory unit is their bound, and the
+// lines of lanes a branch does not reach cost it time (the 32-site ring's
+// apply 16 % faster, J1-J2 4x8 4 %). Fermion models, bound by their VALU
+// work, gather the row's own x instead, which needs no exec-mask changes.
+// The rank lookups and value loads of the eight branches overlap either way.
 template <bool CPLX, int RM, bool SIGNS, int KIND>
 __device__ __forceinline__ void lane_branches(int gbeg, int gend, const unsigned char* smem, const uint4* lb4,
                                               const uint32_t* lbm, const double* lbre, const double* lbim,
-                                              const uint4* tb, uint32_t lo, uint32_t row, const OwnRank& orank,
-                                              const QkScal& sc, const double* __restrict__ x, double& ar, double& ai)
+                                              const uint4* tb, uint32_t lo, uint32_t row, bool active,
+                                              const OwnRank& orank, const QkScal& sc, const double* __restrict__ x,
+                                              double& ar, double& ai)
 {
     for (int g = gbeg; g < gend; g += 8) {
         uint32_t rt[8];
+        bool ok[8];
         double cr[8], ci[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -322,11 +366,13 @@ __device__ __forceinline__ void lane_branches(int gbeg, int gend, const unsigned
                 bi ^= neg;
             }
             const uint64_t m64 = ((uint64_t)m << 32) | m;
-            rt[u] = row ^ ((row ^ (tv.x + lr)) & m);
+            rt[u] = SIGNS ? row ^ ((row ^ (tv.x + lr)) & m) : tv.x + lr;
+            ok[u] = active && m != 0u;
             cr[u] = __builtin_bit_cast(double, br & m64);
             ci[u] = __builtin_bit_cast(double, bi & m64);
         }
-        accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
+        if (SIGNS) accumulate<CPLX, 8>(x, rt, cr, ci, ar, ai);
+        else accumulate_masked<CPLX, 8>(x, rt, ok, cr, ci, ar, ai);
     }
 }
 
@@ -459,16 +505,19 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
         // lo-rank range in each target block, four in flight
         for (uint32_t i = 0; i < ntaken; i += 4) {
             uint32_t rt[4];
+            bool ok[4];
             double cr[4], ci[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const bool in = (i + u) < ntaken;
                 const uint32_t k = in ? i + u : 0u;
                 rt[u] = in ? a_base[k] + my_lr : row;
+                ok[u] = in && active;
                 cr[u] = in ? a_cr[k] : 0.0;
                 ci[u] = (CPLX && in) ? a_ci[k] : 0.0;
             }
-            accumulate<CPLX, 4>(x, rt, cr, ci, ar, ai);
+            if (SIGNS) accumulate<CPLX, 4>(x, rt, cr, ci, ar, ai);
+            else accumulate_masked<CPLX, 4>(x, rt, ok, cr, ci, ar, ai);
         }
 
         // (4) remaining branches: per-lane test of lo & lcm against the two
@@ -476,13 +525,15 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
         // two-level rank, three runs by rank kind
         const OwnRank orank = own_rank<RM>(smem, lo, sc);
         if (RM == 2) {
-            lane_branches<CPLX, RM, SIGNS, 1>(0, sc.nB1, smem, lb4, lbm, lbre, lbim, tb, lo, row, orank, sc, x, ar, ai);
-            lane_branches<CPLX, RM, SIGNS, 2>(sc.nB1, sc.nB1 + sc.nB2, smem, lb4, lbm, lbre, lbim, tb, lo, row, orank,
-                                              sc, x, ar, ai);
-            lane_branches<CPLX, RM, SIGNS, 0>(sc.nB1 + sc.nB2, nB, smem, lb4, lbm, lbre, lbim, tb, lo, row, orank, sc,
+            lane_branches<CPLX, RM, SIGNS, 1>(0, sc.nB1, smem, lb4, lbm, lbre, lbim, tb, lo, row, active, orank, sc,
                                               x, ar, ai);
+            lane_branches<CPLX, RM, SIGNS, 2>(sc.nB1, sc.nB1 + sc.nB2, smem, lb4, lbm, lbre, lbim, tb, lo, row,
+                                              active, orank, sc, x, ar, ai);
+            lane_branches<CPLX, RM, SIGNS, 0>(sc.nB1 + sc.nB2, nB, smem, lb4, lbm, lbre, lbim, tb, lo, row, active,
+                                              orank, sc, x, ar, ai);
         } else {
-            lane_branches<CPLX, RM, SIGNS, 0>(0, nB, smem, lb4, lbm, lbre, lbim, tb, lo, row, orank, sc, x, ar, ai);
+            lane_branches<CPLX, RM, SIGNS, 0>(0, nB, smem, lb4, lbm, lbre, lbim, tb, lo, row, active, orank, sc, x, ar,
+                                              ai);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
@@ -973,15 +1024,20 @@ __device__ __forceinline__ void st_cols(double* __restrict__ Y, size_t r, const 
     }
 }
 
-// acc[c] += sum_u cr[u] * X[rt[u]][c]: all U row loads first, then the FMAs
-// in u order for every column
+// acc[c] += sum_u cr[u] * X[rt[u]][c]: all U row loads first (only for the
+// lanes whose ok[u] is set; the others take 0.0 under a zero coefficient and
+// issue no load), then the FMAs in u order for every column
 template <int NV, int U>
-__device__ __forceinline__ void accumulate_cols(const double* __restrict__ X, const uint32_t* rt, const double* cr,
-                                                double* acc)
+__device__ __forceinline__ void accumulate_cols(const double* __restrict__ X, const uint32_t* rt, const bool* ok,
+                                                const double* cr, double* acc)
 {
     double t[U][NV];
 #pragma unroll
-    for (int u = 0; u < U; ++u) ld_cols<NV>(X, rt[u], t[u]);
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) t[u][c] = 0.0;
+        if (ok[u]) ld_cols<NV>(X, rt[u], t[u]);
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -990,13 +1046,14 @@ __device__ __forceinline__ void accumulate_cols(const double* __restrict__ X, co
 
 template <int RM, int KIND, int NV>
 __device__ __forceinline__ void lane_branches_cols(int gbeg, int gend, const unsigned char* smem, const uint4* lb4,
-                                                   const double* lbre, const uint4* tb, uint32_t lo, uint32_t row,
+                                                   const double* lbre, const uint4* tb, uint32_t lo, bool active,
                                                    const OwnRank& orank, const QkScal& sc,
                                                    const double* __restrict__ X, double* acc)
 {
     constexpr int U = NV >= 4 ? 4 : 8;
     for (int g = gbeg; g < gend; g += U) {
         uint32_t rt[U];
+        bool ok[U];
         double cr[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1006,10 +1063,11 @@ __device__ __forceinline__ void lane_branches_cols(int gbeg, int gend, const uns
             const uint32_t m = 0u - (uint32_t)((lf == tv.y) | (lf == tv.z));
             const uint32_t lr = target_rank<RM, KIND>(smem, lo, L.w, orank, sc);
             const uint64_t m64 = ((uint64_t)m << 32) | m;
-            rt[u] = row ^ ((row ^ (tv.x + lr)) & m);
+            rt[u] = tv.x + lr;
+            ok[u] = active && m != 0u;
             cr[u] = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, lbre[g + u]) & m64);
         }
-        accumulate_cols<NV, U>(X, rt, cr, acc);
+        accumulate_cols<NV, U>(X, rt, ok, cr, acc);
     }
 }
 
@@ -1115,25 +1173,28 @@ __global__ __launch_bounds__(512) void k_apply_cols(const QkTile* __restrict__ t
         // (3) taken hi-only branches, four in flight
         for (uint32_t i = 0; i < ntaken; i += 4) {
             uint32_t rt[4];
+            bool ok[4];
             double cr[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const bool in = (i + u) < ntaken;
                 const uint32_t k = in ? i + u : 0u;
                 rt[u] = in ? a_base[k] + my_lr : row;
+                ok[u] = in && active;
                 cr[u] = in ? a_cr[k] : 0.0;
             }
-            accumulate_cols<NV, 4>(X, rt, cr, acc);
+            accumulate_cols<NV, 4>(X, rt, ok, cr, acc);
         }
 
         // (4) per-lane branches
         const OwnRank orank = own_rank<RM>(smem, lo, sc);
         if (RM == 2) {
-            lane_branches_cols<RM, 1, NV>(0, sc.nB1, smem, lb4, lbre, tb, lo, row, orank, sc, X, acc);
-            lane_branches_cols<RM, 2, NV>(sc.nB1, sc.nB1 + sc.nB2, smem, lb4, lbre, tb, lo, row, orank, sc, X, acc);
-            lane_branches_cols<RM, 0, NV>(sc.nB1 + sc.nB2, nB, smem, lb4, lbre, tb, lo, row, orank, sc, X, acc);
+            lane_branches_cols<RM, 1, NV>(0, sc.nB1, smem, lb4, lbre, tb, lo, active, orank, sc, X, acc);
+            lane_branches_cols<RM, 2, NV>(sc.nB1, sc.nB1 + sc.nB2, smem, lb4, lbre, tb, lo, active, orank, sc, X,
+                                          acc);
+            lane_branches_cols<RM, 0, NV>(sc.nB1 + sc.nB2, nB, smem, lb4, lbre, tb, lo, active, orank, sc, X, acc);
         } else {
-            lane_branches_cols<RM, 0, NV>(0, nB, smem, lb4, lbre, tb, lo, row, orank, sc, X, acc);
+            lane_branches_cols<RM, 0, NV>(0, nB, smem, lb4, lbre, tb, lo, active, orank, sc, X, acc);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
