@@ -256,6 +256,10 @@ typedef struct {
     double start, finish;   /* seconds from trace start */
     double latency;         /* finish - arrival */
     int batch;              /* requests in the batch this one ran in */
+    int ritz_stored;        /* kind 1: the Ritz vector was summed from stored
+                             * Krylov vectors (1) or from the recurrence run
+                             * again (0, when the budget could not hold them);
+                             * the same numbers either way */
 } qkrylov_serve_result_t;
 
 typedef struct {

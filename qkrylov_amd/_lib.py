@@ -65,7 +65,7 @@ class ServeRequest(ctypes.Structure):
 class ServeResult(ctypes.Structure):
     _fields_ = [("energy", c_double), ("iterations", c_int), ("converged", c_int), ("n_cf", c_int),
                 ("cf_norm", c_double), ("start", c_double), ("finish", c_double), ("latency", c_double),
-                ("batch", c_int)]
+                ("batch", c_int), ("ritz_stored", c_int)]
 
 
 class ServeOptions(ctypes.Structure):

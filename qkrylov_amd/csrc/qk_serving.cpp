@@ -19,7 +19,11 @@
 //         columns), accumulating each column's Ritz vector;
 //   CF    S^z_pi applied to the Ritz vectors, then the continued-fraction
 //         recurrence of those columns.
-// A ground-state request is finished when the GS phase ends. Every kernel
+// While the GS phase runs, the Krylov vectors of the S^zz columns are kept
+// (in chunks of steps, allocated as the run goes on while the memory budget
+// allows), and the Ritz vectors are then summed from them; when the budget
+// cannot hold them, the Ritz phase recomputes them instead. Both give the
+// same bits. A ground-state request is finished when the GS phase ends. Every kernel
 // and reduction of a column has a fixed grid and order that depend neither
 // on the block width nor on the other columns, so a request's result is the
 // same bits alone (a batch of one) and in any batch.
@@ -129,11 +133,12 @@ struct Column {
     int kcap = 0, n_gs = 0;
     std::vector<double> z, nrm_gs;   // Ritz coefficients and the GS norms they divide by
     double* gs_out = nullptr;        // caller's host buffer for the ground state (or null)
+    bool ritz_stored = false;        // the Ritz vector came from stored Krylov vectors
 };
 
 struct Batch {
-    std::vector<Column> cols;
-    std::vector<int> cf;             // indices into cols of the S^zz columns
+    std::vector<Column> cols;        // the S^zz columns first
+    std::vector<int> cf;             // indices into cols of the S^zz columns (0 .. ncf - 1)
     int L = 0, sz2 = 0;
     hipStream_t s = nullptr;
     hipEvent_t ev = nullptr;
@@ -142,6 +147,13 @@ struct Batch {
     std::unique_ptr<Engine> H, Op;
     std::unique_ptr<ColsCore> C;
     DevBuf ritz, coef, psi2;
+    // stored Krylov vectors of the S^zz columns: step k in chunks[k / kChunk]
+    // at row-major width cols_width(ncf); `extra` bytes of them count against
+    // the budget; `store` is dropped (and the chunks freed) when the budget
+    // cannot take the next chunk
+    bool store = false;
+    std::vector<DevBuf> chunks;
+    size_t extra = 0;
     Phase phase = kGS;
     int k = 0, kend = 0;             // steps enqueued; steps the phase runs at most
     bool inflight = false;
@@ -167,16 +179,47 @@ void copy_scalars(Batch& b, int nsteps)
               "d2h");
 }
 
-void enqueue_gs_chunk(Batch& b)
+void drop_store(Batch& b, size_t& used)
+{
+    b.store = false;
+    b.chunks.clear();   // stream-ordered release
+    used -= b.extra;
+    b.extra = 0;
+}
+
+double* stored_step(Batch& b, int k)
+{
+    const size_t D = (size_t)b.basis->dim(), w = (size_t)cols_width((int)b.cf.size());
+    return b.chunks[(size_t)k / kChunk].d() + (size_t)(k % kChunk) * D * w;
+}
+
+void enqueue_gs_chunk(Batch& b, size_t& used, size_t budget)
 {
     const int kend = std::min(b.kend, b.k + kChunk);
-    for (; b.k < kend; ++b.k) b.C->step(b.k);
+    const size_t D = (size_t)b.basis->dim();
+    const int ncf = (int)b.cf.size(), w = ncf ? cols_width(ncf) : 0;
+    if (b.store && (size_t)b.k / kChunk >= b.chunks.size()) {
+        const size_t bytes = (size_t)kChunk * D * w * sizeof(double);
+        if (used + bytes <= budget) {
+            b.chunks.emplace_back(bytes, b.s);
+            used += bytes;
+            b.extra += bytes;
+        } else {
+            drop_store(b, used);
+        }
+    }
+    for (; b.k < kend; ++b.k) {
+        // q_k of the S^zz columns (the block's first ncf columns) before step k
+        if (b.store && qkl_copy_strided(D, ncf, b.C->cur(), b.C->nv, stored_step(b, b.k), w, b.s) != 0)
+            throw std::runtime_error("store Krylov vector");
+        b.C->step(b.k);
+    }
     copy_scalars(b, b.k);
     hip_check(hipEventRecord(b.ev, b.s), "event");
     b.inflight = true;
 }
 
-void start_batch(Batch& b)
+void start_batch(Batch& b, size_t& used, size_t budget)
 {
     Sector sec;
     sec.use_sz = true;
@@ -203,12 +246,16 @@ void start_batch(Batch& b)
     b.C->init_norm();
     b.phase = kGS;
     b.k = 0;
-    enqueue_gs_chunk(b);
+    b.store = !b.cf.empty();
+    enqueue_gs_chunk(b, used, budget);
 }
 
-// the CF columns' Ritz vectors: the recurrence again from the same start
-// vectors, each column adding z_k q_k / |q_k| for its own k < n_gs
-void start_ritz(Batch& b)
+void start_cf(Batch& b);
+
+// the CF columns' Ritz vectors: each column adds z_k q_k / |q_k| for its own
+// k < n_gs, from the stored Krylov vectors or, without them, from the
+// recurrence run again from the same start vectors
+void start_ritz(Batch& b, size_t& used)
 {
     const size_t D = (size_t)b.basis->dim();
     const int ncf = (int)b.cf.size(), nvr = cols_width(ncf);
@@ -235,6 +282,16 @@ void start_ritz(Batch& b)
     hip_check(hipMemcpyAsync(b.coef.get(), cf, (size_t)nmax * nvr * sizeof(double), hipMemcpyHostToDevice, b.s),
               "h2d");
     if (qkl_fill_zero(D * nvr, b.ritz.d(), b.s) != 0) throw std::runtime_error("zero");
+    b.kend = cfmax;
+    if (b.store) {
+        for (int kk = 0; kk < nmax; ++kk)
+            if (qkl_axpy_cols(nvr, D, b.coef.d() + (size_t)kk * nvr, stored_step(b, kk), b.ritz.d(), b.s) != 0)
+                throw std::runtime_error("axpy");
+        drop_store(b, used);   // freed in stream order, after the sums that read them
+        for (int j : b.cf) b.cols[j].ritz_stored = true;
+        start_cf(b);
+        return;
+    }
     fill_start(b, nvr);
     b.C->init_norm();
     for (int kk = 0; kk < nmax; ++kk) {
@@ -244,7 +301,6 @@ void start_ritz(Batch& b)
     }
     hip_check(hipEventRecord(b.ev, b.s), "event");
     b.inflight = true;
-    b.kend = cfmax;
     b.phase = kRitz;
 }
 
@@ -279,7 +335,8 @@ void start_cf(Batch& b)
 
 // called when the batch's last event completed: takes the results it
 // brings and enqueues the next phase; returns the requests it finished
-std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a, double* cf_b, int cf_cap)
+std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a, double* cf_b, int cf_cap,
+                         size_t& used, size_t budget)
 {
     std::vector<int> finished;
     const size_t D = (size_t)b.basis->dim();
@@ -310,14 +367,14 @@ std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a
             }
         }
         if (!all) {
-            enqueue_gs_chunk(b);
+            enqueue_gs_chunk(b, used, budget);
             return finished;
         }
         if (b.cf.empty()) {
             b.phase = kDone;
             return finished;
         }
-        start_ritz(b);
+        start_ritz(b, used);
         return finished;
     }
     if (b.phase == kRitz) {
@@ -343,6 +400,7 @@ std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a
                 }
             }
             res.n_cf = na;
+            res.ritz_stored = col.ritz_stored ? 1 : 0;
             if (col.gs_out) {
                 // the normalised Ritz vector the continued fraction started from
                 hip_check(hipMemcpy2DAsync(col.gs_out, sizeof(double), b.ritz.d() + j, nvr * sizeof(double),
@@ -445,6 +503,34 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
             pin_a.push_back(std::make_unique<Pinned>(rb));
             pin_n.push_back(std::make_unique<Pinned>(rb));
         }
+        // the device's stream-ordered pool keeps what the run frees (Krylov
+        // blocks, stored vectors) for the next batch instead of returning it
+        // at every synchronisation; restored when the run ends
+        struct PoolHold {
+            hipMemPool_t pool = nullptr;
+            uint64_t old = 0;
+            explicit PoolHold(int dev)
+            {
+                if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess ||
+                    hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &old) != hipSuccess) {
+                    pool = nullptr;
+                    (void)hipGetLastError();
+                    return;
+                }
+                uint64_t keep = UINT64_MAX;
+                if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+                    pool = nullptr;
+                    (void)hipGetLastError();
+                }
+            }
+            ~PoolHold()
+            {
+                if (pool) {
+                    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &old);
+                    (void)hipMemPoolTrimTo(pool, 0);
+                }
+            }
+        } pool_hold(dev);
         std::vector<int> order(nreq);
         for (int i = 0; i < nreq; ++i) order[i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival < reqs[b].arrival; });
@@ -494,13 +580,15 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                 auto b = std::make_unique<Batch>();
                 b->L = reqs[id].L;
                 b->sz2 = reqs[id].sz2;
-                for (auto t : take) {
-                    Column col;
-                    col.id = *t;
-                    col.rq = reqs[*t];
-                    col.gs_out = (opt && opt->gs_vectors && reqs[*t].kind == 1) ? opt->gs_vectors[*t] : nullptr;
-                    b->cols.push_back(std::move(col));
-                }
+                for (int pass = 0; pass < 2; ++pass)   // the S^zz columns first
+                    for (auto t : take) {
+                        if ((reqs[*t].kind == 1) != (pass == 0)) continue;
+                        Column col;
+                        col.id = *t;
+                        col.rq = reqs[*t];
+                        col.gs_out = (opt && opt->gs_vectors && reqs[*t].kind == 1) ? opt->gs_vectors[*t] : nullptr;
+                        b->cols.push_back(std::move(col));
+                    }
                 const int slot = free_slots.back();
                 free_slots.pop_back();
                 b->s = streams[slot];
@@ -517,8 +605,8 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                 auto next = std::next(take.back());
                 for (auto t : take) pending.erase(t);
                 it = (take.size() == 1) ? next : pending.begin();
-                start_batch(*b);
                 used += b->footprint;
+                start_batch(*b, used, budget);
                 slot_of.push_back(slot);
                 active.push_back(std::move(b));
                 progress = true;
@@ -534,10 +622,10 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                     hip_check(q, "event query");
                     b.inflight = false;
                     progress = true;
-                    for (int id : advance(b, results, cf_a, cf_b, cf_cap)) finish(id);
+                    for (int id : advance(b, results, cf_a, cf_b, cf_cap, used, budget)) finish(id);
                 }
                 if (b.phase == kDone && !b.inflight) {
-                    used -= b.footprint;
+                    used -= b.footprint + b.extra;
                     free_slots.push_back(slot_of[a]);
                     active.erase(active.begin() + (long)a);
                     slot_of.erase(slot_of.begin() + (long)a);

@@ -61,6 +61,24 @@ def test_batched_columns_equal_alone_bitwise():
         assert alone[0]["converged"] == 1
 
 
+def test_ritz_from_stored_vectors_equals_recomputed():
+    """The S^zz requests' Ritz vectors summed from the Krylov vectors kept
+    during the ground-state run, and recomputed by running the recurrence
+    again when the memory budget cannot hold those vectors: the same bits."""
+    reqs = [dict(L=14, sz2=0, delta=d, kind=1, arrival=0.0, tol=1e-10, cf_steps=60, maxiter=200)
+            for d in (0.8, 1.3)]
+    D = serving.xxz_dim(14, 0)
+    stored, _ = serving.run(reqs, realtime=False, max_active=2, want_ground_states=True)
+    assert [o["ritz_stored"] for o in stored] == [1, 1] and [o["batch"] for o in stored] == [2, 2]
+    fp = 8 * D * 8 + (200 + 8) * 16 * 2 + (2 << 20)      # the batch's admission footprint
+    again, _ = serving.run(reqs, realtime=False, max_active=2, want_ground_states=True,
+                           hbm_budget_bytes=fp + 10 * D * 8)   # less than one chunk of 12 steps
+    assert [o["ritz_stored"] for o in again] == [0, 0] and [o["batch"] for o in again] == [2, 2]
+    for a, b in zip(stored, again):
+        same_bits(a, b)
+        assert np.array_equal(a["ground_state"], b["ground_state"])
+
+
 def test_energies_and_cf_vs_oracle():
     """Every energy against dense diagonalisation (1e-10 relative). For the
     S^zz requests the continued fraction is checked on the seed the engine
