@@ -480,7 +480,7 @@ void Engine::apply_cols(const double* X, double* Y, int nv, const double* dscale
     if (cols_width(nv) != nv) throw std::invalid_argument("column block width must be 1, 2, 4 or 8");
     if (!is_real() || wide()) throw std::invalid_argument("column blocks need a real Hamiltonian on <= 32 bits");
     if (T_.signs != 0) throw std::invalid_argument("column blocks: operators with parity masks are not supported");
-    lcheck(qkl_apply_cols(&T_, 0, nv, X, Y, dscale, -1, nullptr, nullptr, nullptr, 0, s), "apply_cols");
+    lcheck(qkl_apply_cols(&T_, 0, nv, X, Y, dscale, -1, nullptr, nullptr, nullptr, 0, nullptr, 0, s), "apply_cols");
 }
 
 void Engine::apply_block(const double* X, double* Y, int ld, int ncols, hipStream_t s) const
@@ -632,7 +632,7 @@ ColsCore::ColsCore(const Engine& e, int nv_, int kcap_, const double* dscale, hi
     nrm = mk((size_t)(kcap + 3) * nv * sizeof(double));
     npA = qkl_cols_partials(E.tables());
     if (npA <= 0) throw std::runtime_error("column apply: the operator tables do not fit in LDS");
-    NB = qkl_blas_partials();
+    NB = qkl_cols_update_partials(D);
     partA = mk((size_t)npA * nv * sizeof(double));
     partB = mk((size_t)NB * nv * sizeof(double));
     dsc = mk(nv * sizeof(double));
@@ -651,20 +651,24 @@ ColsCore::ColsCore(const Engine& e, int nv_, int kcap_, const double* dscale, hi
 
 void ColsCore::init_norm()
 {
-    // an update with coefficient 0 / 1 leaves cur as it is and yields its
-    // column sums of squares
-    lcheck(qkl_update_cols(nv, D, cur(), cur(), zero.d(), one.d(), 0, partB.d(), s), "norm0");
-    lcheck(qkl_reduce(partB.d(), NB, nv, 1, 1, 0, nrm.d(), 0, nv, s), "reduce norm0");
+    // an update with coefficient 0 / 1 leaves cur as it is and yields the
+    // partial sums of its squared column norms, which step 0 totals
+    lcheck(qkl_update_cols(nv, D, cur(), cur(), zero.d(), one.d(), 0, nullptr, 0, partB.d(), s), "norm0");
+}
+
+void ColsCore::flush(int k)
+{
+    lcheck(qkl_total_cols(nv, partB.d(), NB, nrm.d(), k, s), "total norms");
 }
 
 void ColsCore::step(int k)
 {
     if (k < 0 || k > kcap) throw std::logic_error("column recurrence: step beyond the capacity");
     double* w = Q[iw].d();
-    lcheck(qkl_apply_cols(E.tables(), 0, nv, cur(), w, dsc.d(), k, prv(), nrm.d(), partA.d(), npA, s), "apply_cols");
-    lcheck(qkl_reduce(partA.d(), npA, nv, 1, 0, 0, alpha.d(), k, nv, s), "reduce alpha");
-    lcheck(qkl_update_cols(nv, D, w, cur(), alpha.d(), nrm.d(), k, partB.d(), s), "update_cols");
-    lcheck(qkl_reduce(partB.d(), NB, nv, 1, 1, 0, nrm.d(), k + 1, nv, s), "reduce beta");
+    // the apply totals nrm[k] from partB, the update alpha[k] from partA
+    lcheck(qkl_apply_cols(E.tables(), 0, nv, cur(), w, dsc.d(), k, prv(), nrm.d(), partA.d(), npA, partB.d(), NB, s),
+           "apply_cols");
+    lcheck(qkl_update_cols(nv, D, w, cur(), alpha.d(), nrm.d(), k, partA.d(), npA, partB.d(), s), "update_cols");
     const int t = ip;
     ip = ic;
     ic = iw;

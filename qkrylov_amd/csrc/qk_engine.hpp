@@ -240,10 +240,15 @@ struct ColsCore {
     ColsCore(const Engine& e, int nv, int kcap, const double* dscale, hipStream_t st, bool stream_ordered);
     double* cur() { return Q[ic].d(); }
     double* prv() { return Q[ip].d(); }
-    // nrm[0][c] = |cur_c| (cur unchanged): a new recurrence from cur (step 0
-    // reads no previous vector)
+    // a new recurrence from cur (cur unchanged; step 0 reads no previous
+    // vector): the partial sums of |cur_c|^2, which step 0 totals into nrm[0]
     void init_norm();
-    void step(int k);    // alpha[k][c], nrm[k + 1][c]; cur becomes q_{k+1}
+    // step k: nrm[k][c] (totalled by its apply), alpha[k][c], and the partial
+    // sums of nrm[k + 1][c]; cur becomes q_{k+1}
+    void step(int k);
+    // nrm[k][c] now (after step k - 1 or init_norm at k = 0), before the host
+    // or another kernel reads it; the same bits step k would form
+    void flush(int k);
 };
 // widths ColsCore and qkl_apply_cols take: the smallest of 1, 2, 4, 8 holding n columns (0 if n > 8)
 int cols_width(int n);

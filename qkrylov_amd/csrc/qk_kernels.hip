@@ -988,7 +988,10 @@ __global__ __launch_bounds__(QK_TPB) void k_update_block(size_t nrows, int ncol,
 // order, the FMA order, the reduction tree) depends neither on NV nor on the
 // other columns, and the launch grid depends only on the tables, so a column
 // gets the same bits in a block of any width. Real spin tables (no parity
-// masks), 32-bit states.
+// masks), 32-bit states. In the Lanczos form the kernel totals nrm[k][c]
+// itself from the previous update's partial sums (nrm_part[c * n_nrm_part
+// + i], fixed order, workgroup 0 stores it), so a step needs no separate
+// reduction launch.
 template <int NV>
 __device__ __forceinline__ void ld_cols(const double* __restrict__ X, size_t r, double* v)
 {
@@ -1081,12 +1084,14 @@ __global__ __launch_bounds__(512) void k_apply_cols(const QkTile* __restrict__ t
                                                      const QkDiagTerm* __restrict__ dg, QkScal sc,
                                                      const double* __restrict__ X, double* __restrict__ Y,
                                                      const double* __restrict__ dscale, int lz_k,
-                                                     const double* __restrict__ P, const double* __restrict__ nrm,
-                                                     double* __restrict__ partials, int npart, int table_bytes)
+                                                     const double* __restrict__ P, double* __restrict__ nrm,
+                                                     double* __restrict__ partials, int npart, int table_bytes,
+                                                     const double* __restrict__ nrm_part, int n_nrm_part)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     // static LDS: QK_COLS_STATIC_LDS bytes
     __shared__ double wsum[8][NV];
+    __shared__ double tot_sh[NV];
     const int nA = sc.nA, nB = sc.nB, nbk = sc.nA + sc.nB;
     uint4* lb4 = reinterpret_cast<uint4*>(smem + table_bytes);
     double* lbre = reinterpret_cast<double*>(smem + table_bytes + ((size_t)nB * 20 + 7) / 8 * 8);
@@ -1113,7 +1118,8 @@ __global__ __launch_bounds__(512) void k_apply_cols(const QkTile* __restrict__ t
         cprev[c] = 0.0;
         part[c] = 0.0;
         if (LZ) {
-            const double nk = nrm[(size_t)lz_k * NV + c];
+            const double nk = sqrt(block_total(nrm_part + (size_t)c * n_nrm_part, n_nrm_part, &tot_sh[c]));
+            if (blockIdx.x == 0 && threadIdx.x == 0) nrm[(size_t)lz_k * NV + c] = nk;
             inv_n[c] = 1.0 / nk;
             if (lz_k > 0) cprev[c] = nk / nrm[(size_t)(lz_k - 1) * NV + c];
         }
@@ -1229,20 +1235,30 @@ __global__ __launch_bounds__(512) void k_apply_cols(const QkTile* __restrict__ t
 
 // Lanczos update of NV row-major columns, each with its own coefficient:
 // w[r][c] -= (alpha[k][c] / nrm[k][c]) q[r][c]; partials[c * grid + b] of
-// |w_c|^2. Rows are thread-strided over a fixed grid, so a column's partial
-// sums do not depend on NV.
+// |w_c|^2. With a_part, alpha[k][c] is first totalled from the apply's
+// partial sums (a_part[c * n_a_part + i], fixed order; workgroup 0 stores
+// it); without, it is read. Rows are thread-strided over a grid that
+// depends only on nrows, so a column's partial sums do not depend on NV.
 template <int NV>
 __global__ __launch_bounds__(QK_TPB) void k_update_cols(size_t nrows, double* __restrict__ w,
-                                                         const double* __restrict__ q,
-                                                         const double* __restrict__ alpha,
+                                                         const double* __restrict__ q, double* __restrict__ alpha,
                                                          const double* __restrict__ nrm, int k,
+                                                         const double* __restrict__ a_part, int n_a_part,
                                                          double* __restrict__ partials)
 {
     __shared__ double wsum[QK_WPB][NV];
+    __shared__ double tot_sh[NV];
     double coef[NV], part[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
-        coef[c] = alpha[(size_t)k * NV + c] / nrm[(size_t)k * NV + c];
+        double a;
+        if (a_part) {
+            a = block_total(a_part + (size_t)c * n_a_part, n_a_part, &tot_sh[c]);
+            if (blockIdx.x == 0 && threadIdx.x == 0) alpha[(size_t)k * NV + c] = a;
+        } else {
+            a = alpha[(size_t)k * NV + c];
+        }
+        coef[c] = a / nrm[(size_t)k * NV + c];
         part[c] = 0.0;
     }
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1464,6 +1480,17 @@ __global__ __launch_bounds__(64) void k_total(const double* __restrict__ p, int 
     __shared__ double tot_sh;
     const double t = block_total(p, n, &tot_sh);
     if (threadIdx.x == 0) out[idx] = root ? sqrt(t) : t;
+}
+
+// out[idx * ncol + c] = sqrt(total of p[c * n + i]) for column c = blockIdx.x,
+// in the order of block_total (what the column apply's prologue computes)
+__global__ __launch_bounds__(64) void k_total_cols(const double* __restrict__ p, int n, double* __restrict__ out,
+                                                   int idx, int ncol)
+{
+    __shared__ double tot_sh;
+    const int c = blockIdx.x;
+    const double t = block_total(p + (size_t)c * n, n, &tot_sh);
+    if (threadIdx.x == 0) out[(size_t)idx * ncol + c] = sqrt(t);
 }
 
 __global__ void k_set_int(int* __restrict__ p, int v)
@@ -1767,8 +1794,8 @@ struct ApplyCfg {
 #define QK_LDS_LIMIT ((size_t)160 * 1024)
 #define QK_APPLY_STATIC_LDS ((size_t)17 * sizeof(double))
 
-// static LDS of k_apply_cols (wsum[8][NV] at the widest NV = 8)
-#define QK_COLS_STATIC_LDS ((size_t)8 * 8 * sizeof(double))
+// static LDS of k_apply_cols (wsum[8][NV] and tot_sh[NV] at the widest NV = 8)
+#define QK_COLS_STATIC_LDS ((size_t)9 * 8 * sizeof(double))
 
 inline int apply_lds_plan(size_t tables, int nB, int nA, bool cplx, int pref_waves, size_t static_lds, int* waves,
                           size_t* dyn_lds)
@@ -1995,36 +2022,45 @@ const void* lz_apply_fn(const QkTables* T)
 
 } // namespace
 
+struct ColsLz {
+    const double* P = nullptr;        // previous block
+    double* nrm = nullptr;            // nrm[k * nv + c]
+    double* partials = nullptr;       // this apply's partial dots
+    int npart = 0;
+    const double* nrm_part = nullptr; // the previous update's partial squared norms
+    int n_nrm_part = 0;
+};
+
 template <int RM, bool LZ, int NV>
 int launch_cols(const QkTables* T, const ApplyCfg& c, const double* X, double* Y, const double* dscale, int k,
-                const double* P, const double* nrm, double* partials, int npart, hipStream_t s)
+                const ColsLz& lz, hipStream_t s)
 {
     hipLaunchKernelGGL((k_apply_cols<RM, LZ, NV>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles, T->hoff, T->lrank,
-                       T->tl, T->th, T->nth, T->lolist, T->brk, T->dg, make_scal(T), X, Y, dscale, k, P, nrm, partials,
-                       npart, (int)table_bytes(T));
+                       T->tl, T->th, T->nth, T->lolist, T->brk, T->dg, make_scal(T), X, Y, dscale, k, lz.P, lz.nrm,
+                       lz.partials, lz.npart, (int)table_bytes(T), lz.nrm_part, lz.n_nrm_part);
     return rc(hipGetLastError());
 }
 
 template <int RM, bool LZ>
 int cols_nv(int nv, const QkTables* T, const ApplyCfg& c, const double* X, double* Y, const double* dscale, int k,
-            const double* P, const double* nrm, double* partials, int npart, hipStream_t s)
+            const ColsLz& lz, hipStream_t s)
 {
     switch (nv) {
-    case 1: return launch_cols<RM, LZ, 1>(T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
-    case 2: return launch_cols<RM, LZ, 2>(T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
-    case 4: return launch_cols<RM, LZ, 4>(T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
-    case 8: return launch_cols<RM, LZ, 8>(T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    case 1: return launch_cols<RM, LZ, 1>(T, c, X, Y, dscale, k, lz, s);
+    case 2: return launch_cols<RM, LZ, 2>(T, c, X, Y, dscale, k, lz, s);
+    case 4: return launch_cols<RM, LZ, 4>(T, c, X, Y, dscale, k, lz, s);
+    case 8: return launch_cols<RM, LZ, 8>(T, c, X, Y, dscale, k, lz, s);
     default: return -1;
     }
 }
 
 template <int NV>
-int launch_cols_blas(int op, size_t nrows, double* w, const double* q, const double* a, const double* nrm, int k,
-                     double* partials, hipStream_t s)
+int launch_cols_blas(int op, size_t nrows, double* w, const double* q, double* a, const double* nrm, int k,
+                     const double* a_part, int n_a_part, double* partials, hipStream_t s)
 {
     if (op == 0)
-        hipLaunchKernelGGL(k_update_cols<NV>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, a, nrm, k,
-                           partials);
+        hipLaunchKernelGGL(k_update_cols<NV>, dim3(grid_for(nrows)), dim3(QK_TPB), 0, s, nrows, w, q, a, nrm, k, a_part,
+                           n_a_part, partials);
     else if (op == 1)
         hipLaunchKernelGGL(k_axpy_cols<NV>, dim3(grid_for(nrows)), dim3(QK_TPB), 0, s, nrows, a, q, w);
     else
@@ -2032,14 +2068,14 @@ int launch_cols_blas(int op, size_t nrows, double* w, const double* q, const dou
     return rc(hipGetLastError());
 }
 
-int cols_blas(int nv, int op, size_t nrows, double* w, const double* q, const double* a, const double* nrm, int k,
-              double* partials, hipStream_t s)
+int cols_blas(int nv, int op, size_t nrows, double* w, const double* q, double* a, const double* nrm, int k,
+              const double* a_part, int n_a_part, double* partials, hipStream_t s)
 {
     switch (nv) {
-    case 1: return launch_cols_blas<1>(op, nrows, w, q, a, nrm, k, partials, s);
-    case 2: return launch_cols_blas<2>(op, nrows, w, q, a, nrm, k, partials, s);
-    case 4: return launch_cols_blas<4>(op, nrows, w, q, a, nrm, k, partials, s);
-    case 8: return launch_cols_blas<8>(op, nrows, w, q, a, nrm, k, partials, s);
+    case 1: return launch_cols_blas<1>(op, nrows, w, q, a, nrm, k, a_part, n_a_part, partials, s);
+    case 2: return launch_cols_blas<2>(op, nrows, w, q, a, nrm, k, a_part, n_a_part, partials, s);
+    case 4: return launch_cols_blas<4>(op, nrows, w, q, a, nrm, k, a_part, n_a_part, partials, s);
+    case 8: return launch_cols_blas<8>(op, nrows, w, q, a, nrm, k, a_part, n_a_part, partials, s);
     default: return -1;
     }
 }
@@ -2073,39 +2109,54 @@ int qkl_cols_partials(const QkTables* T)
 }
 
 int qkl_apply_cols(const QkTables* T, int wide, int nv, const double* X, double* Y, const double* dscale, int k,
-                   const double* P, const double* nrm, double* partials, int npart, qkl_stream sv)
+                   const double* P, double* nrm, double* partials, int npart, const double* nrm_part,
+                   int n_nrm_part, qkl_stream sv)
 {
     hipStream_t s = (hipStream_t)sv;
     if (wide || T->signs != 0 || !dscale) return -1;
     if (T->ntiles == 0) return 0;
     ApplyCfg c;
     if (!cols_config(T, c)) return -2;
-    if (k >= 0 && (npart != c.nb || !nrm || !partials || (k > 0 && !P))) return -1;
+    if (k >= 0 && (npart != c.nb || !nrm || !partials || !nrm_part || n_nrm_part <= 0 || (k > 0 && !P))) return -1;
     const bool lz = k >= 0;
+    ColsLz a;
+    a.P = P;
+    a.nrm = nrm;
+    a.partials = partials;
+    a.npart = npart;
+    a.nrm_part = nrm_part;
+    a.n_nrm_part = n_nrm_part;
     switch (T->rank_mode) {
-    case 2: return lz ? cols_nv<2, true>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s)
-                      : cols_nv<2, false>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
-    case 1: return lz ? cols_nv<1, true>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s)
-                      : cols_nv<1, false>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
-    default: return lz ? cols_nv<0, true>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s)
-                       : cols_nv<0, false>(nv, T, c, X, Y, dscale, k, P, nrm, partials, npart, s);
+    case 2: return lz ? cols_nv<2, true>(nv, T, c, X, Y, dscale, k, a, s) : cols_nv<2, false>(nv, T, c, X, Y, dscale, k, a, s);
+    case 1: return lz ? cols_nv<1, true>(nv, T, c, X, Y, dscale, k, a, s) : cols_nv<1, false>(nv, T, c, X, Y, dscale, k, a, s);
+    default: return lz ? cols_nv<0, true>(nv, T, c, X, Y, dscale, k, a, s) : cols_nv<0, false>(nv, T, c, X, Y, dscale, k, a, s);
     }
 }
 
-int qkl_update_cols(int nv, size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, int k,
-                    double* partials, qkl_stream sv)
+int qkl_cols_update_partials(size_t nrows) { return grid_for(nrows); }
+
+int qkl_update_cols(int nv, size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
+                    const double* a_part, int n_a_part, double* partials, qkl_stream sv)
 {
-    return cols_blas(nv, 0, nrows, w, q, alpha, nrm, k, partials, (hipStream_t)sv);
+    if (a_part && n_a_part <= 0) return -1;
+    return cols_blas(nv, 0, nrows, w, q, alpha, nrm, k, a_part, n_a_part, partials, (hipStream_t)sv);
+}
+
+int qkl_total_cols(int nv, const double* partials, int npart, double* out, int idx, qkl_stream sv)
+{
+    if (npart <= 0 || nv <= 0) return -1;
+    hipLaunchKernelGGL(k_total_cols, dim3(nv), dim3(64), 0, (hipStream_t)sv, partials, npart, out, idx, nv);
+    return rc(hipGetLastError());
 }
 
 int qkl_axpy_cols(int nv, size_t nrows, const double* coef, const double* X, double* Y, qkl_stream sv)
 {
-    return cols_blas(nv, 1, nrows, Y, X, coef, nullptr, 0, nullptr, (hipStream_t)sv);
+    return cols_blas(nv, 1, nrows, Y, X, const_cast<double*>(coef), nullptr, 0, nullptr, 0, nullptr, (hipStream_t)sv);
 }
 
 int qkl_scale_rows_cols(int nv, size_t nrows, const double* d, const double* X, double* Y, qkl_stream sv)
 {
-    return cols_blas(nv, 2, nrows, Y, X, d, nullptr, 0, nullptr, (hipStream_t)sv);
+    return cols_blas(nv, 2, nrows, Y, X, const_cast<double*>(d), nullptr, 0, nullptr, 0, nullptr, (hipStream_t)sv);
 }
 
 int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y, qkl_stream sv)

@@ -82,20 +82,30 @@ int qkl_reduce(const double* partials, int npart, int ncol, int group, int mode,
 /* Multi-column apply: nv in {1, 2, 4, 8} real columns stored row-major
  * (X[row * nv + c]); column c uses H with its diagonal scaled by dscale[c]
  * (device array of nv). Real spin tables without parity masks, 32-bit
- * states (wide = 0), else -1. Plain form (Y = H_c X) for k < 0; Lanczos
- * form for k >= 0, per column as qkl_apply_block's (nrm[k * nv + c], P the
- * previous block, partials[c * npart + b] with npart = qkl_cols_partials).
- * A column's bits do not depend on nv or on the other columns.
- * qkl_update_cols: the Lanczos update of nv row-major columns, per-column
- * coefficients alpha[k * nv + c] / nrm[k * nv + c], partial sums of the
- * squared norms (npart = qkl_blas_partials). qkl_axpy_cols: Y[r][c] +=
- * coef[c] X[r][c] for every column whose coefficient is not zero.
- * qkl_scale_rows_cols: Y[r][c] = d[r] X[r][c]. */
+ * states (wide = 0), else -1. Plain form (Y = H_c X) for k < 0. Lanczos
+ * form for k >= 0, per column: nrm[k * nv + c] = sqrt(total of
+ * nrm_part[c * n_nrm_part + i]) (the previous update's partial sums, fixed
+ * order; workgroup 0 stores it), then W = H_c X / nrm[k] - nrm[k] P /
+ * nrm[k - 1] and partials[c * npart + b] of <X / nrm[k], W> with npart =
+ * qkl_cols_partials. A column's bits do not depend on nv or on the other
+ * columns.
+ * qkl_update_cols: the Lanczos update of nv row-major columns,
+ * w -= (alpha[k][c] / nrm[k][c]) q, with alpha[k][c] first totalled from
+ * a_part (the apply's partials, same order as above; workgroup 0 stores
+ * it) when a_part is given, else read; partial sums of |w_c|^2 at
+ * partials[c * npart + b], npart = qkl_cols_update_partials(nrows).
+ * qkl_total_cols: out[idx * nv + c] = sqrt(total of partials[c * npart +
+ * i]) in the same order (the value the next apply would form).
+ * qkl_axpy_cols: Y[r][c] += coef[c] X[r][c] for every column whose
+ * coefficient is not zero. qkl_scale_rows_cols: Y[r][c] = d[r] X[r][c]. */
 int qkl_cols_partials(const qk::QkTables* T);
+int qkl_cols_update_partials(size_t nrows);
 int qkl_apply_cols(const qk::QkTables* T, int wide, int nv, const double* X, double* Y, const double* dscale, int k,
-                   const double* P, const double* nrm, double* partials, int npart, qkl_stream s);
-int qkl_update_cols(int nv, size_t nrows, double* w, const double* q, const double* alpha, const double* nrm, int k,
-                    double* partials, qkl_stream s);
+                   const double* P, double* nrm, double* partials, int npart, const double* nrm_part, int n_nrm_part,
+                   qkl_stream s);
+int qkl_update_cols(int nv, size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
+                    const double* a_part, int n_a_part, double* partials, qkl_stream s);
+int qkl_total_cols(int nv, const double* partials, int npart, double* out, int idx, qkl_stream s);
 int qkl_axpy_cols(int nv, size_t nrows, const double* coef, const double* X, double* Y, qkl_stream s);
 int qkl_scale_rows_cols(int nv, size_t nrows, const double* d, const double* X, double* Y, qkl_stream s);
 

@@ -174,6 +174,7 @@ void fill_start(Batch& b, int nv)
 void copy_scalars(Batch& b, int nsteps)
 {
     const size_t nv = (size_t)b.C->nv;
+    b.C->flush(nsteps);
     hip_check(hipMemcpyAsync(b.ha->p, b.C->alpha.d(), nsteps * nv * sizeof(double), hipMemcpyDeviceToHost, b.s), "d2h");
     hip_check(hipMemcpyAsync(b.hn->p, b.C->nrm.d(), (nsteps + 1) * nv * sizeof(double), hipMemcpyDeviceToHost, b.s),
               "d2h");
@@ -319,7 +320,8 @@ void start_cf(Batch& b)
     b.Op->apply_cols(b.ritz.d(), b.C->cur(), nvr, b.C->one.d(), b.s);
     // an update with coefficient 0 / 1 leaves the Ritz block unchanged and
     // yields its column sums of squares
-    if (qkl_update_cols(nvr, D, b.ritz.d(), b.ritz.d(), b.C->zero.d(), b.C->one.d(), 0, b.C->partB.d(), b.s) != 0 ||
+    if (qkl_update_cols(nvr, D, b.ritz.d(), b.ritz.d(), b.C->zero.d(), b.C->one.d(), 0, nullptr, 0, b.C->partB.d(),
+                        b.s) != 0 ||
         qkl_reduce(b.C->partB.d(), b.C->NB, nvr, 1, 0, 0, b.psi2.d(), 0, nvr, b.s) != 0)
         throw std::runtime_error("psi norms");
     b.C->init_norm();
