@@ -5,13 +5,17 @@
     python -m qkrylov_amd.profiling kernels DIR/run_results.db
     python -m qkrylov_amd.profiling counters FETCH.db WRITE.db --copy-bytes N [--match k_apply<]
     python -m qkrylov_amd.profiling pmc DB [DB ...] [--match k_apply<]
+    python -m qkrylov_amd.profiling quad-model [--rows 8 --cols 4 --nl 14 --tiles 200]
 
 `kernels` prints a per-kernel table (calls, mean/min/max duration) from a
 `--kernel-trace` run; `counters` prints FETCH_SIZE / WRITE_SIZE per dispatch
 (KiB) for the kernels whose name contains `--match`, scaled by what the same
 counter reads for the plain copy kernel `k_copy` of known size; `pmc`
 prints the mean per dispatch of every counter collected in the databases
-for the matching kernels.
+for the matching kernels; `quad-model` counts, for sampled 64-row tiles of
+the J1-J2 apply, the vector-memory accesses of its gathers as the L1 counts
+them (per group of four lanes, one access per distinct 64 B sector), with
+and without loads from the lanes a branch does not reach.
 """
 import argparse
 import sqlite3
@@ -134,6 +138,71 @@ def calibrated_traffic_split(fetch_db, write_db, copy_bytes):
     return res
 
 
+def quad_sectors(addr, active=None):
+    """Accesses of one 64-lane gather as the L1 counts them: per group of
+    four lanes, the number of distinct 64 B sectors its lanes (those with
+    active set, when given) load from byte addresses addr."""
+    total = 0
+    for q in range(len(addr) // 4):
+        a = addr[4 * q:4 * q + 4]
+        if active is not None:
+            a = a[active[4 * q:4 * q + 4]]
+        total += len({int(v) >> 6 for v in a})
+    return total
+
+
+def quad_model(rows=8, cols=4, NL=14, tiles=200, seed=0):
+    """Vector-memory accesses per 64-row tile of the pull-form apply of the
+    J1-J2 Heisenberg model on a rows x cols torus (sites r * cols + c), Sz = 0,
+    split at NL low bits: one gather instruction per branch a tile issues
+    (hi-only branches only where the tile's hi takes them), costed per group
+    of four lanes as the number of distinct 64 B sectors its lanes load.
+    Returns {(kind, variant): accesses per tile} with kind in hionly /
+    inblock / mixed and variant 'own' (lanes a branch does not reach load
+    their own row) or 'masked' (they load nothing)."""
+    import numpy as np
+    from math import comb
+    from . import models
+    N = rows * cols
+    NH, nup = N - NL, N // 2
+    bonds = sorted({(min(f[0][1], f[1][1]), max(f[0][1], f[1][1]))
+                    for c, f in models.j1j2_square(rows, cols) if f[0][0] == "Sp"})
+    pc_lo = np.array([bin(v).count("1") for v in range(1 << NL)])
+    lrank = np.zeros(1 << NL, dtype=np.int64)
+    for k in range(NL + 1):
+        idx = np.where(pc_lo == k)[0]
+        lrank[idx] = np.arange(len(idx))
+    pc_hi = np.array([bin(v).count("1") for v in range(1 << NH)])
+    size = np.array([comb(NL, nup - p) if 0 <= nup - p <= NL else 0 for p in pc_hi])
+    hoff = np.concatenate([[0], np.cumsum(size)[:-1]])
+    LM = (1 << NL) - 1
+    rng = np.random.default_rng(seed)
+    acc, n = {}, 0
+    while n < tiles:
+        hi = int(rng.integers(0, 1 << NH))
+        c = nup - pc_hi[hi]
+        if not 0 <= c <= NL:
+            continue
+        los = np.where(pc_lo == c)[0]
+        if len(los) < 64:
+            continue
+        t0 = int(rng.integers(0, len(los) - 63))
+        s = (hi << NL) | los[t0:t0 + 64]
+        own = hoff[hi] + lrank[s & LM]
+        n += 1
+        for a, b in bonds:
+            act = ((s >> a) & 1) != ((s >> b) & 1)
+            hi_only = a >= NL and b >= NL
+            if hi_only and not act.any():
+                continue
+            kind = "hionly" if hi_only else ("inblock" if a < NL and b < NL else "mixed")
+            t = s ^ ((1 << a) | (1 << b))
+            tgt = np.where(act, hoff[t >> NL] + lrank[t & LM], own) * 8
+            acc[(kind, "own")] = acc.get((kind, "own"), 0) + quad_sectors(tgt)
+            acc[(kind, "masked")] = acc.get((kind, "masked"), 0) + quad_sectors(tgt, act)
+    return {k: v / n for k, v in acc.items()}
+
+
 def _main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m qkrylov_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -147,7 +216,18 @@ def _main(argv=None):
     m = sub.add_parser("pmc")
     m.add_argument("dbs", nargs="+")
     m.add_argument("--match", default="k_apply<")
+    q = sub.add_parser("quad-model")
+    q.add_argument("--rows", type=int, default=8)
+    q.add_argument("--cols", type=int, default=4)
+    q.add_argument("--nl", type=int, default=14)
+    q.add_argument("--tiles", type=int, default=200)
     a = ap.parse_args(argv)
+    if a.cmd == "quad-model":
+        r = quad_model(a.rows, a.cols, a.nl, a.tiles)
+        for variant in ("own", "masked"):
+            parts = {k: round(r[(k, variant)], 1) for k in ("hionly", "inblock", "mixed")}
+            print(f"{variant:7s} {parts} total {sum(parts.values()):.1f} accesses per 64-row tile")
+        return 0
     if a.cmd == "pmc":
         for db in a.dbs:
             for ctr, (v, n) in sorted(counter_means(db, a.match).items()):
