@@ -6,6 +6,7 @@
     python -m qkrylov_amd.profiling counters FETCH.db WRITE.db --copy-bytes N [--match k_apply<]
     python -m qkrylov_amd.profiling pmc DB [DB ...] [--match k_apply<]
     python -m qkrylov_amd.profiling quad-model [--rows 8 --cols 4 --nl 14 --tiles 200]
+    python -m qkrylov_amd.profiling tj-locality [--samples 1500]
 
 `kernels` prints a per-kernel table (calls, mean/min/max duration) from a
 `--kernel-trace` run; `counters` prints FETCH_SIZE / WRITE_SIZE per dispatch
@@ -15,7 +16,9 @@ prints the mean per dispatch of every counter collected in the databases
 for the matching kernels; `quad-model` counts, for sampled 64-row tiles of
 the J1-J2 apply, the vector-memory accesses of its gathers as the L1 counts
 them (per group of four lanes, one access per distinct 64 B sector), with
-and without loads from the lanes a branch does not reach.
+and without loads from the lanes a branch does not reach; `tj-locality`
+compares how close (in rows) the targets of the t-J 4x5 entries lie to their
+rows in the public basis order and in a hole-major order.
 """
 import argparse
 import sqlite3
@@ -203,6 +206,92 @@ def quad_model(rows=8, cols=4, NL=14, tiles=200, seed=0):
     return {k: v / n for k, v in acc.items()}
 
 
+def _tj_rank_tables(n, nup, ndn):
+    """cnt[i][u][d]: number of t-J configurations of sites 0..i-1 (each site
+    empty, up or down) with u up and d down electrons."""
+    cnt = [[[0] * (ndn + 1) for _ in range(nup + 1)] for _ in range(n + 1)]
+    cnt[0][0][0] = 1
+    for i in range(n):
+        for u in range(nup + 1):
+            for d in range(ndn + 1):
+                c = cnt[i][u][d]
+                if not c:
+                    continue
+                cnt[i + 1][u][d] += c
+                if u < nup:
+                    cnt[i + 1][u + 1][d] += c
+                if d < ndn:
+                    cnt[i + 1][u][d + 1] += c
+    return cnt
+
+
+def tj_public_rank(state, n, nup, ndn, cnt):
+    """Rank of a t-J state (up at bit 2i, down at bit 2i + 1 of site i) among
+    all states of the sector in ascending bit-string order: the order of the
+    engine's (and the reference's) t-J basis. Site values ascend as empty
+    (0) < up (1) < down (2); walking from the top site, every smaller value at
+    a site adds the completions of the sites below it."""
+    r, u, d = 0, nup, ndn
+    for i in range(n - 1, -1, -1):
+        v = (state >> (2 * i)) & 3
+        if v >= 1:
+            r += cnt[i][u][d]              # site i empty instead
+        if v == 2 and u >= 1:
+            r += cnt[i][u - 1][d]          # site i up instead
+        if v == 1:
+            u -= 1
+        elif v == 2:
+            d -= 1
+    return r
+
+
+def tj_hole_major_rank(state, n, nholes, nup):
+    """Rank with the hole positions slowest (combinatorial rank of the hole
+    set) and the spin string of the occupied sites, in site order, fastest."""
+    from math import comb
+    vals = [(state >> (2 * i)) & 3 for i in range(n)]
+    holes = [i for i, v in enumerate(vals) if v == 0]
+    spins = [v for v in vals if v != 0]
+    ups = [k for k, v in enumerate(spins) if v == 1]
+    rh = sum(comb(p, j + 1) for j, p in enumerate(holes))
+    rs = sum(comb(p, j + 1) for j, p in enumerate(ups))
+    return rh * comb(n - nholes, nup) + rs
+
+
+def tj_locality(rows=4, cols=5, nup=9, ndn=9, samples=1500, windows=(64, 8000, 64000, 500000), seed=0):
+    """Fraction of the off-diagonal entries of sampled rows of the t-J
+    Hamiltonian (open rows x cols lattice, nearest-neighbour hops and
+    exchange) whose target row lies within w rows of the source, for the
+    public (ascending bit-string) order and for a hole-major order. With R =
+    64 columns a row of the block is 512 B, so w = 8000 rows is one XCD's 4 MB
+    L2 and 500000 the 256 MB memory-side cache."""
+    import numpy as np
+    from . import models
+    n = rows * cols
+    cnt = _tj_rank_tables(n, nup, ndn)
+    D = cnt[n][nup][ndn]
+    bonds = models.square_bonds(rows, cols, periodic=False)
+    rng = np.random.default_rng(seed)
+    # random states of the sector: a random arrangement of holes, ups and downs
+    base = [0] * (n - nup - ndn) + [1] * nup + [2] * ndn
+    dp, dh = [], []
+    for _ in range(samples):
+        sites = rng.permutation(base)
+        s = sum(int(v) << (2 * i) for i, v in enumerate(sites))
+        rp, rh = tj_public_rank(s, n, nup, ndn, cnt), tj_hole_major_rank(s, n, n - nup - ndn, nup)
+        for i, j in bonds:
+            a, b = (s >> (2 * i)) & 3, (s >> (2 * j)) & 3
+            if a == b:
+                continue
+            # a hop (one site empty) or an exchange (opposite spins) swaps the two sites
+            t = s & ~((3 << (2 * i)) | (3 << (2 * j))) | (b << (2 * i)) | (a << (2 * j))
+            dp.append(abs(tj_public_rank(t, n, nup, ndn, cnt) - rp))
+            dh.append(abs(tj_hole_major_rank(t, n, n - nup - ndn, nup) - rh))
+    dp, dh = np.array(dp), np.array(dh)
+    return dict(D=D, entries_per_row=len(dp) / samples,
+                windows={w: (float(np.mean(dp < w)), float(np.mean(dh < w))) for w in windows})
+
+
 def _main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m qkrylov_amd.profiling")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -221,7 +310,15 @@ def _main(argv=None):
     q.add_argument("--cols", type=int, default=4)
     q.add_argument("--nl", type=int, default=14)
     q.add_argument("--tiles", type=int, default=200)
+    tj = sub.add_parser("tj-locality")
+    tj.add_argument("--samples", type=int, default=1500)
     a = ap.parse_args(argv)
+    if a.cmd == "tj-locality":
+        r = tj_locality(samples=a.samples)
+        print(f"D = {r['D']}, off-diagonal entries per row {r['entries_per_row']:.2f}")
+        for w, (pub, hm) in r["windows"].items():
+            print(f"within {w:7d} rows: public order {pub:.3f}, hole-major order {hm:.3f}")
+        return 0
     if a.cmd == "quad-model":
         r = quad_model(a.rows, a.cols, a.nl, a.tiles)
         for variant in ("own", "masked"):

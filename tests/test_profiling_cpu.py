@@ -27,3 +27,31 @@ def test_quad_model_invariants_small_lattice():
     assert r[("hionly", "masked")] == r[("hionly", "own")]
     # every instruction costs at least one access per quad of lanes that load
     assert r[("hionly", "own")] >= 16
+
+
+def test_tj_ranks_against_enumeration():
+    """The counting rank equals the position in the ascending enumeration of
+    the sector, and the hole-major rank is a bijection onto 0..D-1 (3x2
+    lattice, 2 up, 2 down: D = 6!/(2! 2! 2!) = 90)."""
+    n, nup, ndn = 6, 2, 2
+    states = []
+    for s in range(1 << (2 * n)):
+        vals = [(s >> (2 * i)) & 3 for i in range(n)]
+        if 3 in vals or vals.count(1) != nup or vals.count(2) != ndn:
+            continue
+        states.append(s)
+    assert len(states) == 90
+    cnt = profiling._tj_rank_tables(n, nup, ndn)
+    assert cnt[n][nup][ndn] == 90
+    assert [profiling.tj_public_rank(s, n, nup, ndn, cnt) for s in states] == list(range(90))
+    hm = sorted(profiling.tj_hole_major_rank(s, n, n - nup - ndn, nup) for s in states)
+    assert hm == list(range(90))
+
+
+def test_tj_locality_counts_entries():
+    r = profiling.tj_locality(rows=2, cols=3, nup=2, ndn=2, samples=200, windows=(1, 100))
+    assert r["D"] == 90
+    assert 0 < r["entries_per_row"] <= 7            # at most one entry per bond
+    for pub, hm in r["windows"].values():
+        assert 0.0 <= pub <= 1.0 and 0.0 <= hm <= 1.0
+    assert r["windows"][100] == (1.0, 1.0)          # every target lies within D rows
