@@ -306,6 +306,39 @@ def cpu_lanczos_c1(budget_s=20.0):
             "algorithm": "oracle restatement of src/solvers/lanczos.cpp:101-202 (CGS reorthogonalisation)"}
 
 
+def cpu_ftlm_cf_tj4x4(n_vec=4, n_steps=100):
+    """The oracle's restatement of the reference's FTLM (ftlm.cpp:122-170:
+    one three-term recurrence per random vector, no reorthogonalisation,
+    one inverse temperature per call) and continued fraction
+    (dynamics.cpp:12-59) on the scaled C4 instance the reference can build,
+    t-J 4x4 open, 2 holes (D = 411,840), on this host's cores: n_vec
+    Rademacher vectors x n_steps steps, and one 100-step fraction."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as O
+    from qkrylov_amd import models
+    m = O.Model(O.TJ, 16, O.sector(use_nup=1, nup=7, use_ndn=1, ndn=7), O.TJ, models.tj_model(4, 4, 1.0, 0.4),
+                enumerate=False)
+    t0 = time.perf_counter()
+    m.build_csr()
+    t_build = time.perf_counter() - t0
+    rng = np.random.default_rng(42)
+    rv = np.where(rng.integers(0, 2, (n_vec, m.dim)) == 1, 1.0, -1.0) + 0j
+    t0 = time.perf_counter()
+    Z, E, Cv = m.ftlm(1.0, rv, n_steps)
+    t_ftlm = time.perf_counter() - t0
+    phi = rng.uniform(-1, 1, m.dim) + 0j
+    t0 = time.perf_counter()
+    al, be, _ = m.cf(phi, 100)
+    t_cf = time.perf_counter() - t0
+    return {"workload": "t-J 4x4 open, t=1 J=0.4, 7 up 7 dn (2 holes), the scaled C4 instance", "D": m.dim,
+            "csr_build_seconds": round(t_build, 3),
+            "ftlm": {"vectors": n_vec, "steps": n_steps, "seconds": round(t_ftlm, 3),
+                     "seconds_per_vector": round(t_ftlm / n_vec, 4), "E_at_beta_1": E},
+            "cf_100_steps_seconds": round(t_cf, 3),
+            "algorithm": "oracle restatements of src/solvers/ftlm.cpp:24-53,122-170 and dynamics.cpp:12-59"}
+
+
 def host_info():
     model = None
     try:
@@ -625,6 +658,8 @@ def main():
         if not args.no_extras:
             out["cpu_baseline"]["lanczos_c1_reference"] = cpu_lanczos_c1()
             log(f"[bench] cpu reference Lanczos (C1): {out['cpu_baseline']['lanczos_c1_reference']}")
+            out["cpu_baseline"]["ftlm_cf_tj4x4_reference"] = cpu_ftlm_cf_tj4x4()
+            log(f"[bench] cpu reference FTLM / CF (t-J 4x4): {out['cpu_baseline']['ftlm_cf_tj4x4_reference']}")
     print(json.dumps(out), flush=True)
 
 
