@@ -597,6 +597,26 @@ def main():
                      "algorithmic_bytes_per_launch": int(alg_bytes)},
     }
 
+    if args.nvec == 1:
+        # diagnostics (not the bar): off-diagonal entries per row from 4 x 8192
+        # sampled rows (the device's own row listing), the gather-inclusive
+        # bytes 8 B x (2 + entries) per row, and branch evaluations per second
+        # (every row tests every off-diagonal branch)
+        import numpy as np
+        n_off, n_rows = 0, 0
+        for f in (0.11, 0.37, 0.61, 0.89):
+            r0 = min(int(f * D), max(D - 8192, 0))
+            for _, v in H.row_entries(r0, min(8192, D - r0)):
+                n_off += int(np.count_nonzero(v[1:]))   # the row's diagonal entry comes first
+                n_rows += 1
+        nbar = n_off / max(n_rows, 1)
+        out["diagnostics"] = {
+            "offdiag_entries_per_row": round(nbar, 3),
+            "gather_inclusive_bytes_per_launch": int(8 * D * (2 + nbar)),
+            "gather_inclusive_GBs": round(8 * D * (2 + nbar) / (ms * 1e-3) / 1e9, 1),
+            "branch_evaluations_per_s": round(D * H.info["n_offdiag_branches"] / (ms * 1e-3), 1),
+        }
+        log(f"[bench] diagnostics: {out['diagnostics']}")
     if ms_split is not None:
         out["apply_split_two_pass"] = {"ms": round(ms_split, 4),
                                        "GBs_alg": round(alg_bytes / (ms_split * 1e-3) / 1e9, 2),
