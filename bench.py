@@ -277,8 +277,8 @@ def lanczos_c1_leg():
         rep = r.report
         res[name] = {"iterations": r.iterations, "converged": bool(r.converged), "energy": r.energy,
                      "rel_err_vs_bethe": abs(r.energy - e_bethe) / abs(e_bethe),
-                     "loop_seconds": round(rep["loop_seconds"], 5),
-                     "iters_per_s": round(rep["steps_run"] / rep["loop_seconds"], 1)}
+                     "loop_seconds": round(rep["loop_seconds"], 5), "steps_run": rep["steps_run"],
+                     "iters_per_s": round(r.iterations / rep["loop_seconds"], 1)}
         log(f"[bench-c1] {name}: {r.iterations} it, E={r.energy:.12f} (Bethe {e_bethe:.12f}), "
             f"{res[name]['iters_per_s']} it/s")
     return res

@@ -163,8 +163,9 @@ typedef struct {
     int want_vector;
     uint64_t seed;        /* start vector uniform[-1,1) from (seed, index) */
     int check_every;      /* 0: automatic */
-    int graph_mode;       /* 0: automatic (replay a captured six-step graph below
-                           * 2^22 rows when no basis is stored), 1: always, 2: never */
+    int graph_mode;       /* 0: automatic (replay a captured six-step graph for
+                           * fixed-step runs below 2^22 rows when no basis is
+                           * stored), 1: always, 2: never */
     int split_mode;       /* two-pass apply over a relabelled basis (real spin-1/2
                            * models whose terms admit it): 1 use it when admitted;
                            * 0 (automatic) and 2 the single-pass apply, which

@@ -590,8 +590,9 @@ def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return
                          fixed_steps=False, start=None, check_every=0, graph=None, split=None):
     """Ground state energy and, as in bindings/python/qkrylov/solvers.py:24-51,
     the eigenvector (complex, `H.dimension` entries; return_state=False
-    skips it). graph: None automatic (replay a captured step graph below
-    2^22 rows when no Krylov basis is stored), True always, False never.
+    skips it). graph: None automatic (replay a captured step graph for
+    fixed-step runs below 2^22 rows when no Krylov basis is stored), True
+    always, False never.
     split: True runs the two-pass apply over a relabelled basis (real
     spin-1/2 models that admit it); None (automatic) and False the
     single-pass apply; report["split"] tells which ran."""
