@@ -425,6 +425,32 @@ def test_lanczos_graph_replay_equals_eager(check_every):
             assert a.energy == r.energy and a.iterations == r.iterations
 
 
+def test_lanczos_run_ahead_changes_no_number():
+    """Runs to a tolerance keep the device one convergence check ahead of
+    the host: whatever the check interval, the same alpha/beta, energy and
+    iteration count (the reference's convergence rule replayed on the host),
+    and the oracle's energy and first coefficients on the same start vector;
+    the device runs at most two check intervals past the last iteration."""
+    L = 14
+    terms = models.heisenberg_chain(L)
+    H = qa.MatrixFreeHamiltonian(qa.SpinHalfBasis(L, sz2=0), qa.SpinHalfSite(), qa.OpSum(terms))
+    om = O.Model(O.SPIN, L, O.sector(use_sz=1, sz2=0), O.SPIN, terms)
+    ref = om.lanczos(200, 1e-12, start=O.start_vector(om.dim, 1234) + 0j, reorth=False)
+    runs = {}
+    for ce in (1, 3, 7, 12):
+        r = qa.lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=False, check_every=ce,
+                                    return_state=False)
+        assert r.iterations <= r.report["steps_run"] <= min(200, r.iterations + 2 * ce)
+        runs[ce] = r
+    a = runs[1]
+    for r in runs.values():
+        assert np.array_equal(a.alphas, r.alphas) and np.array_equal(a.betas, r.betas)
+        assert a.energy == r.energy and a.iterations == r.iterations
+    assert abs(a.energy - ref["energy"]) < 1e-10 * abs(ref["energy"])
+    n = min(20, len(ref["alphas"]))
+    assert np.allclose(a.alphas[:n], ref["alphas"][:n], rtol=1e-11, atol=1e-11)
+
+
 @pytest.mark.parametrize("kind,n,sec,terms,splits", [
     (O.SPIN, 20, dict(use_sz=1, sz2=0), models.j1j2_square(5, 4, 1.0, 0.5), (15, 16, 17, 18, 20)),
     (O.SPIN, 18, dict(use_sz=1, sz2=4), models.xxz_chain(18, 0.7), (16, 18)),
