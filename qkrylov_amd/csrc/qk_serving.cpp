@@ -53,10 +53,11 @@ namespace {
 // two and four columns and 0.83x at eight (L = 28), so four is the default.
 constexpr int kDefaultBatch = 4, kMaxBatch = 8;
 
+// bonds (i, i + 1 mod L); a two-site ring has its one bond once
 OpSum xxz_ring(int L, double delta)
 {
     OpSum o;
-    for (int i = 0; i < L; ++i) {
+    for (int i = 0; i < (L == 2 ? 1 : L); ++i) {
         const int j = (i + 1) % L;
         o.terms.push_back({delta, {{"Sz", i}, {"Sz", j}}});
         o.terms.push_back({0.5, {{"Sp", i}, {"Sm", j}}});

@@ -165,6 +165,33 @@ def test_large_rings_vs_bethe_ansatz():
         assert abs(o["energy"] - e) < 1e-10 * abs(e), (r, o["energy"], e)
 
 
+def test_tiny_sectors_and_edge_requests():
+    """Sectors of one and two states, a fraction of zero steps, a
+    tolerance the first steps already meet, requests of one sector with
+    different caps batched together: energies against dense diagonalisation,
+    the same bits as each request alone."""
+    reqs = [dict(L=4, sz2=4, delta=0.8, kind=1, arrival=0.0, tol=1e-10, cf_steps=5, maxiter=50),   # D = 1
+            dict(L=2, sz2=0, delta=1.2, kind=1, arrival=0.0, tol=1e-10, cf_steps=0, maxiter=50),   # D = 2
+            dict(L=2, sz2=0, delta=0.6, kind=0, arrival=0.0, tol=1e-10, cf_steps=5, maxiter=1),
+            dict(L=6, sz2=2, delta=1.0, kind=1, arrival=0.0, tol=1e-3, cf_steps=7, maxiter=3),
+            dict(L=6, sz2=2, delta=1.4, kind=0, arrival=0.0, tol=1e-12, cf_steps=7, maxiter=200)]
+    out, _ = serving.run(reqs, realtime=False, max_active=4, max_batch=8)
+    assert [o["batch"] for o in out] == [1, 2, 2, 2, 2]
+    for r, o in zip(reqs, out):
+        sec = O.sector(use_sz=1, sz2=r["sz2"])
+        om = O.Model(O.SPIN, r["L"], sec, O.SPIN, models.xxz_chain(r["L"], r["delta"]))
+        w = np.linalg.eigvalsh(om.dense())
+        if r["maxiter"] >= om.dim and r["tol"] < 1e-6:
+            assert abs(o["energy"] - w[0]) < 1e-10 * max(1.0, abs(w[0])), (r, o["energy"], w[0])
+        else:
+            assert o["energy"] >= w[0] - 1e-12 * max(1.0, abs(w[0]))   # a Ritz value lies above E0
+        assert o["iterations"] <= min(r["maxiter"], om.dim)
+        if r["kind"] == 1:
+            assert 0 <= o["n_cf"] <= r["cf_steps"]
+        alone, _ = serving.run([r], realtime=False, max_active=1)
+        same_bits(alone[0], o)
+
+
 def test_admission_respects_budget():
     reqs = serving.make_trace(6, rate=1000.0, L_range=(16, 16), cf_fraction=0.0, maxiter=100, seed=9)
     d = serving.xxz_dim(16, 0)
