@@ -45,6 +45,15 @@ namespace qkrylov {
 // Execution-space tag of the engine: HIP on the current device.
 struct HipSpace {};
 
+// Start vector of lanczos_ground_state when its options leave it to the
+// process (start_mode -1, and the (H, maxiter, tol) form): 0 (default) the
+// engine's counter RNG; 1 / 2 the reference's std::mt19937(seed) complex
+// uniform [-1, 1) draws (src/solvers/lanczos.cpp:115-119), imaginary part
+// drawn first (1, GCC's order) or real part first (2, Clang's), run on
+// complex vectors. Throws std::invalid_argument for other values.
+void set_default_start_mode(int mode);
+int default_start_mode();
+
 }  // namespace qkrylov
 
 // Code written against the reference names its execution space with Kokkos
@@ -347,6 +356,7 @@ struct LanczosOptions {
     bool reorthogonalize = true;    // reference behaviour (lanczos.cpp:149-151)
     bool want_eigenvector = true;   // the reference always builds the Ritz vector
     uint64_t seed = 1234;
+    int start_mode = -1;            // 0 counter RNG, 1 / 2 the reference's draws, -1 default_start_mode()
 };
 
 LanczosResult lanczos_ground_state(const MatrixFreeHamiltonianBase& H, int maxiter = 200, Real tol = 1.0e-12);

@@ -170,6 +170,17 @@ typedef struct {
                            * models whose terms admit it): 1 use it when admitted;
                            * 0 (automatic) and 2 the single-pass apply, which
                            * measured faster on the J1-J2 4x8 and 32-site ring */
+    int start_mode;       /* 0: `start`, else uniform[-1,1) from (seed, index),
+                           * real for real H. 1 / 2: the reference's start
+                           * (src/solvers/lanczos.cpp:115-119): complex entries
+                           * of two std::mt19937(seed) uniform[-1,1) draws, the
+                           * imaginary part drawn first (1: GCC's evaluation of
+                           * its KComplex(dist(rng), dist(rng))) or the real
+                           * part first (2: Clang's); the recurrence runs on
+                           * complex vectors for real H too, a given `start` is
+                           * interleaved complex and `vector` receives 2*dim
+                           * doubles. -1: the process default
+                           * (qkrylov_amd_set_start_mode) */
 } qkrylov_lanczos_options_t;
 
 typedef struct {
@@ -181,11 +192,26 @@ typedef struct {
     double loop_seconds;
     double ritz_seconds;
     int split;            /* 1: the two-pass apply ran */
+    int vector_complex;   /* 1: `vector` holds 2*dim doubles (interleaved complex) */
 } qkrylov_lanczos_report_t;
+
+/* Start vector of the solvers that take no options: the reference-named
+ * Lanczos calls (qkrylov_lanczos_ground_state*, C++ lanczos_ground_state,
+ * Python lanczos_ground_state) and start_mode -1 above. mode 0 (default):
+ * counter RNG; 1 / 2: the reference's std::mt19937(1234) complex draws as
+ * described for start_mode, so that a drop-in run follows the reference's
+ * Krylov trajectory (iteration count, and the Ritz vector up to a sign).
+ * Process-wide; returns -1 for other values. */
+int qkrylov_amd_set_start_mode(int mode);
+int qkrylov_amd_get_start_mode(void);
+/* the reference's start vector itself: n complex entries interleaved into
+ * out[2n] (host memory; no device needed), mode 1 or 2 as above */
+int qkrylov_amd_reference_start_vector(size_t n, uint64_t seed, int mode, double* out);
 
 /* start: optional host start vector (real for real H, interleaved complex
  * otherwise); alphas/betas: host arrays of maxiter entries (optional);
- * vector: host output (dim doubles for real H, 2*dim otherwise; optional) */
+ * vector: host output (dim doubles for real H with start_mode 0, 2*dim
+ * otherwise; optional) */
 int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_options_t* opt, const double* start,
                             double* alphas, double* betas, double* vector, qkrylov_lanczos_report_t* report);
 

@@ -48,13 +48,13 @@ class HamInfo(ctypes.Structure):
 class LanczosOptions(ctypes.Structure):
     _fields_ = [("maxiter", c_int), ("tol", c_double), ("reorthogonalize", c_int), ("fixed_steps", c_int),
                 ("want_vector", c_int), ("seed", c_u64), ("check_every", c_int), ("graph_mode", c_int),
-                ("split_mode", c_int)]
+                ("split_mode", c_int), ("start_mode", c_int)]
 
 
 class LanczosReport(ctypes.Structure):
     _fields_ = [("energy", c_double), ("iterations", c_int), ("converged", c_int), ("steps_run", c_int),
                 ("n_alpha", c_int), ("n_beta", c_int), ("loop_seconds", c_double), ("ritz_seconds", c_double),
-                ("split", c_int)]
+                ("split", c_int), ("vector_complex", c_int)]
 
 
 class ServeRequest(ctypes.Structure):
@@ -154,6 +154,9 @@ SIGNATURES = {
     "qkrylov_amd_synchronize": (c_int, []),
     "qkrylov_amd_fill_random": (c_int, [c_vp, c_size, c_u64, c_int]),
     "qkrylov_amd_copy": (c_int, [c_vp, c_vp, c_size]),
+    "qkrylov_amd_set_start_mode": (c_int, [c_int]),
+    "qkrylov_amd_get_start_mode": (c_int, []),
+    "qkrylov_amd_reference_start_vector": (c_int, [c_size, c_u64, c_int, p_dbl]),
     "qkrylov_hamiltonian_create_ex": (c_vp, [c_vp, c_vp, c_vp, c_int]),
     "qkrylov_operator_create": (c_vp, [c_vp, c_vp, c_vp]),
     "qkrylov_hamiltonian_info": (c_int, [c_vp, P(HamInfo)]),

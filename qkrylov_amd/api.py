@@ -586,8 +586,49 @@ class LanczosResult:
         return f"LanczosResult(energy={self.energy:.12f}, iterations={self.iterations}, converged={self.converged})"
 
 
+_START_MODES = {"counter": 0, "reference": 1, "reference-gcc": 1, "reference-clang": 2}
+
+
+def _start_mode_code(mode):
+    if isinstance(mode, str):
+        if mode not in _START_MODES:
+            raise ValueError(f"start mode {mode!r}: expected one of {sorted(_START_MODES)}")
+        return _START_MODES[mode]
+    if mode not in (0, 1, 2):
+        raise ValueError(f"start mode {mode!r}: expected 0, 1 or 2")
+    return int(mode)
+
+
+def set_start_mode(mode):
+    """Start vector of lanczos_ground_state when its call names none
+    (start_mode=None), process-wide, and of the C ABI's reference-named
+    Lanczos calls: "counter" (default) draws uniform [-1, 1) from the
+    engine's counter RNG; "reference" (= "reference-gcc") the reference's
+    std::mt19937(seed) complex draws (src/solvers/lanczos.cpp:115-119), the
+    imaginary part of each entry drawn first as GCC evaluates its
+    KComplex(dist(rng), dist(rng)); "reference-clang" the real part first."""
+    check(_lib.load().qkrylov_amd_set_start_mode(_start_mode_code(mode)), "set_start_mode")
+
+
+def get_start_mode():
+    code = int(_lib.load().qkrylov_amd_get_start_mode())
+    return {0: "counter", 1: "reference", 2: "reference-clang"}[code]
+
+
+def reference_start_vector(n, seed=1234, mode="reference"):
+    """The reference's Lanczos start vector of n entries (complex128), as
+    start mode `mode` draws it; computed on the host."""
+    code = _start_mode_code(mode)
+    if code == 0:
+        raise ValueError("reference_start_vector: mode must be 'reference' or 'reference-clang'")
+    out = np.zeros(2 * int(n))
+    check(_lib.load().qkrylov_amd_reference_start_vector(int(n), int(seed), code, _dp(out)),
+          "reference_start_vector")
+    return out.view(np.complex128)
+
+
 def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return_state=True, seed=1234,
-                         fixed_steps=False, start=None, check_every=0, graph=None, split=None):
+                         fixed_steps=False, start=None, check_every=0, graph=None, split=None, start_mode=None):
     """Ground state energy and, as in bindings/python/qkrylov/solvers.py:24-51,
     the eigenvector (complex, `H.dimension` entries; return_state=False
     skips it). graph: None automatic (replay a captured step graph for
@@ -595,24 +636,29 @@ def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return
     always, False never.
     split: True runs the two-pass apply over a relabelled basis (real
     spin-1/2 models that admit it); None (automatic) and False the
-    single-pass apply; report["split"] tells which ran."""
+    single-pass apply; report["split"] tells which ran.
+    start_mode: None the process default (set_start_mode), else as there;
+    the reference modes run complex vectors for real H too, and a given
+    `start` is then read as complex."""
     L = _lib.load()
     gm = 0 if graph is None else (1 if graph else 2)
     sm = 0 if split is None else (1 if split else 2)
+    stm = -1 if start_mode is None else _start_mode_code(start_mode)
     o = _lib.LanczosOptions(int(maxiter), float(tol), 1 if reorthogonalize else 0, 1 if fixed_steps else 0,
-                            1 if return_state else 0, int(seed), int(check_every), gm, sm)
+                            1 if return_state else 0, int(seed), int(check_every), gm, sm, stm)
     al = np.zeros(max(maxiter, 1))
     be = np.zeros(max(maxiter, 1))
-    n = H.dimension if H.is_real else 2 * H.dimension
-    vec = np.zeros(n) if return_state else None
+    # room for a complex vector in every case: the process default may pick one
+    vec = np.zeros(2 * H.dimension) if return_state else None
+    cplx_start = (not H.is_real) or (stm if stm >= 0 else int(L.qkrylov_amd_get_start_mode())) != 0
     st = None
     if start is not None:
-        st = np.ascontiguousarray(start, dtype=np.float64 if H.is_real else np.complex128).view(np.float64)
+        st = np.ascontiguousarray(start, dtype=np.complex128 if cplx_start else np.float64).view(np.float64)
     rep = _lib.LanczosReport()
     check(L.qkrylov_lanczos_run_f64(H._h, ctypes.byref(o), _dp(st) if st is not None else None, _dp(al), _dp(be),
                                      _dp(vec) if vec is not None else None, ctypes.byref(rep)), "lanczos")
     if vec is not None:
-        vec = vec.view(np.complex128) if not H.is_real else vec
+        vec = vec.view(np.complex128) if rep.vector_complex else vec[:H.dimension]
         vec = vec.astype(_vec_dtype(H.dtype), copy=False)
     return LanczosResult(rep.energy, rep.iterations, bool(rep.converged), vec, al[:rep.n_alpha].copy(),
                          be[:rep.n_beta].copy(), {f: getattr(rep, f) for f, _ in rep._fields_})

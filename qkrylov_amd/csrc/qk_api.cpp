@@ -7,10 +7,24 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <random>
 #include <stdexcept>
 
 namespace qkrylov {
+
+namespace {
+std::atomic<int> g_start_mode{0};
+}
+
+void set_default_start_mode(int mode)
+{
+    if (mode < 0 || mode > 2) throw std::invalid_argument("start mode must be 0, 1 or 2");
+    g_start_mode.store(mode);
+}
+
+int default_start_mode() { return g_start_mode.load(); }
+
 namespace fp64 {
 
 using qk::DevBuf;
@@ -208,6 +222,7 @@ LanczosResult lanczos_ground_state(const MatrixFreeHamiltonianBase& H, const Lan
     o.reorth = opt.reorthogonalize;
     o.want_vector = opt.want_eigenvector;
     o.seed = opt.seed;
+    o.start_mode = opt.start_mode < 0 ? default_start_mode() : opt.start_mode;
     auto r = qk::lanczos_run(*H.engine(), o, qk::default_stream());
     LanczosResult res;
     res.energy = r.energy;
@@ -216,7 +231,7 @@ LanczosResult lanczos_ground_state(const MatrixFreeHamiltonianBase& H, const Lan
     if (opt.want_eigenvector) {
         const size_t D = H.dimension();
         res.eigenvector.resize(D);
-        if (H.is_real())
+        if (H.is_real() && r.vector.size() == D)
             for (size_t i = 0; i < D && i < r.vector.size(); ++i) res.eigenvector[i] = r.vector[i];
         else
             for (size_t i = 0; i < D && 2 * i + 1 < r.vector.size(); ++i)

@@ -565,6 +565,28 @@ extern "C" {
 
 
 int qkrylov_amd_device_count(void) { return Device::gpu_count(); }
+int qkrylov_amd_set_start_mode(int mode)
+{
+    if (mode < 0 || mode > 2) {
+        g_last_error = "start mode must be 0, 1 or 2";
+        return QKRYLOV_ERROR_INVALID_ARG;
+    }
+    qkrylov::set_default_start_mode(mode);
+    return QKRYLOV_SUCCESS;
+}
+int qkrylov_amd_get_start_mode(void) { return qkrylov::default_start_mode(); }
+int qkrylov_amd_reference_start_vector(size_t n, uint64_t seed, int mode, double* out)
+{
+    if ((n > 0 && !out) || (mode != 1 && mode != 2)) {
+        g_last_error = "reference start vector: mode must be 1 or 2 and out non-null";
+        return QKRYLOV_ERROR_INVALID_ARG;
+    }
+    QK_TRY
+    qk::reference_start_vector(n, seed, mode, out);
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
 int qkrylov_amd_set_device(int id)
 {
     QK_TRY
@@ -896,9 +918,12 @@ int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_optio
     o.graph = opt->graph_mode == 0 ? -1 : (opt->graph_mode == 1 ? 1 : 0);
     if (opt->split_mode < 0 || opt->split_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
     o.split = opt->split_mode == 0 ? -1 : (opt->split_mode == 1 ? 1 : 0);
+    if (opt->start_mode < -1 || opt->start_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    o.start_mode = opt->start_mode < 0 ? qkrylov::default_start_mode() : opt->start_mode;
     o.start = start;
     auto r = qk::lanczos_run(*H->engine(), o, qk::default_stream());
     rep->split = r.split ? 1 : 0;
+    rep->vector_complex = (!H->is_real() || o.start_mode != 0) ? 1 : 0;
     rep->energy = r.energy;
     rep->iterations = r.iterations;
     rep->converged = r.converged ? 1 : 0;
@@ -931,6 +956,10 @@ int qkrylov_lanczos_run_noalloc_f64(qkrylov_hamiltonian_h h, const qkrylov_lancz
     o.seed = opt->seed;
     o.check_every = opt->check_every;
     o.graph = 0;   // graph instantiation may allocate
+    if (opt->start_mode < -1 || opt->start_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    o.start_mode = opt->start_mode < 0 ? qkrylov::default_start_mode() : opt->start_mode;
+    if (o.start_mode != 0)
+        throw std::invalid_argument("the no-allocation Lanczos driver runs real vectors only (start_mode 0)");
     o.start = start;
     int dev = 0;
     qk::hip_check(hipGetDevice(&dev), "hipGetDevice");
@@ -950,6 +979,8 @@ int qkrylov_lanczos_run_noalloc_f64(qkrylov_hamiltonian_h h, const qkrylov_lancz
     arena.base = static_cast<char*>(base);
     arena.cap = cap;
     auto r = qk::lanczos_run(*H->engine(), o, qk::default_stream(), &arena);
+    rep->split = 0;
+    rep->vector_complex = 0;
     rep->energy = r.energy;
     rep->iterations = r.iterations;
     rep->converged = r.converged ? 1 : 0;

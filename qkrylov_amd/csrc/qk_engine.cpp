@@ -15,6 +15,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <random>
 #include <stdexcept>
 
 namespace qk {
@@ -790,6 +791,22 @@ void RealCore::step(int k, const double* basis)
 
 static LanczosOutput lanczos_complex(const Engine& E, const LanczosOptions& o, hipStream_t s);
 
+void reference_start_vector(size_t D, uint64_t seed, int start_mode, double* out)
+{
+    if (start_mode != 1 && start_mode != 2) throw std::invalid_argument("reference start: start_mode must be 1 or 2");
+    // the library's own engine and distribution, as the reference declares
+    // them; the two draws of an entry go to (imag, real) or (real, imag)
+    std::mt19937 rng(static_cast<std::mt19937::result_type>(seed));
+    std::uniform_real_distribution<double> dist(-1.0, 1.0);
+    const size_t first = start_mode == 1 ? 1 : 0;
+    for (size_t i = 0; i < D; ++i) {
+        const double a = dist(rng);
+        const double b = dist(rng);
+        out[2 * i + first] = a;
+        out[2 * i + (1 - first)] = b;
+    }
+}
+
 size_t lanczos_arena_bytes(const Engine& E, const LanczosOptions& o)
 {
     const size_t D = (size_t)E.dim();
@@ -837,6 +854,20 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
     LanczosOutput out;
     const int64_t D = E.dim();
     if (D == 0) return out;
+    if (o.start_mode < 0 || o.start_mode > 2) throw std::invalid_argument("start_mode must be 0, 1 or 2");
+    if (o.start_mode != 0) {
+        // the reference's complex start: complex vectors for real H as well
+        if (arena) throw std::invalid_argument("the workspace Lanczos driver runs real vectors only (start_mode 0)");
+        if (o.maxiter <= 0) return out;
+        std::vector<double> st;
+        LanczosOptions oc = o;
+        if (!o.start && !o.start_dev) {
+            st.resize(2 * (size_t)D);
+            reference_start_vector((size_t)D, o.seed, o.start_mode, st.data());
+            oc.start = st.data();
+        }
+        return lanczos_complex(E, oc, s);
+    }
     if (!E.is_real()) {
         if (arena) throw std::invalid_argument("the workspace Lanczos driver needs a real Hamiltonian");
         return lanczos_complex(E, o, s);

@@ -274,7 +274,20 @@ struct LanczosOptions {
                                  // -1 (automatic) and 0 the single pass
     const double* start = nullptr;   // host start vector (real, or interleaved complex for complex H)
     const double* start_dev = nullptr;   // device start vector (same layout)
+    // 0: the start above, else uniform [-1, 1) from the counter RNG (real for
+    // real H). 1, 2: the reference's start (lanczos.cpp:115-119): complex
+    // entries of two std::mt19937(seed) uniform [-1, 1) draws each, the
+    // imaginary part drawn first (1, as GCC evaluates the reference's
+    // KComplex(dist(rng), dist(rng))) or the real part first (2, as Clang
+    // does); the recurrence then runs on complex vectors whatever H is, a
+    // given start is read as interleaved complex, and the Ritz vector is
+    // complex
+    int start_mode = 0;
 };
+
+// the reference's start vector of lanczos_ground_state (start_mode 1 or 2
+// above), D complex entries interleaved into out[2 D]
+void reference_start_vector(size_t D, uint64_t seed, int start_mode, double* out);
 
 struct LanczosOutput {
     double energy = 0.0;
