@@ -285,12 +285,19 @@ typedef struct {
     int batch;              /* requests in the batch this one ran in */
     int ritz_stored;        /* kind 1: the Ritz vector was summed from stored
                              * Krylov vectors (1) or from the recurrence run
-                             * again (0, when the budget could not hold them);
+                             * again (0, when the store could not hold them);
                              * the same numbers either way */
 } qkrylov_serve_result_t;
 
 typedef struct {
-    size_t hbm_budget_bytes;   /* admission budget (0: 80% of free memory) */
+    size_t hbm_budget_bytes;   /* admission budget for the batches' device
+                                * footprints (0: 80% of free memory). Stored
+                                * Krylov vectors of kind-1 requests use a
+                                * separate buffer: up to a quarter of the
+                                * budget, within the free memory the budget
+                                * leaves less 1 GiB and at most 24 GiB,
+                                * allocated by the first run on the device that
+                                * stores vectors and kept by the process */
     int max_active;            /* concurrent batches on the device */
     int realtime;              /* 1: honour arrival times; 0: all arrive at t=0 */
     double* const* gs_vectors; /* optional, nreq host pointers (each NULL or dim
@@ -302,7 +309,8 @@ typedef struct {
 } qkrylov_serve_options_t;
 
 /* Runs the trace. cf_alphas/cf_betas: nreq * cf_cap host arrays (optional).
- * totals: [0] wall seconds, [1] solves/s, [2] p50 latency, [3] p99 latency */
+ * totals: [0] wall seconds, [1] solves/s, [2] p50 latency, [3] p99 latency.
+ * Runs on one device are serialised (they share its vector store). */
 int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, const qkrylov_serve_options_t* opt,
                       qkrylov_serve_result_t* results, double* cf_alphas, double* cf_betas, int cf_cap,
                       double* totals);

@@ -639,15 +639,12 @@ ColsCore::ColsCore(const Engine& e, int nv_, int kcap_, const double* dscale, hi
     dsc = mk(nv * sizeof(double));
     zero = mk(nv * sizeof(double));
     one = mk(nv * sizeof(double));
-    hsc.assign(3 * (size_t)nv, 0.0);
-    for (int c = 0; c < nv; ++c) {
-        hsc[c] = dscale[c];
-        hsc[2 * (size_t)nv + c] = 1.0;
-    }
-    // stream-ordered copies from host memory that lives as long as the core
-    hip_check(hipMemcpyAsync(dsc.get(), hsc.data(), nv * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
-    hip_check(hipMemcpyAsync(zero.get(), hsc.data() + nv, nv * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
-    hip_check(hipMemcpyAsync(one.get(), hsc.data() + 2 * nv, nv * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    // the per-column scalars as kernel arguments: a copy from pageable host
+    // memory would make the host wait for the stream's hardware queue
+    std::vector<double> z(nv, 0.0), o(nv, 1.0);
+    lcheck(qkl_set_vals(dsc.d(), nv, dscale, s), "set diagonal scales");
+    lcheck(qkl_set_vals(zero.d(), nv, z.data(), s), "set zeros");
+    lcheck(qkl_set_vals(one.d(), nv, o.data(), s), "set ones");
 }
 
 void ColsCore::init_norm()

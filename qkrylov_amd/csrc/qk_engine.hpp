@@ -233,7 +233,6 @@ struct ColsCore {
     size_t D;
     int nv, kcap;
     DevBuf Q[3], alpha, nrm, partA, partB, dsc, zero, one;
-    std::vector<double> hsc;   // host source of dsc, zero, one (kept until the core goes)
     int npA, NB;
     int ip = 0, ic = 1, iw = 2;
     // dscale: nv host values; stream_ordered: buffers from hipMallocAsync on st

@@ -1493,6 +1493,17 @@ __global__ __launch_bounds__(64) void k_total_cols(const double* __restrict__ p,
     if (threadIdx.x == 0) out[(size_t)idx * ncol + c] = sqrt(t);
 }
 
+// up to eight doubles handed over as kernel arguments
+struct QkVals8 {
+    double v[8];
+};
+
+__global__ void k_set_vals(double* __restrict__ dst, int n, QkVals8 vals)
+{
+    const int i = (int)threadIdx.x;
+    if (i < n) dst[i] = vals.v[i];
+}
+
 __global__ void k_set_int(int* __restrict__ p, int v)
 {
     if (threadIdx.x == 0) *p = v;
@@ -2325,6 +2336,16 @@ int qkl_copy_strided(size_t nrows, int width, const double* src, size_t src_ld, 
     if (width <= 0 || (size_t)width > src_ld || (size_t)width > dst_ld) return -1;
     hipLaunchKernelGGL(k_copy_strided, dim3(grid_for(nrows * (size_t)width)), dim3(QK_TPB), 0, (hipStream_t)sv, nrows,
                        width, src, src_ld, dst, dst_ld);
+    return rc(hipGetLastError());
+}
+
+int qkl_set_vals(double* dst, int n, const double* vals, qkl_stream sv)
+{
+    if (n < 0 || n > 8 || (n > 0 && (!dst || !vals))) return -1;
+    if (n == 0) return 0;
+    QkVals8 v{};
+    for (int i = 0; i < n; ++i) v.v[i] = vals[i];
+    hipLaunchKernelGGL(k_set_vals, dim3(1), dim3(64), 0, (hipStream_t)sv, dst, n, v);
     return rc(hipGetLastError());
 }
 

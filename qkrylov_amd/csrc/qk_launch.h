@@ -119,6 +119,9 @@ int qkl_fill_random(int kind, size_t n_doubles, uint64_t seed, uint64_t counter0
 int qkl_fill_zero(size_t n_doubles, double* x, qkl_stream s);
 /* *p = v on the stream (a one-thread kernel: no host staging, graph-safe) */
 int qkl_set_int(int* p, int v, qkl_stream s);
+/* dst[i] = vals[i], i < n <= 8, on the stream; the values travel as kernel
+ * arguments, so the call never waits for a host-to-device copy */
+int qkl_set_vals(double* dst, int n, const double* vals, qkl_stream s);
 int qkl_copy(size_t n_doubles, const double* src, double* dst, qkl_stream s);
 /* dst[r * dst_ld + j] = src[r * src_ld + j], r < nrows, j < width <= both ld */
 int qkl_copy_strided(size_t nrows, int width, const double* src, size_t src_ld, double* dst, size_t dst_ld,

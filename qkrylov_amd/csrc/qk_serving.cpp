@@ -20,10 +20,14 @@
 //   CF    S^z_pi applied to the Ritz vectors, then the continued-fraction
 //         recurrence of those columns.
 // While the GS phase runs, the Krylov vectors of the S^zz columns are kept
-// (in chunks of steps, allocated as the run goes on while the memory budget
-// allows), and the Ritz vectors are then summed from them; when the budget
-// cannot hold them, the Ritz phase recomputes them instead. Both give the
-// same bits. A ground-state request is finished when the GS phase ends. Every kernel
+// (in chunks of steps, taken as the run goes on from a store reserved beside
+// the budget), and the Ritz vectors are then summed from them; when the
+// store cannot hold them, the Ritz phase recomputes them instead. Both give
+// the same bits. Each phase enqueues a few milliseconds of steps per poll.
+// Nothing in the loop waits for the device: engines are built per sector
+// once per run with plain uploads, the store is carved on the host, the
+// per-column scalars are set by kernels, and every free is stream-ordered or
+// deferred to the end of the run. A ground-state request is finished when the GS phase ends. Every kernel
 // and reduction of a column has a fixed grid and order that depend neither
 // on the block width nor on the other columns, so a request's result is the
 // same bits alone (a batch of one) and in any batch.
@@ -39,6 +43,7 @@
 #include <list>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -73,6 +78,95 @@ OpSum szpi(int L)
     const double c = 1.0 / std::sqrt((double)L);
     for (int j = 0; j < L; ++j) o.terms.push_back({(j & 1) ? -c : c, {{"Sz", j}}});
     return o;
+}
+
+// Device memory for the stored Krylov vectors, reserved once per run. The
+// chunks are requested as a run goes on, in many sizes and up to gigabytes;
+// growing the stream-ordered pool by such blocks while batches run held the
+// host for seconds at a time (one measured call took 2.2 s), stalling every
+// batch it had to feed. Here a chunk is a first-fit range of one reserved
+// buffer, handed out and returned on the host without any HIP call; a
+// returned range is reused only after the batch that held it has passed an
+// event recorded after its last use.
+// The buffer is allocated by the first run on a device that stores vectors
+// and kept by the process (allocating and freeing tens of gigabytes per run
+// made later runs slower and slower); each run uses at most its own limit
+// of it. Runs on one device are serialised, since they share it.
+struct StoreArena {
+    char* base = nullptr;
+    size_t cap = 0;
+    std::map<size_t, size_t> free_;   // offset -> bytes, disjoint and merged
+    // start of a run: the first `limit` bytes (at most the buffer) are free
+    void reset(size_t limit)
+    {
+        free_.clear();
+        limit = std::min(limit, cap) & ~(size_t)255;
+        if (limit) free_[0] = limit;
+    }
+    bool take(size_t bytes, size_t& off)
+    {
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            off = it->first;
+            const size_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            return true;
+        }
+        return false;
+    }
+    void give(size_t off, size_t bytes)
+    {
+        bytes = (bytes + 255) & ~(size_t)255;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+constexpr size_t kStoreArenaMax = (size_t)24 << 30;
+
+struct DeviceArena {
+    std::mutex run_mu;   // one serving run at a time on the device
+    StoreArena arena;
+};
+
+DeviceArena& device_arena(int dev)
+{
+    static std::mutex mu;
+    static std::map<int, std::unique_ptr<DeviceArena>> all;
+    std::lock_guard<std::mutex> lk(mu);
+    auto& p = all[dev];
+    if (!p) p = std::make_unique<DeviceArena>();
+    return *p;
+}
+
+// the device's arena, allocated by the first call that can give it room
+// (at most kStoreArenaMax, within `room`); call with run_mu held
+StoreArena& store_arena(DeviceArena& da, size_t want)
+{
+    StoreArena& a = da.arena;
+    want = std::min(want, kStoreArenaMax) & ~(size_t)255;
+    if (!a.base && want) {
+        if (hipMalloc(reinterpret_cast<void**>(&a.base), want) == hipSuccess) {
+            a.cap = want;
+        } else {
+            (void)hipGetLastError();
+            a.base = nullptr;
+        }
+    }
+    return a;
 }
 
 // pinned host buffers for the alpha / norm read-backs, one pair per stream
@@ -123,7 +217,7 @@ size_t footprint_of(const qkrylov_serve_request_t* const* rq, int n)
     return vecs * D * sizeof(double) + (size_t)(kmax + 8) * 16 * nv + (2 << 20);
 }
 
-constexpr int kChunk = 12;   // recurrence steps enqueued per poll
+constexpr int kChunk = 12;   // recurrence steps per stored-vector chunk, and at most per poll
 
 enum Phase { kGS, kRitz, kCF, kDone };
 
@@ -145,18 +239,20 @@ struct Batch {
     hipEvent_t ev = nullptr;
     Pinned *ha = nullptr, *hn = nullptr;   // the slot's read-back buffers
     std::shared_ptr<BasisSpec> basis;
-    std::unique_ptr<Engine> H, Op;
+    std::shared_ptr<const Engine> H, Op;   // the sector's, shared (SectorEngines)
     std::unique_ptr<ColsCore> C;
     DevBuf ritz, coef, psi2;
     // stored Krylov vectors of the S^zz columns: step k in chunks[k / kChunk]
-    // at row-major width cols_width(ncf); `extra` bytes of them count against
-    // the budget; `store` is dropped (and the chunks freed) when the budget
-    // cannot take the next chunk
+    // (offset, bytes in the run's StoreArena) at row-major width
+    // cols_width(ncf); `store` is dropped when the arena cannot take the
+    // next chunk. Dropped chunks wait in `retired` for the batch's next
+    // event.
     bool store = false;
-    std::vector<DevBuf> chunks;
-    size_t extra = 0;
+    std::vector<std::pair<size_t, size_t>> chunks, retired;
+    StoreArena* arena = nullptr;
     Phase phase = kGS;
     int k = 0, kend = 0;             // steps enqueued; steps the phase runs at most
+    int rn = 0;                      // Ritz phase without stored vectors: terms of the sum
     bool inflight = false;
     size_t footprint = 0;
     uint64_t seed = 1234;
@@ -181,34 +277,51 @@ void copy_scalars(Batch& b, int nsteps)
               "d2h");
 }
 
-void drop_store(Batch& b, size_t& used)
+void drop_store(Batch& b)
 {
     b.store = false;
-    b.chunks.clear();   // stream-ordered release
-    used -= b.extra;
-    b.extra = 0;
+    b.retired.insert(b.retired.end(), b.chunks.begin(), b.chunks.end());
+    b.chunks.clear();
+}
+
+// the batch's retired chunks go back to the arena (call after an event the
+// batch recorded behind their last use has completed)
+void give_back(Batch& b)
+{
+    for (const auto& c : b.retired) b.arena->give(c.first, c.second);
+    b.retired.clear();
+}
+
+// Recurrence steps one poll enqueues: about 4 ms of device work, at most
+// kChunk. The stream slots share a few hardware queues that run their
+// packets in order, so a long run of one batch's steps holds back every
+// batch queued behind it; polls of similar length keep the slots sharing
+// the device evenly. The step cost is modelled from the ring's column apply
+// (about 1.6e-8 ms per row and column on an MI355X).
+int steps_per_poll(const Batch& b, int nv)
+{
+    const double ms = 1.6e-8 * (double)b.basis->dim() * (double)nv;
+    if (ms <= 0.0) return kChunk;
+    return std::max(1, std::min(kChunk, (int)(4.0 / ms)));
 }
 
 double* stored_step(Batch& b, int k)
 {
     const size_t D = (size_t)b.basis->dim(), w = (size_t)cols_width((int)b.cf.size());
-    return b.chunks[(size_t)k / kChunk].d() + (size_t)(k % kChunk) * D * w;
+    return reinterpret_cast<double*>(b.arena->base + b.chunks[(size_t)k / kChunk].first) + (size_t)(k % kChunk) * D * w;
 }
 
-void enqueue_gs_chunk(Batch& b, size_t& used, size_t budget)
+void enqueue_gs_chunk(Batch& b)
 {
-    const int kend = std::min(b.kend, b.k + kChunk);
+    // never past the stored-vector chunk the first step of this poll is in
+    const int kend = std::min({b.kend, b.k + steps_per_poll(b, b.C->nv), (b.k / kChunk + 1) * kChunk});
     const size_t D = (size_t)b.basis->dim();
     const int ncf = (int)b.cf.size(), w = ncf ? cols_width(ncf) : 0;
     if (b.store && (size_t)b.k / kChunk >= b.chunks.size()) {
         const size_t bytes = (size_t)kChunk * D * w * sizeof(double);
-        if (used + bytes <= budget) {
-            b.chunks.emplace_back(bytes, b.s);
-            used += bytes;
-            b.extra += bytes;
-        } else {
-            drop_store(b, used);
-        }
+        size_t off = 0;
+        if (b.arena && b.arena->take(bytes, off)) b.chunks.emplace_back(off, bytes);
+        else drop_store(b);
     }
     for (; b.k < kend; ++b.k) {
         // q_k of the S^zz columns (the block's first ncf columns) before step k
@@ -221,22 +334,54 @@ void enqueue_gs_chunk(Batch& b, size_t& used, size_t budget)
     b.inflight = true;
 }
 
-void start_batch(Batch& b, size_t& used, size_t budget)
+// The engines of one (L, Sz) sector, shared by every batch of the sector
+// for the whole run: H (the exchange part and the Ising diagonal at
+// Delta = 1; column c scales the diagonal by its own Delta) and the S^z_pi
+// operator of the continued fraction. They are built with plain (not
+// stream-ordered) uploads: a copy ordered on a batch's stream would make the
+// host wait for that stream's hardware queue, which the other batches' work
+// shares, and the run keeps them to its end, so that no hipFree (which
+// waits for the whole device) happens while requests are in flight.
+struct SectorEngines {
+    std::shared_ptr<BasisSpec> basis;
+    std::shared_ptr<const Engine> H, Op;
+};
+using EngineCache = std::map<std::pair<int, int>, SectorEngines>;
+
+const SectorEngines& sector_engines(EngineCache& cache, int L, int sz2, bool want_op)
 {
-    Sector sec;
-    sec.use_sz = true;
-    sec.sz2 = b.sz2;
-    b.basis = std::make_shared<BasisSpec>(Kind::SpinHalf, b.L, sec);
-    EngineOptions eo;
-    eo.stream = b.s;
-    eo.NL = choose_NL(*b.basis);   // the split whose tables the run keeps resident
-    // the exchange part and the Ising diagonal at Delta = 1; column c scales
-    // the diagonal by its own Delta
-    b.H = std::make_unique<Engine>(b.basis, Kind::SpinHalf, xxz_ring(b.L, 1.0), eo);
-    // one front of XCD-contiguous tile chunks: 2-6 % faster column applies
-    // than one segment per XCD for rings of L = 24-28 (launch shapes never
-    // change results)
-    b.H->set_tuning(0, 2);
+    SectorEngines& e = cache[{L, sz2}];
+    if (!e.H) {
+        Sector sec;
+        sec.use_sz = true;
+        sec.sz2 = sz2;
+        e.basis = std::make_shared<BasisSpec>(Kind::SpinHalf, L, sec);
+        EngineOptions eo;
+        eo.NL = choose_NL(*e.basis);   // the split whose tables the run keeps resident
+        auto H = std::make_shared<Engine>(e.basis, Kind::SpinHalf, xxz_ring(L, 1.0), eo);
+        // one front of XCD-contiguous tile chunks: 2-6 % faster column
+        // applies than one segment per XCD for rings of L = 24-28 (launch
+        // shapes never change results)
+        H->set_tuning(0, 2);
+        e.H = H;
+    }
+    if (want_op && !e.Op) {
+        EngineOptions eo;
+        eo.NL = e.H->NL();
+        e.Op = std::make_shared<Engine>(e.basis, Kind::SpinHalf, szpi(L), eo);
+    }
+    return e;
+}
+
+void start_batch(Batch& b, EngineCache& engines, StoreArena& arena)
+{
+    bool any_cf = false;
+    for (const auto& col : b.cols) any_cf = any_cf || col.rq.kind == 1;
+    const SectorEngines& se = sector_engines(engines, b.L, b.sz2, any_cf);
+    b.basis = se.basis;
+    b.H = se.H;
+    b.Op = se.Op;
+    b.arena = &arena;
     const int n = (int)b.cols.size(), nv = cols_width(n);
     std::vector<double> dsc(nv);
     b.kend = 0;
@@ -253,15 +398,17 @@ void start_batch(Batch& b, size_t& used, size_t budget)
     b.phase = kGS;
     b.k = 0;
     b.store = !b.cf.empty();
-    enqueue_gs_chunk(b, used, budget);
+    enqueue_gs_chunk(b);
 }
 
 void start_cf(Batch& b);
+void enqueue_ritz_chunk(Batch& b);
+void enqueue_cf_chunk(Batch& b);
 
 // the CF columns' Ritz vectors: each column adds z_k q_k / |q_k| for its own
 // k < n_gs, from the stored Krylov vectors or, without them, from the
 // recurrence run again from the same start vectors
-void start_ritz(Batch& b, size_t& used)
+void start_ritz(Batch& b)
 {
     const size_t D = (size_t)b.basis->dim();
     const int ncf = (int)b.cf.size(), nvr = cols_width(ncf);
@@ -293,21 +440,32 @@ void start_ritz(Batch& b, size_t& used)
         for (int kk = 0; kk < nmax; ++kk)
             if (qkl_axpy_cols(nvr, D, b.coef.d() + (size_t)kk * nvr, stored_step(b, kk), b.ritz.d(), b.s) != 0)
                 throw std::runtime_error("axpy");
-        drop_store(b, used);   // freed in stream order, after the sums that read them
+        drop_store(b);   // back to the arena after the next event, behind the sums that read them
         for (int j : b.cf) b.cols[j].ritz_stored = true;
         start_cf(b);
         return;
     }
     fill_start(b, nvr);
     b.C->init_norm();
-    for (int kk = 0; kk < nmax; ++kk) {
-        if (qkl_axpy_cols(nvr, D, b.coef.d() + (size_t)kk * nvr, b.C->cur(), b.ritz.d(), b.s) != 0)
+    b.rn = nmax;
+    b.k = 0;
+    b.phase = kRitz;
+    enqueue_ritz_chunk(b);
+}
+
+// terms [k, ...) of the recomputed Ritz sum: add term k, then step k
+void enqueue_ritz_chunk(Batch& b)
+{
+    const size_t D = (size_t)b.basis->dim();
+    const int nvr = b.C->nv;
+    const int kend = std::min(b.rn, b.k + steps_per_poll(b, nvr));
+    for (; b.k < kend; ++b.k) {
+        if (qkl_axpy_cols(nvr, D, b.coef.d() + (size_t)b.k * nvr, b.C->cur(), b.ritz.d(), b.s) != 0)
             throw std::runtime_error("axpy");
-        if (kk + 1 < nmax) b.C->step(kk);
+        if (b.k + 1 < b.rn) b.C->step(b.k);
     }
     hip_check(hipEventRecord(b.ev, b.s), "event");
     b.inflight = true;
-    b.phase = kRitz;
 }
 
 // phi_c = S^z_pi psi_c, |psi_c|^2, then cf_steps steps of the plain recurrence from phi
@@ -315,13 +473,6 @@ void start_cf(Batch& b)
 {
     const size_t D = (size_t)b.basis->dim();
     const int nvr = b.C->nv;
-    Sector sec;
-    sec.use_sz = true;
-    sec.sz2 = b.sz2;
-    EngineOptions eo;
-    eo.stream = b.s;
-    eo.NL = choose_NL(*b.basis);
-    b.Op = std::make_unique<Engine>(b.basis, Kind::SpinHalf, szpi(b.L), eo);
     b.Op->apply_cols(b.ritz.d(), b.C->cur(), nvr, b.C->one.d(), b.s);
     // an update with coefficient 0 / 1 leaves the Ritz block unchanged and
     // yields its column sums of squares
@@ -330,20 +481,31 @@ void start_cf(Batch& b)
         qkl_reduce(b.C->partB.d(), b.C->NB, nvr, 1, 0, 0, b.psi2.d(), 0, nvr, b.s) != 0)
         throw std::runtime_error("psi norms");
     b.C->init_norm();
-    for (int kk = 0; kk < b.kend; ++kk) b.C->step(kk);
-    copy_scalars(b, b.kend);
-    hip_check(hipMemcpyAsync(b.ha->p + (size_t)(b.kend + 1) * nvr, b.psi2.d(), nvr * sizeof(double),
-                             hipMemcpyDeviceToHost, b.s),
-              "d2h");
+    b.k = 0;
+    b.phase = kCF;
+    enqueue_cf_chunk(b);
+}
+
+// CF steps [k, ...); after the last one the read-back of the scalars and of
+// |psi|^2
+void enqueue_cf_chunk(Batch& b)
+{
+    const int nvr = b.C->nv;
+    const int kend = std::min(b.kend, b.k + steps_per_poll(b, nvr));
+    for (; b.k < kend; ++b.k) b.C->step(b.k);
+    if (b.k == b.kend) {
+        copy_scalars(b, b.kend);
+        hip_check(hipMemcpyAsync(b.ha->p + (size_t)(b.kend + 1) * nvr, b.psi2.d(), nvr * sizeof(double),
+                                 hipMemcpyDeviceToHost, b.s),
+                  "d2h");
+    }
     hip_check(hipEventRecord(b.ev, b.s), "event");
     b.inflight = true;
-    b.phase = kCF;
 }
 
 // called when the batch's last event completed: takes the results it
 // brings and enqueues the next phase; returns the requests it finished
-std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a, double* cf_b, int cf_cap,
-                         size_t& used, size_t budget)
+std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a, double* cf_b, int cf_cap)
 {
     std::vector<int> finished;
     const size_t D = (size_t)b.basis->dim();
@@ -374,18 +536,23 @@ std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a
             }
         }
         if (!all) {
-            enqueue_gs_chunk(b, used, budget);
+            enqueue_gs_chunk(b);
             return finished;
         }
         if (b.cf.empty()) {
             b.phase = kDone;
             return finished;
         }
-        start_ritz(b, used);
+        start_ritz(b);
         return finished;
     }
     if (b.phase == kRitz) {
-        start_cf(b);
+        if (b.k < b.rn) enqueue_ritz_chunk(b);
+        else start_cf(b);
+        return finished;
+    }
+    if (b.phase == kCF && b.k < b.kend) {
+        enqueue_cf_chunk(b);
         return finished;
     }
     if (b.phase == kCF) {
@@ -444,7 +611,15 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        // the budget admits batches by their footprint; beside it, up to a
+        // quarter of it (within the free memory the budget leaves, less a
+        // margin) is reserved for the S^zz requests' stored Krylov vectors
         const size_t budget = (opt && opt->hbm_budget_bytes) ? opt->hbm_budget_bytes : (size_t)(0.8 * free_b);
+        bool any_cf = false;
+        for (int i = 0; i < nreq; ++i) any_cf = any_cf || reqs[i].kind == 1;
+        const size_t margin = (size_t)1 << 30;
+        const size_t room = free_b > budget + margin ? free_b - budget - margin : 0;
+        const size_t store_bytes = any_cf ? std::min(budget / 4, room) : 0;
         const int max_active = (opt && opt->max_active > 0) ? opt->max_active : 32;
         const bool realtime = opt ? opt->realtime != 0 : true;
         const int max_batch = (opt && opt->max_batch > 0) ? std::min(opt->max_batch, kMaxBatch) : kDefaultBatch;
@@ -482,6 +657,14 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         // while its kernels may still run; the read-back buffers outlive
         // `release` below
         std::vector<std::unique_ptr<Pinned>> pin_a, pin_n;
+        // every sector's engines (and through them its basis tables), kept
+        // to the end of the run; declared before `release`, so they go after
+        // the streams are drained
+        EngineCache engines;
+        DeviceArena& darena = device_arena(dev);
+        std::lock_guard<std::mutex> run_lock(darena.run_mu);
+        StoreArena& arena = store_arena(darena, any_cf ? room : 0);
+        arena.reset(store_bytes);
         std::vector<hipStream_t> streams;
         std::vector<hipEvent_t> events;
         std::vector<std::unique_ptr<Batch>> active;
@@ -512,10 +695,15 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         }
         // the device's stream-ordered pool keeps what the run frees (Krylov
         // blocks, stored vectors) for the next batch instead of returning it
-        // at every synchronisation; restored when the run ends
+        // at every synchronisation, and may not hand a block freed on one
+        // stream to another stream by making the second wait for the first:
+        // such inserted waits chain unrelated batches together (stalls of
+        // seconds under load were measured). Restored when the run ends.
         struct PoolHold {
             hipMemPool_t pool = nullptr;
             uint64_t old = 0;
+            int old_dep = 1;
+            bool dep_set = false;
             explicit PoolHold(int dev)
             {
                 if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess ||
@@ -528,11 +716,19 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                 if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
                     pool = nullptr;
                     (void)hipGetLastError();
+                    return;
                 }
+                int no = 0;
+                if (hipMemPoolGetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &old_dep) == hipSuccess &&
+                    hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &no) == hipSuccess)
+                    dep_set = true;
+                else
+                    (void)hipGetLastError();
             }
             ~PoolHold()
             {
                 if (pool) {
+                    if (dep_set) (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &old_dep);
                     (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &old);
                     (void)hipMemPoolTrimTo(pool, 0);
                 }
@@ -613,7 +809,7 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                 for (auto t : take) pending.erase(t);
                 it = (take.size() == 1) ? next : pending.begin();
                 used += b->footprint;
-                start_batch(*b, used, budget);
+                start_batch(*b, engines, arena);
                 slot_of.push_back(slot);
                 active.push_back(std::move(b));
                 progress = true;
@@ -629,10 +825,13 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                     hip_check(q, "event query");
                     b.inflight = false;
                     progress = true;
-                    for (int id : advance(b, results, cf_a, cf_b, cf_cap, used, budget)) finish(id);
+                    give_back(b);
+                    for (int id : advance(b, results, cf_a, cf_b, cf_cap)) finish(id);
                 }
                 if (b.phase == kDone && !b.inflight) {
-                    used -= b.footprint + b.extra;
+                    used -= b.footprint;
+                    drop_store(b);
+                    give_back(b);
                     free_slots.push_back(slot_of[a]);
                     active.erase(active.begin() + (long)a);
                     slot_of.erase(slot_of.begin() + (long)a);
