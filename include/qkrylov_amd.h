@@ -295,7 +295,7 @@ typedef struct {
                                 * Krylov vectors of kind-1 requests use a
                                 * separate buffer: up to a quarter of the
                                 * budget, within the free memory the budget
-                                * leaves less 1 GiB and at most 24 GiB,
+                                * leaves less 1 GiB and at most 64 GiB,
                                 * allocated by the first run on the device that
                                 * stores vectors and kept by the process */
     int max_active;            /* concurrent batches on the device */

@@ -135,7 +135,7 @@ struct StoreArena {
     }
 };
 
-constexpr size_t kStoreArenaMax = (size_t)24 << 30;
+constexpr size_t kStoreArenaMax = (size_t)64 << 30;
 
 struct DeviceArena {
     std::mutex run_mu;   // one serving run at a time on the device
