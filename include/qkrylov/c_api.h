@@ -2,8 +2,8 @@
  *
  * Every declaration in the first part keeps the name, argument types and
  * meaning of the reference ABI (include/qkrylov/c_api.h:14-168 of
- * sjp95/qkrylov), so existing callers (the Julia ccall bindings,
- * bindings/julia/src/*.jl, and tests/test_c_api.cpp) link unchanged:
+ * sjp95/qkrylov), so existing callers (the Julia ccall bindings in
+ * bindings/julia/src, and tests/test_c_api.cpp) link unchanged:
  *   - opaque handles, caller-owned; *_destroy(NULL) is a no-op
  *   - 0 success / -1 invalid argument / -2 exception; creators return NULL
  *   - buffers are caller-allocated with sizes implied by dimension()

@@ -538,18 +538,6 @@ double tridiag_min(const double* alpha, const double* beta, int n, std::vector<d
 
 namespace {
 
-struct Scratch {
-    DevBuf partA, partB, scal;
-};
-
-double host_scalar(const double* dev, int idx, hipStream_t s)
-{
-    double v;
-    hip_check(hipMemcpyAsync(&v, dev + idx, sizeof(double), hipMemcpyDeviceToHost, s), "d2h scalar");
-    hip_check(hipStreamSynchronize(s), "sync");
-    return v;
-}
-
 // <x, y> (complex conj-linear when cplx) -> host
 std::complex<double> dot_host(bool cplx, size_t n, const double* x, const double* y, double* part, double* out,
                               hipStream_t s)
