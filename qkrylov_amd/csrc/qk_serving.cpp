@@ -27,10 +27,11 @@
 // Nothing in the loop waits for the device: engines are built per sector
 // once per run with plain uploads, the store is carved on the host, the
 // per-column scalars are set by kernels, and every free is stream-ordered or
-// deferred to the end of the run. A ground-state request is finished when the GS phase ends. Every kernel
-// and reduction of a column has a fixed grid and order that depend neither
-// on the block width nor on the other columns, so a request's result is the
-// same bits alone (a batch of one) and in any batch.
+// deferred to the end of the run. A ground-state request is finished when
+// the GS phase ends. Every kernel and reduction of a column has a fixed grid
+// and order that depend neither on the block width nor on the other columns,
+// so a request's result is the same bits alone (a batch of one) and in any
+// batch.
 #include "qkrylov_amd.h"
 
 #include "qk_engine.hpp"
