@@ -246,6 +246,34 @@ def serve_leg(args):
     }
 
 
+def bench_gather_probe(args):
+    """Diagnostic, not a reference metric: the rate at which waves issue
+    coalesced gathers of random 64-lane segments (8 or 16 B per lane) from
+    device arrays of 2 MB to 4.6 GB -- the instruction rate that bounds the
+    apply's gathers whatever the array size."""
+    import ctypes
+    from qkrylov_amd import _lib
+    L = _lib.load()
+    rows = []
+    for nbytes in (2 << 20, 16 << 20, 256 << 20, 4600 << 20):
+        for w in (8, 16):
+            ms, g = ctypes.c_double(), ctypes.c_double()
+            _lib.check(L.qkrylov_amd_gather_probe(nbytes, w, max(args.steps, 1), ctypes.byref(ms), ctypes.byref(g)),
+                       "gather probe")
+            rate = g.value / ms.value
+            rows.append({"array_bytes": nbytes, "bytes_per_lane": w, "ms_per_launch": round(ms.value, 4),
+                         "wave_gathers_per_launch": int(g.value), "wave_gathers_per_ms": round(rate, 0),
+                         "GB_per_s": round(rate * 64 * w / 1e6, 1)})
+            log(f"[bench] gather probe {nbytes >> 20} MiB, {w} B/lane: {ms.value:.4f} ms, "
+                f"{rate / 1e6:.2f} M wave gathers/ms, {rate * 64 * w / 1e6:.0f} GB/s")
+    best = max(r["wave_gathers_per_ms"] for r in rows)
+    print(json.dumps({"metric": "wave gathers per ms (diagnostic)", "value": best, "unit": "wave gathers/ms",
+                      "n_gpus": 1, "steps": args.steps, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f64", "data": "synthetic random segments",
+                      "config": {"workload": "gather_probe", "grid": "2048 x 512 threads, 64 rounds of 16 gathers"},
+                      "rows": rows}), flush=True)
+
+
 def bench_serve(args):
     o = serve_leg(args)
     res = {
@@ -441,7 +469,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="j1j2_4x8", choices=sorted(WORKLOADS) + ["ftlm_tj4x5", "serve_xxz"])
+    ap.add_argument("--workload", default="j1j2_4x8", choices=sorted(WORKLOADS) + ["ftlm_tj4x5", "serve_xxz", "gather_probe"])
     ap.add_argument("--split-bits", type=int, default=-1,
                     help="low state bits ranked per lane (-1: the engine's cost model)")
     ap.add_argument("--ftlm-R", type=int, default=64)
@@ -500,6 +528,8 @@ def main():
         return bench_ftlm(args)
     if args.workload == "serve_xxz":
         return bench_serve(args)
+    if args.workload == "gather_probe":
+        return bench_gather_probe(args)
     w = WORKLOADS[args.workload]
     if args.copy_calib_gib > 0:
         n = int(args.copy_calib_gib * (1 << 30) / 8)

@@ -31,6 +31,13 @@ int qkrylov_amd_fill_random(double* x_dev, size_t n, uint64_t seed, int kind);
 /* dst[i] = src[i] for n doubles with a plain streaming kernel (synchronous);
  * also the reference kernel for calibrating HBM byte counters */
 int qkrylov_amd_copy(double* dst_dev, const double* src_dev, size_t n);
+/* Measurement of the vector memory path, not part of any solver (no
+ * counterpart in the reference): wave-wide coalesced gathers of random
+ * 64-lane segments, 8 or 16 bytes per lane, from a device array of
+ * array_bytes; reports the mean time of one launch over `reps` launches and
+ * the wave gathers one launch issues. */
+int qkrylov_amd_gather_probe(size_t array_bytes, int bytes_per_lane, int reps, double* ms_per_launch,
+                             double* gathers_per_launch);
 
 /* ---- sector ---- */
 /* Fixes one species of a Hubbard / t-J sector alone (species 0: up

@@ -154,6 +154,7 @@ SIGNATURES = {
     "qkrylov_amd_synchronize": (c_int, []),
     "qkrylov_amd_fill_random": (c_int, [c_vp, c_size, c_u64, c_int]),
     "qkrylov_amd_copy": (c_int, [c_vp, c_vp, c_size]),
+    "qkrylov_amd_gather_probe": (c_int, [c_size, c_int, c_int, p_dbl, p_dbl]),
     "qkrylov_amd_set_start_mode": (c_int, [c_int]),
     "qkrylov_amd_get_start_mode": (c_int, []),
     "qkrylov_amd_reference_start_vector": (c_int, [c_size, c_u64, c_int, p_dbl]),

@@ -632,6 +632,45 @@ int qkrylov_amd_copy(double* dst, const double* src, size_t n)
     return QKRYLOV_SUCCESS;
     QK_CATCH
 }
+int qkrylov_amd_gather_probe(size_t array_bytes, int bytes_per_lane, int reps, double* ms_per_launch,
+                             double* gathers_per_launch)
+{
+    if (array_bytes < 2048 || (bytes_per_lane != 8 && bytes_per_lane != 16) || reps <= 0 || !ms_per_launch ||
+        !gathers_per_launch)
+        return QKRYLOV_ERROR_INVALID_ARG;
+    QK_TRY
+    hipStream_t s = qk::default_stream();
+    const size_t n = array_bytes / sizeof(double);
+    const int blocks = 2048, tiles = 64;
+    qk::DevBuf x(n * sizeof(double)), y((size_t)blocks * 512 * sizeof(double));
+    if (qkl_fill_zero(n, x.d(), s) != 0) throw std::runtime_error("fill launch failed");
+    hipEvent_t a = nullptr, b = nullptr;
+    qk::hip_check(hipEventCreate(&a), "event");
+    qk::hip_check(hipEventCreate(&b), "event");
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ev{a, b};
+    if (qkl_gather_probe(x.d(), n, bytes_per_lane, y.d(), blocks, tiles, s) != 0)
+        throw std::runtime_error("probe launch failed");
+    qk::hip_check(hipEventRecord(a, s), "event");
+    for (int i = 0; i < reps; ++i)
+        if (qkl_gather_probe(x.d(), n, bytes_per_lane, y.d(), blocks, tiles, s) != 0)
+            throw std::runtime_error("probe launch failed");
+    qk::hip_check(hipEventRecord(b, s), "event");
+    qk::hip_check(hipEventSynchronize(b), "event sync");
+    float ms = 0.0f;
+    qk::hip_check(hipEventElapsedTime(&ms, a, b), "elapsed");
+    *ms_per_launch = (double)ms / reps;
+    *gathers_per_launch = (double)blocks * 8 * tiles * 16;
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
 int qkrylov_amd_fill_random(double* x, size_t n, uint64_t seed, int kind)
 {
     if (!x || (kind != 0 && kind != 1)) return QKRYLOV_ERROR_INVALID_ARG;

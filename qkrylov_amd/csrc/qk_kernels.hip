@@ -566,6 +566,51 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
     }
 }
 
+// ------------------------------------------------------------------ gather-rate probe
+// The rate at which waves issue coalesced gathers of random 64-lane
+// segments (W bytes per lane) from an array of nseg segments: every wave
+// runs `tiles` rounds of 16 gathers, four in flight, as the apply's hi-only
+// gathers do. A measurement of the vector memory path, not part of any
+// solver.
+__device__ __forceinline__ uint64_t probe_mix(uint64_t v)
+{
+    v ^= v >> 33;
+    v *= 0xff51afd7ed558ccdULL;
+    v ^= v >> 33;
+    v *= 0xc4ceb9fe1a85ec53ULL;
+    v ^= v >> 33;
+    return v;
+}
+
+template <int W>
+__global__ __launch_bounds__(512) void k_gather_probe(const double* __restrict__ x, size_t nseg,
+                                                      double* __restrict__ y, uint32_t tiles)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    double acc = 0.0;
+    for (uint32_t t = 0; t < tiles; ++t) {
+        const uint64_t key = wave * tiles + t;
+#pragma unroll
+        for (int g0 = 0; g0 < 16; g0 += 4) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint64_t seg = probe_mix(key * 131 + (uint64_t)(g0 + u)) % nseg;
+                if (W == 8) {
+                    v[u] = x[seg * 64 + lane];
+                } else {
+                    const qk_double2 p2 = reinterpret_cast<const qk_double2*>(x)[seg * 64 + lane];
+                    v[u] = p2.x + p2.y;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += v[u];
+        }
+    }
+    y[wave * 64 + lane] = acc;
+}
+
 // ------------------------------------------------------------------ block apply
 // Y = H X for up to 64 real columns of a row-major block (row stride ld).
 // One wave per tile, up to 16 waves per workgroup sharing the lo-rank table.
@@ -2321,6 +2366,20 @@ int qkl_fill_random(int kind, size_t nrows, uint64_t seed, uint64_t counter0, in
 {
     hipLaunchKernelGGL(k_fill_random, dim3(grid_for(nrows * (size_t)ncol)), dim3(QK_TPB), 0, (hipStream_t)sv, kind,
                        nrows, ncol, ld, c0, seed, counter0, x);
+    return rc(hipGetLastError());
+}
+
+int qkl_gather_probe(const double* x, size_t n, int bytes_per_lane, double* y, int blocks, int tiles, qkl_stream sv)
+{
+    // 8 waves per workgroup; y holds blocks * 512 doubles
+    const size_t seg_doubles = bytes_per_lane == 16 ? 128 : 64;
+    if ((bytes_per_lane != 8 && bytes_per_lane != 16) || n < seg_doubles || blocks <= 0 || tiles <= 0 || !x || !y)
+        return -1;
+    const size_t nseg = n / seg_doubles;
+    if (bytes_per_lane == 8)
+        hipLaunchKernelGGL(k_gather_probe<8>, dim3(blocks), dim3(512), 0, (hipStream_t)sv, x, nseg, y, (uint32_t)tiles);
+    else
+        hipLaunchKernelGGL(k_gather_probe<16>, dim3(blocks), dim3(512), 0, (hipStream_t)sv, x, nseg, y, (uint32_t)tiles);
     return rc(hipGetLastError());
 }
 

@@ -123,6 +123,10 @@ int qkl_set_int(int* p, int v, qkl_stream s);
  * arguments, so the call never waits for a host-to-device copy */
 int qkl_set_vals(double* dst, int n, const double* vals, qkl_stream s);
 int qkl_copy(size_t n_doubles, const double* src, double* dst, qkl_stream s);
+/* gather-rate probe: blocks x 8 waves, each `tiles` rounds of 16 coalesced
+ * gathers of random 64-lane segments (8 or 16 bytes per lane) of x[n];
+ * y holds blocks * 512 doubles */
+int qkl_gather_probe(const double* x, size_t n, int bytes_per_lane, double* y, int blocks, int tiles, qkl_stream s);
 /* dst[r * dst_ld + j] = src[r * src_ld + j], r < nrows, j < width <= both ld */
 int qkl_copy_strided(size_t nrows, int width, const double* src, size_t src_ld, double* dst, size_t dst_ld,
                      qkl_stream s);
