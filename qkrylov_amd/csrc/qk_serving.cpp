@@ -23,7 +23,9 @@
 // (in chunks of steps, taken as the run goes on from a store reserved beside
 // the budget), and the Ritz vectors are then summed from them; when the
 // store cannot hold them, the Ritz phase recomputes them instead. Both give
-// the same bits. Each phase enqueues a few milliseconds of steps per poll.
+// the same bits. Each phase enqueues a few milliseconds of steps per poll,
+// a small request's phase often in one poll; the polls of the largest
+// requests go onto the device at most two at a time, oldest first.
 // Nothing in the loop waits for the device: engines are built per sector
 // once per run with plain uploads, the store is carved on the host, the
 // per-column scalars are set by kernels, and every free is stream-ordered or
@@ -218,9 +220,11 @@ size_t footprint_of(const qkrylov_serve_request_t* const* rq, int n)
     return vecs * D * sizeof(double) + (size_t)(kmax + 8) * 16 * nv + (2 << 20);
 }
 
-constexpr int kChunk = 12;   // recurrence steps per stored-vector chunk, and at most per poll
+constexpr int kChunk = 12;   // recurrence steps per stored-vector chunk
 
 enum Phase { kGS, kRitz, kCF, kDone };
+// the batch's next enqueue, held back while the window of large polls is full
+enum Next { kNone, kGsChunk, kStartRitz, kRitzChunk, kStartCF, kCFChunk };
 
 struct Column {
     int id = 0;
@@ -252,6 +256,8 @@ struct Batch {
     std::vector<std::pair<size_t, size_t>> chunks, retired;
     StoreArena* arena = nullptr;
     Phase phase = kGS;
+    Next next = kNone;
+    bool large = false;              // the poll in flight counts against the window of large polls
     int k = 0, kend = 0;             // steps enqueued; steps the phase runs at most
     int rn = 0;                      // Ritz phase without stored vectors: terms of the sum
     bool inflight = false;
@@ -293,17 +299,20 @@ void give_back(Batch& b)
     b.retired.clear();
 }
 
-// Recurrence steps one poll enqueues: about 4 ms of device work, at most
-// kChunk. The stream slots share a few hardware queues that run their
-// packets in order, so a long run of one batch's steps holds back every
-// batch queued behind it; polls of similar length keep the slots sharing
-// the device evenly. The step cost is modelled from the ring's column apply
-// (about 1.6e-8 ms per row and column on an MI355X).
+// Recurrence steps one poll enqueues: about 4 ms of device work. The stream
+// slots share a few hardware queues that run their packets in order, so a
+// long run of one batch's steps holds back every batch queued behind it;
+// polls of similar length keep the slots sharing the device evenly. Each
+// poll of a batch in turn waits behind what the others have queued, so a
+// small request is served in as few polls as its steps fit: alone an L = 22
+// request takes 7 ms, and at 12 steps per poll it took 0.39 s in the
+// Poisson stream. The step cost is modelled from the ring's column apply
+// (about 1.6e-8 ms per row and column on an MI355X, and at least 15 us for
+// the step's two launches).
 int steps_per_poll(const Batch& b, int nv)
 {
-    const double ms = 1.6e-8 * (double)b.basis->dim() * (double)nv;
-    if (ms <= 0.0) return kChunk;
-    return std::max(1, std::min(kChunk, (int)(4.0 / ms)));
+    const double ms = std::max(0.015, 1.6e-8 * (double)b.basis->dim() * (double)nv);
+    return std::max(1, (int)(4.0 / ms));
 }
 
 double* stored_step(Batch& b, int k)
@@ -314,17 +323,20 @@ double* stored_step(Batch& b, int k)
 
 void enqueue_gs_chunk(Batch& b)
 {
-    // never past the stored-vector chunk the first step of this poll is in
-    const int kend = std::min({b.kend, b.k + steps_per_poll(b, b.C->nv), (b.k / kChunk + 1) * kChunk});
+    // the first poll runs up to 64 steps (requests of the stream converge
+    // after 50-70), later ones up to 16; steps past a column's convergence
+    // are run and not used
+    const int want = std::max(16, 64 - b.k);
+    const int kend = std::min(b.kend, b.k + std::min(want, steps_per_poll(b, b.C->nv)));
     const size_t D = (size_t)b.basis->dim();
     const int ncf = (int)b.cf.size(), w = ncf ? cols_width(ncf) : 0;
-    if (b.store && (size_t)b.k / kChunk >= b.chunks.size()) {
-        const size_t bytes = (size_t)kChunk * D * w * sizeof(double);
-        size_t off = 0;
-        if (b.arena && b.arena->take(bytes, off)) b.chunks.emplace_back(off, bytes);
-        else drop_store(b);
-    }
     for (; b.k < kend; ++b.k) {
+        if (b.store && (size_t)b.k / kChunk >= b.chunks.size()) {
+            const size_t bytes = (size_t)kChunk * D * w * sizeof(double);
+            size_t off = 0;
+            if (b.arena && b.arena->take(bytes, off)) b.chunks.emplace_back(off, bytes);
+            else drop_store(b);
+        }
         // q_k of the S^zz columns (the block's first ncf columns) before step k
         if (b.store && qkl_copy_strided(D, ncf, b.C->cur(), b.C->nv, stored_step(b, b.k), w, b.s) != 0)
             throw std::runtime_error("store Krylov vector");
@@ -399,7 +411,7 @@ void start_batch(Batch& b, EngineCache& engines, StoreArena& arena)
     b.phase = kGS;
     b.k = 0;
     b.store = !b.cf.empty();
-    enqueue_gs_chunk(b);
+    b.next = kGsChunk;
 }
 
 void start_cf(Batch& b);
@@ -537,23 +549,22 @@ std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a
             }
         }
         if (!all) {
-            enqueue_gs_chunk(b);
+            b.next = kGsChunk;
             return finished;
         }
         if (b.cf.empty()) {
             b.phase = kDone;
             return finished;
         }
-        start_ritz(b);
+        b.next = kStartRitz;
         return finished;
     }
     if (b.phase == kRitz) {
-        if (b.k < b.rn) enqueue_ritz_chunk(b);
-        else start_cf(b);
+        b.next = b.k < b.rn ? kRitzChunk : kStartCF;
         return finished;
     }
     if (b.phase == kCF && b.k < b.kend) {
-        enqueue_cf_chunk(b);
+        b.next = kCFChunk;
         return finished;
     }
     if (b.phase == kCF) {
@@ -590,6 +601,35 @@ std::vector<int> advance(Batch& b, qkrylov_serve_result_t* results, double* cf_a
         b.phase = kDone;
     }
     return finished;
+}
+
+// enqueues the batch's next poll (each ends with the batch's event)
+void launch(Batch& b)
+{
+    const Next n = b.next;
+    b.next = kNone;
+    switch (n) {
+    case kGsChunk: enqueue_gs_chunk(b); break;
+    case kStartRitz: start_ritz(b); break;
+    case kRitzChunk: enqueue_ritz_chunk(b); break;
+    case kStartCF: start_cf(b); break;
+    case kCFChunk: enqueue_cf_chunk(b); break;
+    case kNone: break;
+    }
+}
+
+// Large polls, those of batches whose step costs at least 0.1 ms (rings of
+// L = 26-28), are let onto the device at most two at a time, oldest batch
+// first; smaller polls always go. Left to share the device evenly, a large
+// request is slowed by everything that overlaps it (L = 28 requests ran
+// 2.4-3.5x their time alone in the Poisson stream at about two-thirds
+// load); two large polls keep the device busy, so the oldest large
+// requests run close to their time alone and the small ones still pass.
+constexpr int kLargeWindow = 2;
+
+bool large_poll(const Batch& b)
+{
+    return 1.6e-8 * (double)b.basis->dim() * (double)b.C->nv >= 0.1;
 }
 
 // pinned read-back room of one request's column (alpha and norm arrays, the psi norms)
@@ -743,7 +783,7 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
         for (int i = max_active - 1; i >= 0; --i) free_slots.push_back(i);
         std::vector<int> slot_of;
         size_t used = 0;
-        int done = 0;
+        int done = 0, large_inflight = 0;
         const double t0 = now_s();
         auto finish = [&](int id) {
             auto& r = results[id];
@@ -815,6 +855,7 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                 active.push_back(std::move(b));
                 progress = true;
             }
+            // batches in admission order, so the oldest take the large-poll window first
             for (size_t a = 0; a < active.size();) {
                 Batch& b = *active[a];
                 if (b.inflight) {
@@ -825,11 +866,26 @@ extern "C" int qkrylov_serve_run(const qkrylov_serve_request_t* reqs, int nreq, 
                     }
                     hip_check(q, "event query");
                     b.inflight = false;
+                    if (b.large) {
+                        b.large = false;
+                        --large_inflight;
+                    }
                     progress = true;
                     give_back(b);
                     for (int id : advance(b, results, cf_a, cf_b, cf_cap)) finish(id);
                 }
-                if (b.phase == kDone && !b.inflight) {
+                if (b.next != kNone) {
+                    const bool large = large_poll(b);
+                    if (!large || large_inflight < kLargeWindow) {
+                        launch(b);
+                        if (large) {
+                            b.large = true;
+                            ++large_inflight;
+                        }
+                        progress = true;
+                    }
+                }
+                if (b.phase == kDone && !b.inflight && b.next == kNone) {
                     used -= b.footprint;
                     drop_store(b);
                     give_back(b);
