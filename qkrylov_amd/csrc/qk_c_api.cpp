@@ -52,6 +52,11 @@ int fail_unknown()
     g_last_error = "unknown exception";
     return QKRYLOV_ERROR_EXCEPTION;
 }
+int invalid(const char* what)
+{
+    g_last_error = what;
+    return QKRYLOV_ERROR_INVALID_ARG;
+}
 
 #define QK_TRY try {
 #define QK_CATCH                                     \
@@ -953,11 +958,11 @@ int qkrylov_lanczos_run_f64(qkrylov_hamiltonian_h h, const qkrylov_lanczos_optio
     o.want_vector = (opt->want_vector != 0) && vector != nullptr;
     o.seed = opt->seed;
     o.check_every = opt->check_every;
-    if (opt->graph_mode < 0 || opt->graph_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    if (opt->graph_mode < 0 || opt->graph_mode > 2) return invalid("lanczos: graph_mode must be 0, 1 or 2");
     o.graph = opt->graph_mode == 0 ? -1 : (opt->graph_mode == 1 ? 1 : 0);
-    if (opt->split_mode < 0 || opt->split_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    if (opt->split_mode < 0 || opt->split_mode > 2) return invalid("lanczos: split_mode must be 0, 1 or 2");
     o.split = opt->split_mode == 0 ? -1 : (opt->split_mode == 1 ? 1 : 0);
-    if (opt->start_mode < -1 || opt->start_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    if (opt->start_mode < -1 || opt->start_mode > 2) return invalid("lanczos: start_mode must be -1, 0, 1 or 2");
     o.start_mode = opt->start_mode < 0 ? qkrylov::default_start_mode() : opt->start_mode;
     o.start = start;
     auto r = qk::lanczos_run(*H->engine(), o, qk::default_stream());
@@ -983,7 +988,7 @@ int qkrylov_lanczos_run_noalloc_f64(qkrylov_hamiltonian_h h, const qkrylov_lancz
 {
     const auto* H = H_of(h);
     if (!H || !opt || !rep) return QKRYLOV_ERROR_INVALID_ARG;
-    if (opt->graph_mode < 0 || opt->graph_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    if (opt->graph_mode < 0 || opt->graph_mode > 2) return invalid("lanczos: graph_mode must be 0, 1 or 2");
     QK_TRY
     if (!H->is_real()) throw std::invalid_argument("the no-allocation Lanczos driver needs a real Hamiltonian");
     qk::LanczosOptions o;
@@ -995,7 +1000,7 @@ int qkrylov_lanczos_run_noalloc_f64(qkrylov_hamiltonian_h h, const qkrylov_lancz
     o.seed = opt->seed;
     o.check_every = opt->check_every;
     o.graph = 0;   // graph instantiation may allocate
-    if (opt->start_mode < -1 || opt->start_mode > 2) return QKRYLOV_ERROR_INVALID_ARG;
+    if (opt->start_mode < -1 || opt->start_mode > 2) return invalid("lanczos: start_mode must be -1, 0, 1 or 2");
     o.start_mode = opt->start_mode < 0 ? qkrylov::default_start_mode() : opt->start_mode;
     if (o.start_mode != 0)
         throw std::invalid_argument("the no-allocation Lanczos driver runs real vectors only (start_mode 0)");

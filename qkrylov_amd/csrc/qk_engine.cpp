@@ -590,8 +590,9 @@ RealCore::RealCore(const Engine& e, int kcap_, hipStream_t st, bool reorth, bool
     auto mk = [&](size_t bytes) { return ar ? ar->take(bytes) : (so ? DevBuf(bytes, st) : DevBuf(bytes)); };
     for (auto& q : Q) q = mk(D * sizeof(double));
     if (split) tmp = mk(D * sizeof(double));
-    alpha = mk((kcap + 2) * sizeof(double));
-    nrm = mk((kcap + 3) * sizeof(double));
+    // alpha[kcap + 2] and nrm[kcap + 3] in one allocation: one copy reads both back
+    alpha = mk((size_t)(2 * kcap + 5) * sizeof(double));
+    nrm = DevBuf::view(alpha.d() + kcap + 2, (size_t)(kcap + 3) * sizeof(double));
     npA = split ? qkl_apply_partials_acc(split->tables2()) : qkl_apply_partials(E.tables(), E.wide() ? 1 : 0);
     if (npA <= 0) throw std::runtime_error("apply: the operator tables do not fit in LDS");
     NB = qkl_blas_partials();
@@ -696,7 +697,7 @@ void RealCore::steps_graph(int k, int n)
             int* kn = (i % 3 == 2) ? kd + (1 - i / 3) : nullptr;
             rc |= lanczos_apply(0, kc, i % 3);
             rc |= qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), 0, kc, i % 3, partA.d(), npA, partB.d(), kn,
-                                      s);
+                                      host_mirror, s);
             const int t = ip;
             ip = ic;
             ic = iw;
@@ -725,12 +726,16 @@ void RealCore::init_norm()
     lcheck(qkl_fill_zero(D, prv(), s), "zero");
     lcheck(qkl_dot_partials(0, D, cur(), cur(), partB.d(), s), "dot");
     lcheck(qkl_total(partB.d(), NB, 1, nrm.d(), 0, s), "total");
+    if (host_mirror) lcheck(qkl_total(partB.d(), NB, 1, host_mirror + kcap + 2, 0, s), "total");
     pending = -1;
 }
 
 void RealCore::flush()
 {
-    if (pending >= 0) lcheck(qkl_total(partB.d(), NB, 1, nrm.d(), pending, s), "total");
+    if (pending >= 0) {
+        lcheck(qkl_total(partB.d(), NB, 1, nrm.d(), pending, s), "total");
+        if (host_mirror) lcheck(qkl_total(partB.d(), NB, 1, host_mirror + kcap + 2, pending, s), "total");
+    }
     pending = -1;
 }
 
@@ -743,10 +748,10 @@ int RealCore::lanczos_apply(int k, const int* kdev, int koff)
         const int rc = qkl_apply(split->tables1(), 0, 0, cur(), tmp.d(), s);
         if (rc != 0) return rc;
         return qkl_lanczos_apply_acc(split->tables2(), cur(), wrk(), k, kdev, koff, prv(), nrm.d(), partB.d(), NB,
-                                     partA.d(), npA, tmp.d(), s);
+                                     partA.d(), npA, tmp.d(), host_mirror ? host_mirror + kcap + 2 : nullptr, s);
     }
     return qkl_lanczos_apply(E.tables(), E.wide() ? 1 : 0, cur(), wrk(), k, kdev, koff, prv(), nrm.d(), partB.d(), NB,
-                             partA.d(), npA, s);
+                             partA.d(), npA, host_mirror ? host_mirror + kcap + 2 : nullptr, s);
 }
 
 void RealCore::step(int k, const double* basis)
@@ -756,7 +761,8 @@ void RealCore::step(int k, const double* basis)
     if (!split && qkl_apply_partials(E.tables(), E.wide() ? 1 : 0) != npA)
         throw std::logic_error("apply launch shape changed during a run");
     lcheck(lanczos_apply(k, nullptr, 0), "apply");
-    lcheck(qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), k, nullptr, 0, partA.d(), npA, partB.d(), nullptr, s),
+    lcheck(qkl_lanczos_update1(D, wrk(), cur(), alpha.d(), nrm.d(), k, nullptr, 0, partA.d(), npA, partB.d(), nullptr,
+                               host_mirror, s),
            "update");
     if (basis) {
         // classical Gram-Schmidt against v_0..v_k (stored normalised)
@@ -818,15 +824,16 @@ size_t lanczos_arena_bytes(const Engine& E, const LanczosOptions& o)
 }
 
 // Pinned host memory for the Lanczos driver's read-backs, kept per host
-// thread and only grown (never freed while the process runs: freeing pinned
-// memory synchronises the whole device)
+// thread and only grown, at the start of a run (freeing pinned memory
+// synchronises the whole device, so it is never freed while a run is on it)
 static double* pinned_readback(size_t n)
 {
     thread_local double* p = nullptr;
     thread_local size_t cap = 0;
     if (n > cap) {
         double* q = nullptr;
-        hip_check(hipHostMalloc((void**)&q, n * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        // coherent: kernels store into it directly
+        hip_check(hipHostMalloc((void**)&q, n * sizeof(double), hipHostMallocCoherent), "hipHostMalloc");
         if (p) (void)hipHostFree(p);
         p = q;
         cap = n;
@@ -884,8 +891,28 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
     DevBuf basis;
     if (store) basis = mk(basis_bytes);
     const size_t core_mark = arena ? arena->mark() : 0;
+    // launch-bound sizes running a fixed number of steps replay a captured
+    // six-step graph; runs to a tolerance launch eagerly, ahead of the
+    // host's checks (below), which keeps the GPU as busy without the
+    // capture's fixed cost
+    const bool graph = !arena && !store && !reorth && kmax >= 3 &&
+                       (o.graph > 0 || (o.graph < 0 && o.fixed_steps && D < (1 << 22)));
+    // host convergence checks: every 4 steps up to 2^25 rows, every step
+    // beyond; the replay fixes the iteration count whatever the cadence.
+    // A check costs an event (about 3 us, a fifth of a step of the 20-site
+    // ring) and a sparse one runs more steps past convergence: that ring to
+    // 1e-12 took 1.12, 1.04, 1.04, 1.02 and 1.10 ms checking every 1, 2, 3,
+    // 4 and 6 steps
+    int check = o.check_every > 0 ? o.check_every : (D < (1 << 25) ? (graph ? 12 : 4) : 1);
+    if (o.fixed_steps) check = kmax;
+    // runs to a tolerance check while the device runs ahead; the kernels
+    // that total alpha and the norms store them into pinned host memory as
+    // well, so a check reads them after an event, with no copy
+    const bool ahead = !(arena || o.fixed_steps || kmax <= check);
+    double* hb = ahead ? pinned_readback(2 * (size_t)kmax + 6) : nullptr;
     auto Cp = std::make_unique<RealCore>(E, kmax, s, reorth, false, arena, sp);
     RealCore& C = *Cp;
+    C.host_mirror = hb;
     out.split = sp != nullptr;
 
     // dst: the recurrence's start; stage: a scratch vector for the permutation
@@ -901,24 +928,13 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
     if (store) lcheck(qkl_copy_scaled((size_t)D, C.cur(), basis.d(), C.nrm.d(), 0, s), "store v0");
 
     LanczosReplay rp{kmax, o.tol, o.fixed_steps, {}, {}};
-    // launch-bound sizes running a fixed number of steps replay a captured
-    // six-step graph; runs to a tolerance launch eagerly, one check ahead of
-    // the host (below), which keeps the GPU as busy without the capture's
-    // fixed cost (the 20-site ring to 1e-12: 1.12 ms eager against 1.35 ms
-    // with graphs, both one check ahead)
-    const bool graph = !arena && !store && !reorth && kmax >= 3 &&
-                       (o.graph > 0 || (o.graph < 0 && o.fixed_steps && D < (1 << 22)));
-    // host convergence checks: every 6-12 steps while launches dominate,
-    // every 4 up to 2^25 rows (a read-back costs a tenth of a step there),
-    // every step beyond; the replay fixes the iteration count either way
-    int check = o.check_every > 0 ? o.check_every : (D < (1 << 22) ? (graph ? 12 : 6) : (D < (1 << 25) ? 4 : 1));
-    if (o.fixed_steps) check = kmax;
     const double t0 = now_s();
     int k = 0;
     std::vector<double> hnorm(kmax + 3);   // the norms read back so far (the Ritz pass divides by them)
-    // enqueue the steps up to the next check and the read-back of alpha and
-    // the norms so far into (ha, hn)
-    auto enqueue_chunk = [&](double* ha, double* hn) {
+    // enqueue the steps up to the next check and one read-back of alpha and
+    // the norms so far into hbuf (alpha at 0, the norms at kmax + 2, as on
+    // the device); returns the steps whose alpha and following norm it holds
+    auto enqueue_chunk = [&](double* hbuf) {
         const int kend = std::min(kmax, k + check);
         if (graph && kend - k >= 3) {
             const int n3 = (kend - k) / 3 * 3;
@@ -933,28 +949,39 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
                        "store v");
             }
         }
-        C.flush();
-        hip_check(hipMemcpyAsync(ha, C.alpha.d(), k * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
-        hip_check(hipMemcpyAsync(hn, C.nrm.d(), (k + 1) * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        // the norm of the newest vector is totalled by the next chunk's
+        // first apply, which the replay waits for; only the last chunk
+        // totals it here (a total and a second copy per check cost as much
+        // as a step of the 20-site ring)
+        if (k >= kmax) C.flush();
+        if (!C.host_mirror)
+            hip_check(hipMemcpyAsync(hbuf, C.alpha.d(), (size_t)(kmax + 2 + k + 1) * sizeof(double),
+                                     hipMemcpyDeviceToHost, s),
+                      "d2h");
+        return C.pending >= 0 ? C.pending - 1 : k;
     };
-    if (arena || o.fixed_steps || kmax <= check) {
+    if (!ahead) {
         // one check at a time (no pinned read-back buffers: the workspace
         // entry point allocates nothing)
-        std::vector<double> ha(kmax + 2), hn(kmax + 3);
+        std::vector<double> hbuf(2 * (size_t)kmax + 6);
         while (!rp.done) {
-            enqueue_chunk(ha.data(), hn.data());
+            const int avail = enqueue_chunk(hbuf.data());
             hip_check(hipStreamSynchronize(s), "sync");
-            std::copy(hn.begin(), hn.begin() + k + 1, hnorm.begin());
-            while (!rp.done && rp.next < k) rp.feed(ha[rp.next], hn[rp.next + 1]);
+            const double* ha = hbuf.data();
+            const double* hn = ha + kmax + 2;
+            std::copy(hn, hn + avail + 1, hnorm.begin());
+            while (!rp.done && rp.next < avail) rp.feed(ha[rp.next], hn[rp.next + 1]);
             if (k >= kmax) rp.done = true;
         }
     } else {
         // the device runs one check ahead: while the host replays the
         // convergence test on chunk i, chunk i + 1 is already queued, so the
-        // GPU does not idle through the read-back and the replay; steps past
-        // convergence are run and not used (steps_run counts them)
-        const size_t per = 2 * (size_t)kmax + 6;
-        double* hb = pinned_readback(2 * per);
+        // GPU does not idle through the replay; steps past convergence are
+        // run and not used (steps_run counts them). The kernels store alpha
+        // and the norms into hb as they total them; the host reads only
+        // entries that its event orders before it.
+        const double* ha = hb;
+        const double* hn = ha + kmax + 2;
         hipEvent_t ev[2] = {nullptr, nullptr};
         struct Events {
             hipEvent_t* e;
@@ -965,9 +992,9 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
             }
         } ev_guard{ev};
         for (auto& e : ev) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-        int pending[2] = {-1, -1};
+        int pending[2] = {-1, -1}, avail[2] = {0, 0};
         auto enqueue_into = [&](int b) {
-            enqueue_chunk(hb + b * per, hb + b * per + kmax + 2);
+            avail[b] = enqueue_chunk(hb);
             hip_check(hipEventRecord(ev[b], s), "event");
             pending[b] = k;
         };
@@ -979,10 +1006,8 @@ LanczosOutput lanczos_run(const Engine& E, const LanczosOptions& o, hipStream_t 
             hip_check(hipEventSynchronize(ev[b]), "event sync");
             const int kb = pending[b];
             pending[b] = -1;
-            const double* ha = hb + b * per;
-            const double* hn = ha + kmax + 2;
-            std::copy(hn, hn + kb + 1, hnorm.begin());
-            while (!rp.done && rp.next < kb) rp.feed(ha[rp.next], hn[rp.next + 1]);
+            std::copy(hn, hn + avail[b] + 1, hnorm.begin());
+            while (!rp.done && rp.next < avail[b]) rp.feed(ha[rp.next], hn[rp.next + 1]);
             if (kb >= kmax) rp.done = true;
             if (rp.done) break;
             if (k < kmax) enqueue_into(b);

@@ -188,6 +188,10 @@ struct RealCore {
     hipGraphExec_t gexec = nullptr;
     int cap_ic = -1;   // rotation state the graph was captured in
     int pending = -1;  // nrm index whose partial sums are not yet totalled
+    // set: coherent pinned host memory laid out as alpha and nrm on the
+    // device (alpha at 0, nrm at kcap + 2), into which the kernels that
+    // total them store them too
+    double* host_mirror = nullptr;
 
     Arena* arena = nullptr;   // set: every device buffer comes from it
     bool stream_ordered = false;   // buffers from hipMallocAsync on s

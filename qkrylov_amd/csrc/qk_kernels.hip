@@ -394,7 +394,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
                                                    double* __restrict__ partials, int table_bytes,
                                                    const int* __restrict__ kdev, int koff,
                                                    const double* __restrict__ nrm_part, int n_nrm_part,
-                                                   const double* __restrict__ acc)
+                                                   const double* __restrict__ acc, double* __restrict__ nrm_host)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     // static LDS: QK_APPLY_STATIC_LDS bytes, counted by apply_lds_plan
@@ -432,7 +432,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((LZ && !CP
     double inv_n = 1.0, cprev = 0.0;
     if (LZ) {
         const double nk = sqrt(block_total(nrm_part, n_nrm_part, &tot_sh));
-        if (blockIdx.x == 0 && threadIdx.x == 0) nrm[lzk] = nk;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            nrm[lzk] = nk;
+            if (nrm_host) nrm_host[lzk] = nk;   // the host's copy (coherent pinned memory)
+        }
         inv_n = 1.0 / nk;
         if (lzk > 0) cprev = nk / nrm[lzk - 1];
     }
@@ -932,7 +935,7 @@ __global__ __launch_bounds__(QK_TPB) void k_update1(size_t n, double* __restrict
                                                      int k, double* __restrict__ partials,
                                                      const int* __restrict__ kdev, int koff,
                                                      const double* __restrict__ a_part, int n_a_part,
-                                                     int* __restrict__ knext)
+                                                     int* __restrict__ knext, double* __restrict__ alpha_host)
 {
     __shared__ double wsum[QK_WPB];
     __shared__ double tot_sh;
@@ -944,6 +947,7 @@ __global__ __launch_bounds__(QK_TPB) void k_update1(size_t n, double* __restrict
         a = block_total(a_part, n_a_part, &tot_sh);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             alpha[kk] = a;
+            if (alpha_host) alpha_host[kk] = a;   // the host's copy (coherent pinned memory)
             // a graph's last step of a turn advances the other step counter
             // (one this launch does not read) to the next turn
             if (knext) *knext = kk - koff + 3;
@@ -1972,6 +1976,7 @@ struct LzArgs {
     const double* nrm_part = nullptr;
     int n_nrm_part = 0;
     int n_part = 0;           // partial sums the caller holds room for
+    double* nrm_host = nullptr;   // also store nrm[k] here (host memory), when set
 };
 
 template <typename ST, bool CPLX, int RM, bool LZ, bool SIGNS, bool ACC = false>
@@ -1990,7 +1995,7 @@ int launch_apply(const QkTables* T, const double* x, double* y, int k, const dou
     hipLaunchKernelGGL((k_apply<ST, CPLX, RM, LZ, SIGNS, ACC>), dim3(c.nb), dim3(64 * c.wpb), c.lds, s, T->tiles,
                        T->hoff, T->lrank, T->tl, T->th, T->nth,
                        T->lolist, T->brk, T->dg, make_scal(T), x, y, k, prev, nrm, partials, (int)table_bytes(T),
-                       lz.kdev, lz.koff, lz.nrm_part, lz.n_nrm_part, acc);
+                       lz.kdev, lz.koff, lz.nrm_part, lz.n_nrm_part, acc, lz.nrm_host);
     return rc(hipGetLastError());
 }
 
@@ -2232,11 +2237,12 @@ int qkl_apply(const QkTables* T, int cplx, int wide, const double* x, double* y,
 // reductions folded in (see qk_launch.h).
 int qkl_lanczos_apply(const QkTables* T, int wide, const double* x, double* y, int k, const int* kdev, int koff,
                       const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
-                      int n_part, qkl_stream sv)
+                      int n_part, double* nrm_host, qkl_stream sv)
 {
     hipStream_t s = (hipStream_t)sv;
     if (T->ntiles == 0 || !nrm_part || n_nrm_part <= 0) return -1;
     LzArgs lz;
+    lz.nrm_host = nrm_host;
     lz.n_part = n_part;
     lz.kdev = kdev;
     lz.koff = koff;
@@ -2256,10 +2262,11 @@ int qkl_apply_acc(const QkTables* T, const double* x, double* y, const double* a
 
 int qkl_lanczos_apply_acc(const QkTables* T, const double* x, double* y, int k, const int* kdev, int koff,
                           const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
-                          int n_part, const double* acc, qkl_stream sv)
+                          int n_part, const double* acc, double* nrm_host, qkl_stream sv)
 {
     if (T->ntiles == 0 || !nrm_part || n_nrm_part <= 0 || !acc) return -1;
     LzArgs lz;
+    lz.nrm_host = nrm_host;
     lz.n_part = n_part;
     lz.kdev = kdev;
     lz.koff = koff;
@@ -2270,11 +2277,11 @@ int qkl_lanczos_apply_acc(const QkTables* T, const double* x, double* y, int k, 
 
 int qkl_lanczos_update1(size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
                         const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, int* knext,
-                        qkl_stream sv)
+                        double* alpha_host, qkl_stream sv)
 {
     if (!a_part || n_a_part <= 0) return -1;
     hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, (hipStream_t)sv, nrows, w, q, alpha,
-                       nrm, k, partials, kdev, koff, a_part, n_a_part, knext);
+                       nrm, k, partials, kdev, koff, a_part, n_a_part, knext, alpha_host);
     return rc(hipGetLastError());
 }
 
@@ -2309,10 +2316,12 @@ int qkl_lanczos_update(int cplx, size_t nrows, int ncol, int ld, double* w, cons
         double* aw = const_cast<double*>(alpha);
         if (cplx)
             hipLaunchKernelGGL(k_update1<true>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, aw, nrm, k,
-                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0, (int*)nullptr);
+                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0, (int*)nullptr,
+                               (double*)nullptr);
         else
             hipLaunchKernelGGL(k_update1<false>, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, w, q, aw, nrm, k,
-                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0, (int*)nullptr);
+                               partials, (const int*)nullptr, 0, (const double*)nullptr, 0, (int*)nullptr,
+                               (double*)nullptr);
     } else {
         if (cplx || ncol > 64) return -1;
         hipLaunchKernelGGL(k_update_block, dim3(QK_BLAS_BLOCKS), dim3(QK_TPB), 0, s, nrows, ncol, ld, w, q, alpha, nrm,

@@ -54,13 +54,16 @@ int qkl_block_partials(const qk::QkTables* T);
  * Totals are fixed-order (qkl_total computes the same bits). k is taken
  * from *kdev + koff when kdev is set (launches captured once in a graph and
  * replayed), else from the argument. n_part is the room in partials; it
- * must equal qkl_apply_partials(T, wide) or the launch is refused. */
+ * must equal qkl_apply_partials(T, wide) or the launch is refused. When
+ * nrm_host / alpha_host is set (coherent pinned host memory), workgroup 0
+ * also stores nrm[k] / alpha[k] there, so the host reads them after an
+ * event without a copy. */
 int qkl_lanczos_apply(const qk::QkTables* T, int wide, const double* x, double* y, int k, const int* kdev, int koff,
                       const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
-                      int n_part, qkl_stream s);
+                      int n_part, double* nrm_host, qkl_stream s);
 int qkl_lanczos_update1(size_t nrows, double* w, const double* q, double* alpha, const double* nrm, int k,
                         const int* kdev, int koff, const double* a_part, int n_a_part, double* partials, int* knext,
-                        qkl_stream s);
+                        double* alpha_host, qkl_stream s);
 /* out[idx] = total of partials[0..npart) (sqrt when root), in the order of
  * the two launches above */
 int qkl_total(const double* partials, int npart, int root, double* out, int idx, qkl_stream s);
@@ -160,7 +163,7 @@ int qkl_apply_diag_terms(const qk::QkTables* T, int wide, int xcplx, const doubl
 int qkl_apply_acc(const qk::QkTables* T, const double* x, double* y, const double* acc, qkl_stream s);
 int qkl_lanczos_apply_acc(const qk::QkTables* T, const double* x, double* y, int k, const int* kdev, int koff,
                           const double* prev, double* nrm, const double* nrm_part, int n_nrm_part, double* partials,
-                          int n_part, const double* acc, qkl_stream s);
+                          int n_part, const double* acc, double* nrm_host, qkl_stream s);
 int qkl_apply_partials_acc(const qk::QkTables* T);
 /* rows of the relabelled basis of Tint <-> rows of the public basis of Tpub
  * (32-bit states; bytemap: 4 x 256 public-bit tables of the internal bytes) */

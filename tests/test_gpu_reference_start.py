@@ -111,3 +111,11 @@ def test_reference_start_edge_cases():
     rep = _lib.LanczosReport()
     assert _lib.load().qkrylov_lanczos_run_noalloc_f64(H._h, ctypes.byref(o), None, None, None, None,
                                                        ctypes.byref(rep)) != 0
+    # out-of-range modes are refused with a message naming the field
+    for field, bad in (("start_mode", 3), ("graph_mode", -1), ("split_mode", 5)):
+        ob = _lib.LanczosOptions(maxiter=50, tol=1e-12, reorthogonalize=0, fixed_steps=0, want_vector=0, seed=1234,
+                                 check_every=0, graph_mode=0, split_mode=0, start_mode=0)
+        setattr(ob, field, bad)
+        assert _lib.load().qkrylov_lanczos_run_f64(H._h, ctypes.byref(ob), None, None, None, None,
+                                                   ctypes.byref(rep)) == -1
+        assert field in _lib.last_error()
