@@ -208,12 +208,20 @@ typedef struct {
  * counter RNG; 1 / 2: the reference's std::mt19937(1234) complex draws as
  * described for start_mode, so that a drop-in run follows the reference's
  * Krylov trajectory (iteration count, and the Ritz vector up to a sign).
- * Process-wide; returns -1 for other values. */
+ * qkrylov_ftlm always draws the reference's complex normal vectors
+ * (src/solvers/ftlm.cpp:133-143, std::mt19937(42)); mode 1 makes the first
+ * draw of an entry its imaginary part (GCC's argument order), any other
+ * mode its real part (Clang's). Process-wide; returns -1 for other values. */
 int qkrylov_amd_set_start_mode(int mode);
 int qkrylov_amd_get_start_mode(void);
 /* the reference's start vector itself: n complex entries interleaved into
  * out[2n] (host memory; no device needed), mode 1 or 2 as above */
 int qkrylov_amd_reference_start_vector(size_t n, uint64_t seed, int mode, double* out);
+/* the reference's FTLM draws (std::mt19937(seed), std::normal_distribution
+ * (0, 1), one stream over all random vectors): n complex entries
+ * interleaved into out[2n], in the order of mode 1 or 2 (host memory; no
+ * device needed). Replaces the draws of src/solvers/ftlm.cpp:133-143. */
+int qkrylov_amd_reference_normal_vector(size_t n, uint64_t seed, int mode, double* out);
 
 /* start: optional host start vector (real for real H, interleaved complex
  * otherwise); alphas/betas: host arrays of maxiter entries (optional);

@@ -9,7 +9,7 @@ from .api import (  # noqa: F401
     Op, OpSum, Site, SpinHalfBasis, SpinHalfSite, Sm, Sp, Sx, Sy, Sz, TJBasis, TJSite, continued_fraction_batched,
     continued_fraction_coeffs, davidson_lowest, evaluate_spectral_function, ftlm, ftlm_batched, lanczos_ground_state,
     OpProduct, OpTermSum, StructureFactorResult, structure_factor, tridiag_eigen, tridiag_lowest,
-    get_start_mode, reference_start_vector, set_start_mode,
+    get_start_mode, reference_normal_vector, reference_start_vector, set_start_mode,
 )
 from . import models  # noqa: F401
 from .storage import load_result, save_result  # noqa: F401
@@ -38,6 +38,6 @@ __all__ = [
     "continued_fraction_coeffs", "evaluate_spectral_function", "FTLMResult", "ftlm",
     "ftlm_batched", "continued_fraction_batched", "DeviceArray", "structure_factor", "StructureFactorResult",
     "OpProduct", "OpTermSum", "tridiag_eigen", "tridiag_lowest",
-    "set_start_mode", "get_start_mode", "reference_start_vector",
+    "set_start_mode", "get_start_mode", "reference_start_vector", "reference_normal_vector",
     "find_gpu", "gpu_count", "save_result", "load_result",
 ]

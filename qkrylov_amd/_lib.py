@@ -158,6 +158,7 @@ SIGNATURES = {
     "qkrylov_amd_set_start_mode": (c_int, [c_int]),
     "qkrylov_amd_get_start_mode": (c_int, []),
     "qkrylov_amd_reference_start_vector": (c_int, [c_size, c_u64, c_int, p_dbl]),
+    "qkrylov_amd_reference_normal_vector": (c_int, [c_size, c_u64, c_int, p_dbl]),
     "qkrylov_hamiltonian_create_ex": (c_vp, [c_vp, c_vp, c_vp, c_int]),
     "qkrylov_operator_create": (c_vp, [c_vp, c_vp, c_vp]),
     "qkrylov_hamiltonian_info": (c_int, [c_vp, P(HamInfo)]),

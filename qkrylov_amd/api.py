@@ -627,6 +627,20 @@ def reference_start_vector(n, seed=1234, mode="reference"):
     return out.view(np.complex128)
 
 
+def reference_normal_vector(n, seed=42, mode="reference"):
+    """The reference's FTLM draws (std::mt19937(seed) and
+    std::normal_distribution(0, 1), one stream): n complex entries
+    (complex128) in the argument order of start mode `mode`; ftlm() uses
+    the first n_random * dim of them for seed 42, vector by vector."""
+    code = _start_mode_code(mode)
+    if code == 0:
+        raise ValueError("reference_normal_vector: mode must be 'reference' or 'reference-clang'")
+    out = np.zeros(2 * int(n))
+    check(_lib.load().qkrylov_amd_reference_normal_vector(int(n), int(seed), code, _dp(out)),
+          "reference_normal_vector")
+    return out.view(np.complex128)
+
+
 def lanczos_ground_state(H, maxiter=200, tol=1e-12, reorthogonalize=True, return_state=True, seed=1234,
                          fixed_steps=False, start=None, check_every=0, graph=None, split=None, start_mode=None):
     """Ground state energy and, as in bindings/python/qkrylov/solvers.py:24-51,

@@ -289,9 +289,10 @@ FTLMResult ftlm(const MatrixFreeHamiltonianBase& H, Real beta, int n_random, int
     const size_t D = H.dimension();
     if (D == 0 || n_random <= 0) return res;
     // complex normal start vectors as in the reference (ftlm.cpp:129-144):
-    // std::mt19937(42), std::normal_distribution(0, 1), real part drawn first
-    std::mt19937 rng(42);
-    std::normal_distribution<Real> dist(0.0, 1.0);
+    // std::mt19937(42), std::normal_distribution(0, 1), one stream for all
+    // vectors; the first draw of an entry is its imaginary part under start
+    // mode 1 (GCC's argument order), its real part otherwise (Clang's)
+    qk::ReferenceNormal draws(42, default_start_mode());
     std::vector<std::vector<double>> al, be;
     std::vector<double> n2;
     hipStream_t s = qk::default_stream();
@@ -300,14 +301,14 @@ FTLMResult ftlm(const MatrixFreeHamiltonianBase& H, Real beta, int n_random, int
         // complex vectors as column pairs of one real block
         for (int r0 = 0; r0 < n_random; r0 += 32) {
             const int nv = std::min(32, n_random - r0), nc = 2 * nv;
-            std::vector<double> hb(D * (size_t)nc);
-            for (int v = 0; v < nv; ++v)
+            std::vector<double> hb(D * (size_t)nc), hv(2 * D);
+            for (int v = 0; v < nv; ++v) {
+                draws.fill(D, hv.data());
                 for (size_t i = 0; i < D; ++i) {
-                    const double re = dist(rng);
-                    const double im = dist(rng);
-                    hb[i * nc + 2 * v] = re;
-                    hb[i * nc + 2 * v + 1] = im;
+                    hb[i * nc + 2 * v] = hv[2 * i];
+                    hb[i * nc + 2 * v + 1] = hv[2 * i + 1];
                 }
+            }
             DevBuf X(hb.size() * sizeof(double));
             hip_check(hipMemcpyAsync(X.get(), hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
             auto bt = qk::block_tridiag_run(E, X.d(), nc, 2, n_steps, s);
@@ -320,10 +321,7 @@ FTLMResult ftlm(const MatrixFreeHamiltonianBase& H, Real beta, int n_random, int
     } else {
         std::vector<double> hv(2 * D);
         for (int r = 0; r < n_random; ++r) {
-            for (size_t i = 0; i < D; ++i) {
-                hv[2 * i] = dist(rng);
-                hv[2 * i + 1] = dist(rng);
-            }
+            draws.fill(D, hv.data());
             auto t = qk::tridiag_run(E, hv.data(), true, n_steps, s);
             al.push_back(t.alphas);
             be.push_back(t.betas);

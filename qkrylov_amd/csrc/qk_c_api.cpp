@@ -592,6 +592,18 @@ int qkrylov_amd_reference_start_vector(size_t n, uint64_t seed, int mode, double
     QK_CATCH
 }
 
+int qkrylov_amd_reference_normal_vector(size_t n, uint64_t seed, int mode, double* out)
+{
+    if ((n > 0 && !out) || (mode != 1 && mode != 2)) {
+        g_last_error = "reference normal vector: mode must be 1 or 2 and out non-null";
+        return QKRYLOV_ERROR_INVALID_ARG;
+    }
+    QK_TRY
+    qk::ReferenceNormal(seed, mode).fill(n, out);
+    return QKRYLOV_SUCCESS;
+    QK_CATCH
+}
+
 int qkrylov_amd_set_device(int id)
 {
     QK_TRY

@@ -798,6 +798,21 @@ void reference_start_vector(size_t D, uint64_t seed, int start_mode, double* out
     }
 }
 
+ReferenceNormal::ReferenceNormal(uint64_t seed, int start_mode)
+    : rng_(static_cast<std::mt19937::result_type>(seed)), dist_(0.0, 1.0), first_(start_mode == 1 ? 1 : 0)
+{
+}
+
+void ReferenceNormal::fill(size_t n, double* out)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const double a = dist_(rng_);
+        const double b = dist_(rng_);
+        out[2 * i + first_] = a;
+        out[2 * i + (1 - first_)] = b;
+    }
+}
+
 size_t lanczos_arena_bytes(const Engine& E, const LanczosOptions& o)
 {
     const size_t D = (size_t)E.dim();

@@ -8,6 +8,7 @@
 
 #include <memory>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -291,6 +292,23 @@ struct LanczosOptions {
 // the reference's start vector of lanczos_ground_state (start_mode 1 or 2
 // above), D complex entries interleaved into out[2 D]
 void reference_start_vector(size_t D, uint64_t seed, int start_mode, double* out);
+
+// The reference's FTLM draws (src/solvers/ftlm.cpp:133-143): one
+// std::mt19937(seed) and one std::normal_distribution(0, 1) for all random
+// vectors, entry i of a vector = KComplex(dist(rng), dist(rng)). The
+// distribution keeps state between calls (the polar method's second
+// deviate), so the draws are one stream: fill() continues it. start_mode 1
+// stores the first draw of an entry as its imaginary part (GCC's order of
+// the two arguments), anything else as its real part (Clang's).
+class ReferenceNormal {
+public:
+    ReferenceNormal(uint64_t seed, int start_mode);
+    void fill(size_t n, double* out);   // n complex entries, interleaved
+private:
+    std::mt19937 rng_;
+    std::normal_distribution<double> dist_;
+    size_t first_;
+};
 
 struct LanczosOutput {
     double energy = 0.0;

@@ -5,7 +5,8 @@ lanczos.cpp:149-151) started from the same vector: the whole alpha/beta
 trajectory, the iteration count and convergence flag, the energy, and the
 Ritz vector (equal up to the sign the tridiagonal eigensolver gives its
 eigenvector). The start vector itself is pinned on the CPU
-(test_start_vector_cpu.py)."""
+(test_start_vector_cpu.py). Likewise ftlm() from the reference's own complex
+normal draws against the oracle's FTLM from the same vectors."""
 import ctypes
 
 import numpy as np
@@ -119,3 +120,29 @@ def test_reference_start_edge_cases():
         assert _lib.load().qkrylov_lanczos_run_f64(H._h, ctypes.byref(ob), None, None, None, None,
                                                    ctypes.byref(rep)) == -1
         assert field in _lib.last_error()
+
+
+@pytest.mark.parametrize("mode", ["reference", "reference-clang", "counter"])
+@pytest.mark.parametrize("name", ["heis10", "complex10"])
+def test_ftlm_from_the_reference_draws_matches_the_oracle(name, mode):
+    """ftlm() draws the reference's complex normal vectors (std::mt19937(42),
+    one normal_distribution stream, src/solvers/ftlm.cpp:133-143) in the
+    order the start mode names (GCC's for "reference", Clang's otherwise);
+    the oracle's FTLM (ftlm.cpp:122-170) from those same vectors gives the
+    same Z, E and C_v."""
+    H, om = heis(10) if name == "heis10" else complex_spin(10)
+    n_random, M = 4, 15
+    order = "reference" if mode == "reference" else "reference-clang"
+    rv = qa.reference_normal_vector(n_random * om.dim, 42, order).reshape(n_random, om.dim)
+    old = qa.get_start_mode()
+    try:
+        qa.set_start_mode(mode)
+        got = {beta: qa.ftlm(H, beta, n_random, M) for beta in (0.3, 1.0)}
+    finally:
+        qa.set_start_mode(old)
+    for beta, r in got.items():
+        Z, E, Cv = om.ftlm(beta, rv, M)
+        print(f"{name} {mode} beta={beta}: Z {r.partition_function:.12g} (oracle {Z:.12g})", flush=True)
+        assert abs(r.partition_function - Z) < 1e-8 * abs(Z)
+        assert abs(r.internal_energy - E) < 1e-8 * max(1.0, abs(E))
+        assert abs(r.specific_heat - Cv) < 1e-8 * max(1.0, abs(Cv))

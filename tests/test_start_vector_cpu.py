@@ -1,6 +1,7 @@
 """The reference's Lanczos start vector (src/solvers/lanczos.cpp:115-119:
 std::mt19937 rng(1234), std::uniform_real_distribution<Real> dist(-1, 1),
-v(i) = KComplex(dist(rng), dist(rng))), as the library draws it on the host.
+v(i) = KComplex(dist(rng), dist(rng))), as the library draws it on the host;
+and, at the end, the reference's FTLM normal draws.
 
 Pinned three ways, none of which reuses the library's code:
 * numpy's legacy RandomState(seed) is MT19937 seeded by init_genrand, the
@@ -102,3 +103,65 @@ def test_start_mode_setting_round_trip_and_errors():
             qa.reference_start_vector(4, 1234, 0)
     finally:
         qa.set_start_mode(old)
+
+
+# The reference's FTLM draws (src/solvers/ftlm.cpp:133-143): std::mt19937(42),
+# std::normal_distribution(0, 1), one stream over all random vectors, entry
+# i = KComplex(dist(rng), dist(rng)). The polar method behind libstdc++'s
+# normal_distribution keeps its second deviate for the next call, so the
+# stream is checked whole, across many entries, against the same declarations
+# compiled by g++ and by clang++.
+NORMAL_PROBE = r"""
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+struct Cplx {
+    double re, im;
+    Cplx(double a, double b) : re(a), im(b) {}
+};
+int main(int argc, char** argv) {
+    const unsigned seed = (unsigned)std::strtoul(argv[1], nullptr, 10);
+    const int n = std::atoi(argv[2]);
+    std::mt19937 rng(seed);
+    std::normal_distribution<double> dist(0.0, 1.0);
+    for (int i = 0; i < n; ++i) {
+        Cplx z(dist(rng), dist(rng));
+        std::printf("%.17g %.17g\n", z.re, z.im);
+    }
+    return 0;
+}
+"""
+
+
+def _normal_stream(cxx, tmp_path, seed, n):
+    src = tmp_path / "normal_probe.cpp"
+    src.write_text(NORMAL_PROBE)
+    exe = tmp_path / ("normal_probe_" + os.path.basename(cxx))
+    subprocess.run([cxx, "-O2", str(src), "-o", str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe), str(seed), str(n)], check=True, capture_output=True, text=True).stdout.split()
+    v = np.array([float(t) for t in out])
+    return v[0::2] + 1j * v[1::2]
+
+
+@pytest.mark.parametrize("cxx,mode", [("g++", "reference"), ("/opt/rocm/llvm/bin/clang++", "reference-clang")])
+@pytest.mark.parametrize("seed", [42, 7])
+def test_ftlm_normal_draws_follow_each_compilers_stream(cxx, mode, seed, tmp_path):
+    path = shutil.which(cxx) or (cxx if os.path.exists(cxx) else None)
+    if path is None:
+        pytest.skip(f"{cxx} not available")
+    n = 3001    # odd: the stream ends with a deviate held back by the polar method
+    z = _normal_stream(path, tmp_path, seed, n)
+    assert np.array_equal(z, qa.reference_normal_vector(n, seed, mode))
+
+
+def test_ftlm_normal_draws_known_answer_and_order():
+    g = qa.reference_normal_vector(2, 42, "reference")
+    c = qa.reference_normal_vector(2, 42, "reference-clang")
+    # the first two deviates of the stream, and the two orders swap them
+    assert g[0] == complex(0.51543306969120128, -0.55023449442049355)
+    assert c[0] == complex(-0.55023449442049355, 0.51543306969120128)
+    assert g[1] == complex(c[1].imag, c[1].real)
+    # a shorter request is a prefix of a longer one (one stream)
+    assert np.array_equal(qa.reference_normal_vector(1000, 42)[:999], qa.reference_normal_vector(999, 42))
+    with pytest.raises(ValueError):
+        qa.reference_normal_vector(4, 42, "counter")
