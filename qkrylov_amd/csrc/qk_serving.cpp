@@ -309,10 +309,14 @@ void give_back(Batch& b)
 // Poisson stream. The step cost is modelled from the ring's column apply
 // (about 1.6e-8 ms per row and column on an MI355X, and at least 15 us for
 // the step's two launches).
+double step_ms(const Batch& b, int nv)
+{
+    return 1.6e-8 * (double)b.basis->dim() * (double)nv;
+}
+
 int steps_per_poll(const Batch& b, int nv)
 {
-    const double ms = std::max(0.015, 1.6e-8 * (double)b.basis->dim() * (double)nv);
-    return std::max(1, (int)(4.0 / ms));
+    return std::max(1, (int)(4.0 / std::max(0.015, step_ms(b, nv))));
 }
 
 double* stored_step(Batch& b, int k)
@@ -629,7 +633,7 @@ constexpr int kLargeWindow = 2;
 
 bool large_poll(const Batch& b)
 {
-    return 1.6e-8 * (double)b.basis->dim() * (double)b.C->nv >= 0.1;
+    return step_ms(b, b.C->nv) >= 0.1;
 }
 
 // pinned read-back room of one request's column (alpha and norm arrays, the psi norms)
