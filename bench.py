@@ -469,7 +469,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="j1j2_4x8", choices=sorted(WORKLOADS) + ["ftlm_tj4x5", "serve_xxz", "gather_probe"])
+    ap.add_argument("--workload", default="j1j2_4x8",
+                    choices=sorted(WORKLOADS) + ["ftlm_tj4x5", "serve_xxz", "gather_probe", "lanczos_c1"])
     ap.add_argument("--split-bits", type=int, default=-1,
                     help="low state bits ranked per lane (-1: the engine's cost model)")
     ap.add_argument("--ftlm-R", type=int, default=64)
@@ -530,6 +531,15 @@ def main():
         return bench_serve(args)
     if args.workload == "gather_probe":
         return bench_gather_probe(args)
+    if args.workload == "lanczos_c1":
+        o = lanczos_c1_leg()
+        p = o["plain"]
+        print(json.dumps({"metric": "Lanczos iters/s (FP64), run to tol 1e-12", "value": p["iters_per_s"],
+                          "unit": "it/s", "n_gpus": 1, "steps": p["steps_run"], "warmup": 1,
+                          "ms_per_step": round(1e3 * p["loop_seconds"] / p["steps_run"], 5), "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded start)",
+                          "config": {"workload": o["workload"], "D": o["D"]}, "lanczos_c1": o}), flush=True)
+        return
     w = WORKLOADS[args.workload]
     if args.copy_calib_gib > 0:
         n = int(args.copy_calib_gib * (1 << 30) / 8)
